@@ -1,0 +1,206 @@
+"""Benchmark of the PDHG projector on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3]
+
+Workload (default, "c2"): the mixed MILP with 50,000 rows × 100,000 columns
+(70 % binary, 30 % continuous in [0, 10]), 1,000,000 nonzeros, row lengths
+uniform in 10-30, values N(0,1), b = A x0 + |N(0,1)| for a random feasible
+x0 (seed 1).  A step is one warm-started ℓ1 projection toward a new random
+0/1 target at relative KKT tolerance 1e-4, the way the pump calls the
+projector (src/pump.jl:329-359).  The scaled matrix, its transpose and every
+iterate stay in HBM; the timed region covers set_target + pdfp_solve.
+
+value = Σ PDHG iterations / Σ projection wall time of the product form (FP32
+data and iterates, fused launches, FP64 reductions), the reference's own
+accounting (src/pump.jl:357-359).  The plain FP64 form (the reference's
+launch structure) runs the same projections and is reported beside it.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch over its mean duration, timed with HIP events on the library's
+stream) and the CPU baseline (the FP64 C oracle on one host core, on a
+bounded number of iterations of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PDHG iter/s + time-to-first-feasible, batched FP32 vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(name: str):
+    from pdfp_amd import generators as G
+
+    if name == "c2":
+        m = G.mixed(50_000, 100_000, 1)
+        desc = ("mixed MILP 50000 rows x 100000 cols (70% binary, 30% continuous [0,10]), "
+                "1M nnz, rows of 10-30, N(0,1) values, b = A x0 + |N(0,1)|, seed 1")
+    elif name == "c3":
+        m = G.mixed(500_000, 1_000_000, 1, power_law=True, nnz=5_000_000, max_len=5000)
+        desc = ("mixed MILP 500000 rows x 1000000 cols, 5M nnz, power-law row lengths "
+                "(Zipf 1.5, max 5000), N(0,1) values, seed 1")
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return m, desc
+
+
+def kernel_bytes_fp32(m: int, n: int, nnz: int) -> tuple[float, float]:
+    """Algorithmic HBM bytes of the two fused FP32 launches of one iteration.
+    Transpose pass (columns of A): Aᵀ values+indices 8·nnz, row pointers
+    4(n+1), z read once 4m, read x, x⁰, y, y⁰, ℓ, u, x̃, w 32n, write x, y,
+    x̄ 12n.  A pass (rows): 8·nnz, 4(m+1), x̄ read once 4n, read z, z⁰, b 12m,
+    is_eq m, write z 4m."""
+    at = 8 * nnz + 4 * (n + 1) + 4 * m + 32 * n + 12 * n
+    a = 8 * nnz + 4 * (m + 1) + 4 * n + 12 * m + m + 4 * m
+    return float(at), float(a)
+
+
+def kernel_bytes_fp64(m: int, n: int, nnz: int) -> tuple[float, float]:
+    """Algorithmic HBM bytes of the two SpMVs of the plain FP64 form:
+    12·nnz + 4(rows+1) + 8·(gathered vector) + 8·rows (the output)."""
+    at = 12 * nnz + 4 * (n + 1) + 8 * m + 8 * n
+    a = 12 * nnz + 4 * (m + 1) + 8 * n + 8 * m
+    return float(at), float(a)
+
+
+def run_projections(s, pp, milp, targets, tol):
+    from pdfp_amd import pdfp as P
+
+    iters = 0
+    t_total = 0.0
+    statuses = []
+    pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
+    for xt in targets:
+        p = P.SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                               iteration_limit=500_000)
+        t0 = time.perf_counter()
+        P.set_target(s, xt / pp.D_c)
+        st = P.pdfp_solve(s, p, pw)
+        t_total += time.perf_counter() - t0
+        iters += s.total_count
+        statuses.append(st)
+    return iters, t_total, statuses
+
+
+def cpu_baseline(pp, milp, target, iterations: int):
+    """The FP64 C oracle (the reference's CPU path restated) on one core:
+    PDHG iterations of one projection of the same workload, ‖A‖₂ excluded."""
+    from oracle import oracle as O
+    from pdfp_amd.pdfp import SolverParameters
+
+    o = O.OraclePDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    O.set_target(o, target / pp.D_c)
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                         iteration_limit=iterations)
+    pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
+    o.initialize(p, pw)                       # power iteration, untimed
+    t0 = time.perf_counter()
+    O.pdfp_solve(o, p, pw)
+    dt = time.perf_counter() - t0
+    it = o.total_count
+    return {"value": it / dt, "unit": "PDHG iter/s", "cores": 1, "kind": "port",
+            "sample": f"{it} PDHG iterations of one projection of the same workload "
+                      f"(FP64 C oracle, single thread, ||A||_2 excluded), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-iterations", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the projector has no multi-GPU path in this build "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    from pdfp_amd import generators as G
+    from pdfp_amd import pdfp as P
+
+    t0 = time.perf_counter()
+    milp, desc = workload(args.workload)
+    pp = P.projector_problem(milp)
+    mm, nn = milp.A.shape
+    nnz = milp.A.nnz
+    print(f"[bench] workload {args.workload}: {mm}x{nn}, nnz {nnz}, built in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    targets_w = [G.binary_targets(milp, 1000 + k) for k in range(args.warmup)]
+    targets = [G.binary_targets(milp, 2000 + k) for k in range(args.steps)]
+
+    results = {}
+    for prec in (["fp32"] if args.no_fp64 else ["fp32", "fp64"]):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision=prec)
+        s.set_x(np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
+        t0 = time.perf_counter()
+        run_projections(s, pp, milp, targets_w, args.tol)  # warmup (includes ‖A‖₂)
+        print(f"[bench] {prec} warmup {time.perf_counter() - t0:.2f}s", file=sys.stderr,
+              flush=True)
+        it, dt, st = run_projections(s, pp, milp, targets, args.tol)
+        ms_at, ms_a = s.time_kernels(200)
+        ms_iter = s.time_iterations(400)
+        results[prec] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
+                             ms_iter=ms_iter, info=s.info())
+        print(f"[bench] {prec}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
+              f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; iteration "
+              f"{ms_iter * 1e3:.1f}us; {st}", file=sys.stderr, flush=True)
+        s.close()
+
+    r32 = results["fp32"]
+    value = r32["iters"] / r32["seconds"]
+    bat, ba = kernel_bytes_fp32(mm, nn, nnz)
+    dom_bytes, dom_ms, dom_name = max((bat, r32["ms_at"], "transpose pass (A^T z + x/y update)"),
+                                      (ba, r32["ms_a"], "row pass (A xbar + z update)"),
+                                      key=lambda t: t[1])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "PDHG iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "step": "one warm-started l1 projection to a new 0/1 "
+                   f"target at relative KKT tolerance {args.tol:g}",
+                   "precision": "FP32 data/iterates, FP64 reductions and polish",
+                   "iterations_timed": r32["iters"]},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "kernel_ms": dom_ms},
+        "iteration_us": r32["ms_iter"] * 1e3,
+    }
+    if "fp64" in results:
+        r64 = results["fp64"]
+        v64 = r64["iters"] / r64["seconds"]
+        out["fp64_plain_value"] = v64
+        out["speedup_fp32_vs_fp64"] = value / v64
+        out["fp64_iteration_us"] = r64["ms_iter"] * 1e3
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(pp, milp, targets[0], args.cpu_iterations)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,245 @@
+/*
+ * libpdfp_hip — MI355X (gfx950) PDHG projector and root-LP solver for the
+ * PDFP feasibility pump.
+ *
+ * The library replaces the device side of the reference's two PDHG solvers:
+ *   - the projector  PDFPState / pdfp_solve!      (src/pdfp.jl:15-288,
+ *                                                   src/pdfp_kernels.jl:1-86)
+ *   - the root LP    PDLPState / pdlp_solve!      (src/pdlp.jl:33-372,
+ *                                                   src/pdlp_kernels.jl:1-63)
+ *   - the ‖A‖₂ power iteration                    (src/preconditioner.jl:108-125)
+ * In the reference these are selected by Julia dispatch on CuVector states;
+ * here they sit behind a C ABI that a Julia `ccall` (or Python ctypes) binds.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero pdfp_status
+ *     otherwise; the message is in pdfp_last_error(ctx) (or
+ *     pdfp_last_error(NULL) when no context exists yet).
+ *   - Input arrays are borrowed for the duration of the call and copied into
+ *     HBM; output arrays are caller-allocated.
+ *   - The matrix comes in compressed-sparse-column form, exactly as Julia's
+ *     SparseMatrixCSC stores it (colptr / rowval / nzval with Int64 indices);
+ *     `index_base` is 1 for Julia, 0 for C/Python.  The library builds both
+ *     CSR(A) and CSR(Aᵀ) in device memory (the reference builds Aᵀ on the host
+ *     with sparse(transpose(A)), src/pdfp.jl:68).
+ *   - Every call blocks until its results are on the host, matching the
+ *     reference's synchronous semantics.  One context belongs to one host
+ *     thread; different contexts are independent (own stream, own buffers).
+ */
+#ifndef PDFP_HIP_H
+#define PDFP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status and termination codes ------------------------------------- */
+
+enum pdfp_status {
+  PDFP_OK = 0,
+  PDFP_ERR_ARGUMENT = 1,  /* bad size, null pointer, malformed CSC          */
+  PDFP_ERR_DEVICE = 2,    /* a HIP runtime call failed                      */
+  PDFP_ERR_STATE = 3,     /* call out of order (e.g. solve before target)   */
+  PDFP_ERR_NOMEM = 4
+};
+
+/* Termination reasons: the reference's Symbols (src/pdfp.jl:273-287,
+ * src/pdlp.jl:349-371) as integers. */
+enum pdfp_termination {
+  PDFP_UNSPECIFIED = 0,
+  PDFP_OPTIMAL = 1,
+  PDFP_ITERATION_LIMIT = 2,
+  PDFP_TIME_LIMIT = 3,
+  PDFP_PRIMAL_INFEASIBLE = 4,
+  PDFP_DUAL_INFEASIBLE = 5
+};
+
+/* Arithmetic of the iterates. */
+enum pdfp_precision {
+  /* FP64 everywhere; the reference's launch structure: SpMV Aᵀz, primal
+   * update, y update, SpMV Ax̄, z update — five launches per iteration. */
+  PDFP_PREC_FP64 = 0,
+  /* FP32 CSR values and iterates, FP64 accumulation inside each row, two
+   * fused launches per iteration, FP64 restart/KKT reductions and an FP64
+   * polish of the returned point. */
+  PDFP_PREC_FP32_MIXED = 1
+};
+
+/* Mirrors SolverParameters + RestartParameters (src/pdlp.jl:10-31); the
+ * defaults (pdfp_default_params) are the reference's. */
+typedef struct pdfp_params {
+  double eps_optimal_relative;              /* 1e-4      */
+  double eps_feasible_relative;             /* 1e-4      */
+  int64_t iteration_limit;                  /* 100000    */
+  double time_limit;                        /* 3600 s    */
+  int32_t termination_evaluation_frequency; /* 200       */
+  int32_t sv_max_iterations;                /* 5000      */
+  double reflection_coefficient;            /* 1.0       */
+  double sv_tolerance;                      /* 1e-4      */
+  double artificial_restart_threshold;      /* 0.36      */
+  double sufficient_reduction_for_restart;  /* 0.2       */
+  double necessary_reduction_for_restart;   /* 0.5       */
+  double k_p;                               /* 0.99      */
+  double k_i;                               /* 0.01      */
+  double k_d;                               /* 0.0       */
+  double i_smooth;                          /* 0.3       */
+} pdfp_params;
+
+/* Scalars of the last solve (fields of PDFPState, src/pdfp.jl:31-51). */
+typedef struct pdfp_info {
+  double relative_primal_residual;
+  double relative_dual_residual;
+  double relative_l1_gap;
+  double l1_distance;        /* Σ_{j∈I} w_j |x̂_j - x̃_j| of the returned point  */
+  double fixed_point_error;
+  double initial_fixed_point_error;
+  double primal_weight;
+  double step_size;
+  double max_singular_value;
+  double solve_seconds;      /* wall time of the last pdfp_solve                */
+  int64_t total_count;       /* PDHG iterations of the last solve               */
+  int32_t termination_reason;
+  int32_t n_restarts;
+  int32_t precision;
+  int32_t n_evaluations;
+} pdfp_info;
+
+/* Fills `p` with the reference defaults. */
+void pdfp_default_params(pdfp_params* p);
+
+/* ---- projector (src/pdfp.jl) ------------------------------------------ */
+
+typedef struct pdfp_ctx pdfp_ctx;
+
+/* Message of the last failure on `ctx`, or of the last failed
+ * pdfp_create / pdlp_create when ctx is NULL.  Never NULL. */
+const char* pdfp_last_error(const pdfp_ctx* ctx);
+
+/* PDFPState(A, b, is_eq, is_int, var_lb, var_ub, w; use_gpu)
+ *   src/pdfp.jl:60-76.  A (m×n, scaled) in CSC; b, is_eq of length m;
+ *   is_int, var_lb, var_ub, w of length n (±Inf allowed in the bounds).
+ *   Iterates start at zero, the target at zero. */
+int pdfp_create(pdfp_ctx** out, int64_t m, int64_t n, int64_t nnz,
+                const int64_t* colptr, const int64_t* rowval,
+                const double* nzval, int32_t index_base, const double* b,
+                const uint8_t* is_eq, const uint8_t* is_int,
+                const double* var_lb, const double* var_ub, const double* w,
+                int32_t precision, int32_t device);
+
+/* original_primal_residual!(s, D_r)  src/pdfp.jl:85-90. */
+int pdfp_set_original_primal_residual(pdfp_ctx* ctx, const double* D_r);
+
+/* copyto!(current_x, x); copyto!(initial_x, x)  src/pump.jl:305-306. */
+int pdfp_set_x(pdfp_ctx* ctx, const double* x_scaled);
+
+/* set_target!(s, x̃)  src/pdfp.jl:98-101 (copies x̃, then warm_start!). */
+int pdfp_set_target(pdfp_ctx* ctx, const double* x_tilde_scaled);
+
+/* warm_start!(s)  src/pdfp.jl:104-114. */
+int pdfp_warm_start(pdfp_ctx* ctx);
+
+/* pdfp_solve!(s, p, primal_weight)  src/pdfp.jl:252-288.  `params` may be
+ * NULL for the defaults.  termination / total_count may be NULL. */
+int pdfp_solve(pdfp_ctx* ctx, const pdfp_params* params, double primal_weight,
+               int32_t* termination, int64_t* total_count);
+
+/* Array(s.pdhg_x)  src/pump.jl:338 — the projection in scaled units, as
+ * FP64.  In PDFP_PREC_FP32_MIXED the point is polished: a coordinate that
+ * the last primal step clamped to a bound takes that bound's FP64 value. */
+int pdfp_get_x(pdfp_ctx* ctx, double* x_scaled_out);
+
+/* Other iterates, as FP64 (for checks and warm restarts by the caller).
+ * which: 0 pdhg_x, 1 pdhg_y, 2 pdhg_z, 3 current_x, 4 current_y,
+ *        5 current_z, 6 dual_slack, 7 x_tilde */
+int pdfp_get_vector(pdfp_ctx* ctx, int32_t which, double* out);
+
+/* Scalars of the last solve. */
+int pdfp_get_info(pdfp_ctx* ctx, pdfp_info* out);
+
+/* Frees every device buffer of the context. NULL is a no-op. */
+void pdfp_destroy(pdfp_ctx* ctx);
+
+/* ---- root LP (src/pdlp.jl) -------------------------------------------- */
+
+typedef struct pdlp_ctx pdlp_ctx;
+
+/* Message of the last failure on `ctx` (NULL: as pdfp_last_error(NULL)). */
+const char* pdlp_last_error(const pdlp_ctx* ctx);
+
+typedef struct pdlp_info {
+  double relative_primal_residual;
+  double relative_dual_residual;
+  double relative_optimality_gap;
+  double objective_value;
+  double fixed_point_error;
+  double primal_weight;
+  double step_size;
+  double solve_seconds;
+  int64_t total_count;
+  int32_t termination_reason;
+  int32_t n_restarts;
+} pdlp_info;
+
+/* PDLPState(sp; use_gpu)  src/pdlp.jl:82-96 on an already scaled problem
+ * (c, var_lb/ub, con_lb/ub of the ScaledProblem, src/preconditioner.jl:13-24). */
+int pdlp_create(pdlp_ctx** out, int64_t m, int64_t n, int64_t nnz,
+                const int64_t* colptr, const int64_t* rowval,
+                const double* nzval, int32_t index_base, const double* c,
+                const double* var_lb, const double* var_ub,
+                const double* con_lb, const double* con_ub,
+                double bound_scale, double objective_scale, int32_t device);
+
+/* cupdlpx_termination!(s, D_r, D_c)  src/pdlp.jl:105-110. */
+int pdlp_set_cupdlpx_termination(pdlp_ctx* ctx, const double* D_r,
+                                 const double* D_c);
+
+/* pdlp_solve!(s, p)  src/pdlp.jl:325-372. */
+int pdlp_solve(pdlp_ctx* ctx, const pdfp_params* params, int32_t* termination,
+               int64_t* total_count);
+
+/* Array(s.pdhg_primal), Array(s.dual_slack) in scaled units
+ * (src/pdlp.jl:391-392). */
+int pdlp_get_primal(pdlp_ctx* ctx, double* out);
+int pdlp_get_dual_slack(pdlp_ctx* ctx, double* out);
+int pdlp_get_info(pdlp_ctx* ctx, pdlp_info* out);
+void pdlp_destroy(pdlp_ctx* ctx);
+
+/* ---- building blocks (exposed for measurement and tests) --------------- */
+
+/* ‖A‖₂ by power iteration on AAᵀ on the device
+ * (estimate_max_singular_value, src/preconditioner.jl:108-125), from the
+ * library's fixed pseudo-random start (pdfp_start_vector).  Uses the
+ * projector context's matrix. */
+int pdfp_max_singular_value(pdfp_ctx* ctx, int32_t max_iterations,
+                            double tolerance, double* sigma_out);
+
+/* The start vector of the power iteration: m standard normals from a
+ * SplitMix64 stream seeded with `seed` (Box–Muller). Host only. */
+void pdfp_start_vector(int64_t m, uint64_t seed, double* out);
+
+/* y = A x (transpose=0) or y = Aᵀ x (transpose=1) on the context's scaled
+ * matrix, in the context's precision, x and y on the host as FP64; used by
+ * the SpMV parity tests. */
+int pdfp_spmv(pdfp_ctx* ctx, int32_t transpose, const double* x, double* y);
+
+/* Runs `iterations` PDHG iterations with the current scalars and no
+ * evaluation, timing them on the device with events on the context's
+ * stream; returns the mean milliseconds per iteration.  For measurement. */
+int pdfp_time_iterations(pdfp_ctx* ctx, int32_t iterations, double* ms_out);
+
+/* Per-kernel mean duration (ms) of the kernels that pdfp_time_iterations
+ * launched, measured with events around each launch on the context's
+ * stream: which 0 = transpose (Aᵀz) pass, 1 = A (Ax̄) pass. */
+int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at,
+                      double* ms_a);
+
+/* Device memory held by the context, in bytes. */
+int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDFP_HIP_H */

@@ -1,0 +1,62 @@
+"""Builds libpdfp_hip.so in-tree with hipcc for gfx950.
+
+    python -m pdfp_amd.build            # build the library
+    python -m pdfp_amd.build --check    # print the exported C-ABI symbols
+
+The library is written next to this file (pdfp_amd/libpdfp_hip.so).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpdfp_hip.so")
+SOURCES = ["pdfp_hip.hip", "pdlp_hip.hip"]
+
+# -ffp-contract=off: no a*b+c contraction, so the FP64 form performs the
+# reference's operations one by one.  The packed-FP32 target feature is
+# switched off for all device code.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose: bool = False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def exported_symbols(path: str = LIB) -> list[str]:
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    return sorted(line.split()[-1] for line in out.splitlines() if " T " in line)
+
+
+if __name__ == "__main__":
+    if "--check" in sys.argv:
+        print("\n".join(s for s in exported_symbols() if s.startswith(("pdfp_", "pdlp_"))))
+    else:
+        print(build(verbose=True))

@@ -1,0 +1,241 @@
+// CSR row-pass engine for gfx950: one launch walks the rows of a CSR matrix
+// in a precomputed row-block schedule, forms each row's inner product with a
+// gathered vector (or two), and hands the sums to a fused per-row epilogue.
+//
+// Schedule (built on the host, see schedule.cpp): consecutive rows are packed
+// into a block while the block holds at most BLOCK rows and at most NPB
+// nonzeros.  A row longer than NPB sits alone in its block.
+//
+//   stream mode  (block nnz <= NPB): the block streams its contiguous slice of
+//     (col, val) once, coalesced, forms the products a_ij * x_j in FP64 and
+//     stages them in LDS; then groups of `tpr` lanes (a power of two, chosen
+//     from the rows in the block, at most one wavefront) reduce one row each
+//     with xor-shuffles inside the group.
+//   vector mode  (one row, nnz > NPB): all BLOCK lanes stride the row and the
+//     row sum is a workgroup reduction.
+//
+// Products of FP32 operands are exact in FP64 and every row sum accumulates
+// in FP64: on long power-law rows this replaces compensated FP32 summation.
+// With tpr == 1 each row is summed in ascending column order, the order of
+// the reference's CSC product.
+//
+// Per-block FP64 partial sums (NACC of them) of the epilogues' reduction
+// terms are written to `partials[block * NACC + k]`; a finalize kernel adds
+// them in block order, so the reductions are deterministic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdfp {
+
+constexpr int kWave = 64;
+
+template <typename TV>
+struct Csr {
+  const int32_t* __restrict__ rowptr;  // nrows + 1
+  const int32_t* __restrict__ col;     // nnz
+  const TV* __restrict__ val;          // nnz
+  const int32_t* __restrict__ blocks;  // nblocks + 1 row boundaries
+  int32_t nblocks;
+  int32_t nrows;
+};
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD gets a
+// contiguous range of row blocks (neighbouring rows share L2 on banded data).
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+  constexpr int kXcd = 8;
+  const int per = nb / kXcd;
+  const int rem = nb % kXcd;
+  const int xcd = bid % kXcd;
+  const int idx = bid / kXcd;
+  // XCDs 0..rem-1 own per+1 blocks, the others per.
+  const int start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+  return start + idx;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum of v over the block, returned on thread 0 (other threads: garbage).
+template <int BLOCK>
+__device__ __forceinline__ double block_sum_to0(double v, double* red) {
+  constexpr int kW = BLOCK / kWave;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < kW; ++w) s += red[w];
+  }
+  return s;
+}
+
+template <int BLOCK, int NACC>
+__device__ __forceinline__ void block_store_partials(const double (&acc)[NACC],
+                                                     double* __restrict__ out) {
+  constexpr int kW = BLOCK / kWave;
+  __shared__ double red[kW][NACC];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) red[wid][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NACC) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kW; ++w) s += red[w][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// ---- gathers: the per-nonzero products ----------------------------------
+
+// p0 = a * x[c]
+template <typename TX>
+struct Gather1 {
+  static constexpr int NV = 1;
+  const TX* __restrict__ x;
+  __device__ __forceinline__ void operator()(int c, double a, double* p) const {
+    p[0] = a * (double)x[c];
+  }
+};
+
+// p0 = a * x[c],  p1 = a * (x[c] - r[c])  (the difference is formed in FP64)
+template <typename TX>
+struct GatherDelta {
+  static constexpr int NV = 2;
+  const TX* __restrict__ x;
+  const TX* __restrict__ r;
+  __device__ __forceinline__ void operator()(int c, double a, double* p) const {
+    const double xc = (double)x[c];
+    p[0] = a * xc;
+    p[1] = a * (xc - (double)r[c]);
+  }
+};
+
+// ---- the pass ---------------------------------------------------------------
+
+// Epi: __device__ void operator()(int row, const double* sums, double* acc)
+template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC>
+__global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
+                                                  double* __restrict__ partials) {
+  constexpr int NV = Gather::NV;
+  constexpr int U = 4;  // nonzeros in flight per lane while staging
+  __shared__ double stage[NV][NPB];
+  __shared__ double red[BLOCK / kWave];
+
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int r0 = A.blocks[b];
+  const int r1 = A.blocks[b + 1];
+  const int base = A.rowptr[r0];
+  const int cnt = A.rowptr[r1] - base;
+
+  double acc[NACC > 0 ? NACC : 1];
+#pragma unroll
+  for (int k = 0; k < (NACC > 0 ? NACC : 1); ++k) acc[k] = 0.0;
+
+  if (cnt <= NPB) {
+    for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
+      int cc[U];
+      double vv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * BLOCK + tid;
+        cc[u] = k < cnt ? A.col[base + k] : 0;
+        vv[u] = k < cnt ? (double)A.val[base + k] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * BLOCK + tid;
+        if (k < cnt) {
+          double p[NV];
+          g(cc[u], vv[u], p);
+#pragma unroll
+          for (int q = 0; q < NV; ++q) stage[q][k] = p[q];
+        }
+      }
+    }
+    __syncthreads();
+    const int nrows = r1 - r0;
+    int tpr = 1;
+    while (tpr < kWave && tpr * 2 * nrows <= BLOCK) tpr *= 2;
+    const int grp = tid / tpr;
+    const int lane = tid & (tpr - 1);
+    const int row = r0 + grp;
+    double s[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q] = 0.0;
+    if (row < r1) {
+      const int k1 = A.rowptr[row + 1] - base;
+      for (int k = A.rowptr[row] - base + lane; k < k1; k += tpr) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) s[q] += stage[q][k];
+      }
+    }
+    for (int off = tpr >> 1; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) s[q] += __shfl_xor(s[q], off, tpr);
+    }
+    if (lane == 0 && row < r1) epi(row, s, acc);
+  } else {
+    double s[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q] = 0.0;
+    for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
+      int cc[U];
+      double vv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * BLOCK + tid;
+        cc[u] = k < cnt ? A.col[base + k] : 0;
+        vv[u] = k < cnt ? (double)A.val[base + k] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * BLOCK + tid;
+        if (k < cnt) {
+          double p[NV];
+          g(cc[u], vv[u], p);
+#pragma unroll
+          for (int q = 0; q < NV; ++q) s[q] += p[q];
+        }
+      }
+    }
+    double t[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) t[q] = block_sum_to0<BLOCK>(s[q], red);
+    if (tid == 0) epi(r0, t, acc);
+  }
+  if constexpr (NACC > 0) {
+    block_store_partials<BLOCK, NACC>(acc, partials + (size_t)b * NACC);
+  }
+}
+
+// Adds per-block partials (nb blocks × nacc terms) in block order into
+// out[0..nacc).  One workgroup.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void finalize_partials(const double* __restrict__ partials,
+                                                           int nb, int nacc,
+                                                           double* __restrict__ out) {
+  __shared__ double red[BLOCK / kWave];
+  for (int k = 0; k < nacc; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLOCK) s += partials[(size_t)b * nacc + k];
+    s = block_sum_to0<BLOCK>(s, red);
+    if (threadIdx.x == 0) out[k] = s;
+  }
+}
+
+}  // namespace pdfp

@@ -1,0 +1,916 @@
+// libpdfp_hip: the projector (src/pdfp.jl) on gfx950 behind a C ABI.
+//
+// Host side: the solve loop of pdfp_solve! (src/pdfp.jl:252-288), the
+// restart rule and PID primal weight (src/pdlp.jl:167-205), warm starts and
+// targets (src/pdfp.jl:98-114).  Device side: everything that touches a
+// vector.  Per 200-iteration evaluation block the host reads back one small
+// array of FP64 sums (fixed-point error, KKT residuals, ℓ1 gap, restart
+// distances), so a block costs one synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pdfp_hip.h"
+#include "engine.h"
+#include "host_util.h"
+#include "pdfp_kernels.h"
+
+namespace pdfp {
+
+constexpr int kBlock = 256;  // threads per workgroup of the CSR passes
+constexpr int kNpb = 2048;   // nonzeros staged per workgroup
+constexpr uint64_t kSvSeed = 20240517ull;
+constexpr double kJuliaEps = 2.220446049250313e-16;
+
+std::string g_create_error = "no error";
+
+// ---- restart logic shared with the root LP (src/pdlp.jl:167-205) ---------
+
+struct RestartState {
+  double primal_weight = 1.0, best_primal_weight = 1.0;
+  double best_primal_dual_residual_gap = INFINITY;
+  double primal_weight_error_sum = 0.0, primal_weight_last_error = 0.0;
+  double fixed_point_error = INFINITY, initial_fixed_point_error = INFINITY;
+  double last_trial_fixed_point_error = INFINITY;
+  int64_t inner_count = 0, total_count = 0;
+  double relative_primal_residual = INFINITY, relative_dual_residual = INFINITY;
+};
+
+// should_restart  src/pdlp.jl:167-181
+inline bool should_restart(RestartState& s, const pdfp_params& p) {
+  bool restart = false;
+  if (s.total_count == p.termination_evaluation_frequency) {
+    restart = true;
+  } else if (s.total_count > p.termination_evaluation_frequency) {
+    if (s.fixed_point_error <= p.sufficient_reduction_for_restart * s.initial_fixed_point_error)
+      restart = true;
+    if (s.fixed_point_error <= p.necessary_reduction_for_restart * s.initial_fixed_point_error &&
+        s.fixed_point_error > s.last_trial_fixed_point_error)
+      restart = true;
+    if ((double)s.inner_count >= p.artificial_restart_threshold * (double)s.total_count)
+      restart = true;
+  }
+  s.last_trial_fixed_point_error = s.fixed_point_error;
+  return restart;
+}
+
+inline double jmax(double a, double b) {
+  if (std::isnan(a) || std::isnan(b)) return NAN;
+  return a > b ? a : b;
+}
+
+// update_primal_weight!  src/pdlp.jl:185-205
+inline void update_primal_weight(RestartState& s, const pdfp_params& rp, double primal_dist,
+                                 double dual_dist) {
+  const double ratio = s.relative_dual_residual / jmax(s.relative_primal_residual, kJuliaEps);
+  if (1e-16 < primal_dist && primal_dist < 1e12 && 1e-16 < dual_dist && dual_dist < 1e12 &&
+      1e-8 < ratio && ratio < 1e8) {
+    const double err = std::log(dual_dist) - std::log(primal_dist) - std::log(s.primal_weight);
+    s.primal_weight_error_sum *= rp.i_smooth;
+    s.primal_weight_error_sum += err;
+    const double delta_err = err - s.primal_weight_last_error;
+    s.primal_weight *=
+        std::exp(rp.k_p * err + rp.k_i * s.primal_weight_error_sum + rp.k_d * delta_err);
+    s.primal_weight_last_error = err;
+  } else {
+    s.primal_weight = s.best_primal_weight;
+    s.primal_weight_error_sum = 0.0;
+    s.primal_weight_last_error = 0.0;
+  }
+  const double gap =
+      std::fabs(std::log10(s.relative_dual_residual / jmax(s.relative_primal_residual, kJuliaEps)));
+  if (gap < s.best_primal_dual_residual_gap) {
+    s.best_primal_dual_residual_gap = gap;
+    s.best_primal_weight = s.primal_weight;
+  }
+}
+
+// SplitMix64 + Box–Muller normals: the fixed start of the power iteration.
+inline uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline void start_vector(int64_t m, uint64_t seed, double* out) {
+  uint64_t s = seed;
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t i = 0; i < m; i += 2) {
+    const double u1 = ((double)(splitmix64(s) >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    out[i] = r * std::cos(two_pi * u2);
+    if (i + 1 < m) out[i + 1] = r * std::sin(two_pi * u2);
+  }
+}
+
+inline int grid_for(int64_t n) { return (int)((n + 255) / 256); }
+
+// Device CSR + schedule of one operator (A or Aᵀ).
+template <typename TV>
+struct DevCsr {
+  DevBuf<int32_t> rowptr, col, blocks;
+  DevBuf<TV> val;
+  int32_t nblocks = 0, nrows = 0;
+  int64_t nnz = 0;
+  void upload(const HostCsr& h, hipStream_t s) {
+    nrows = (int32_t)h.nrows;
+    nnz = (int64_t)h.col.size();
+    std::vector<int32_t> blk = row_blocks(h.rowptr, kBlock, kNpb);
+    nblocks = (int32_t)blk.size() - 1;
+    rowptr.alloc(h.rowptr.size());
+    rowptr.upload(h.rowptr.data(), h.rowptr.size(), s);
+    col.alloc(std::max<size_t>(h.col.size(), 1));
+    col.upload(h.col.data(), h.col.size(), s);
+    std::vector<TV> v(h.val.size());
+    for (size_t k = 0; k < v.size(); ++k) v[k] = (TV)h.val[k];
+    val.alloc(std::max<size_t>(v.size(), 1));
+    val.upload(v.data(), v.size(), s);
+    blocks.alloc(blk.size());
+    blocks.upload(blk.data(), blk.size(), s);
+    PDFP_HIP(hipStreamSynchronize(s));  // the host vectors die with this scope
+  }
+  Csr<TV> view() const {
+    return Csr<TV>{rowptr.p, col.p, val.p, blocks.p, nblocks, nrows};
+  }
+  size_t bytes() const { return rowptr.bytes() + col.bytes() + blocks.bytes() + val.bytes(); }
+};
+
+// Launch one CSR pass (no-op on an empty operator).
+template <int NACC, typename TV, class G, class E>
+inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
+  if (A.nblocks == 0) return;
+  csr_pass<kBlock, kNpb, TV, G, E, NACC><<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+}
+
+struct ProjectorBase {
+  virtual ~ProjectorBase() = default;
+  virtual void set_original_primal_residual(const double* D_r) = 0;
+  virtual void set_x(const double* x) = 0;
+  virtual void set_target(const double* xt) = 0;
+  virtual void warm_start() = 0;
+  virtual int solve(const pdfp_params& p, double pw) = 0;
+  virtual void get_x(double* out) = 0;
+  virtual void get_vector(int which, double* out) = 0;
+  virtual void info(pdfp_info* out) = 0;
+  virtual double max_singular_value(int iters, double tol) = 0;
+  virtual void spmv(int transpose, const double* x, double* y) = 0;
+  virtual double time_iterations(int iters) = 0;
+  virtual void time_kernels(int iters, double* ms_at, double* ms_a) = 0;
+  virtual int64_t device_bytes() const = 0;
+};
+
+template <typename T>
+struct Projector final : ProjectorBase {
+  static constexpr bool kFused = std::is_same<T, float>::value;
+
+  int64_t m, n, nnz;
+  int device;
+  hipStream_t stream = nullptr;
+  DevCsr<T> A, At;  // A: m×n rows, At: n×m rows (columns of A)
+
+  DevBuf<T> b, lb, ub, w, xt;
+  DevBuf<T> x, x0, y, y0, rx, px, ds, py, ry;  // n
+  DevBuf<T> z, z0, pz, rz;                     // m
+  DevBuf<uint8_t> is_eq;
+  DevBuf<double> row_unscale;
+  DevBuf<double> atz64, ax64;                 // plain FP64 form scratch
+  DevBuf<double> lb64, ub64, xt64, w64, x64;  // FP32 form: polish
+  DevBuf<double> part_t, part_a, part_v, sums;
+  DevBuf<StepParams> d_sp;
+  PinnedBuf<StepParams> h_sp;
+  PinnedBuf<double> h_sums;
+
+  std::vector<double> h_w, h_xt;  // w (0 off I) and x̃, FP64, for denominators
+  bool orig_primal = false;
+  double b_norm = 0.0, b_norm_original = NAN;
+
+  double sv = NAN, step_size = 0.0;
+  RestartState r;
+  double relative_l1_gap = INFINITY, dual_den = 1.0, l1_den = 1.0;
+  double l1_distance = NAN;
+  int termination = PDFP_UNSPECIFIED;
+  int n_restarts = 0, n_evals = 0;
+  double solve_seconds = 0.0;
+  bool x64_valid = false;
+
+  Projector(int64_t m_, int64_t n_, const int64_t* colptr, const int64_t* rowval,
+            const double* nzval, int32_t base, const double* b_, const uint8_t* is_eq_,
+            const uint8_t* is_int_, const double* var_lb, const double* var_ub, const double* w_,
+            int dev)
+      : m(m_), n(n_), device(dev) {
+    PDFP_HIP(hipSetDevice(device));
+    PDFP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HostCsr ha = csc_to_csr(m, n, colptr, rowval, nzval, base);
+    HostCsr ht = csc_as_transpose_csr(m, n, colptr, rowval, nzval, base);
+    nnz = (int64_t)ha.col.size();
+    A.upload(ha, stream);
+    At.upload(ht, stream);
+
+    h_w.assign((size_t)n, 0.0);
+    for (int64_t j = 0; j < n; ++j) h_w[(size_t)j] = is_int_[j] ? w_[j] : 0.0;
+    for (int64_t j = 0; j < n; ++j)
+      if (is_int_[j] && !(w_[j] > 0.0))
+        throw Error(PDFP_ERR_ARGUMENT, "weights of integer columns must be positive");
+    h_xt.assign((size_t)n, 0.0);
+    double acc = 0.0;
+    for (int64_t i = 0; i < m; ++i) acc += b_[i] * b_[i];
+    b_norm = std::sqrt(acc);
+
+    auto vec = [&](DevBuf<T>& d, int64_t len, const double* src) {
+      d.alloc((size_t)std::max<int64_t>(len, 1));
+      std::vector<T> tmp((size_t)std::max<int64_t>(len, 1), (T)0);
+      if (src)
+        for (int64_t k = 0; k < len; ++k) tmp[(size_t)k] = (T)src[k];
+      d.upload(tmp.data(), tmp.size(), stream);
+      PDFP_HIP(hipStreamSynchronize(stream));
+    };
+    vec(b, m, b_);
+    vec(lb, n, var_lb);
+    vec(ub, n, var_ub);
+    vec(w, n, h_w.data());
+    vec(xt, n, nullptr);
+    for (DevBuf<T>* v : {&x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry}) vec(*v, n, nullptr);
+    for (DevBuf<T>* v : {&z, &z0, &pz, &rz}) vec(*v, m, nullptr);
+    is_eq.alloc((size_t)std::max<int64_t>(m, 1));
+    is_eq.upload(is_eq_, (size_t)m, stream);
+    if (!kFused) {
+      atz64.alloc((size_t)std::max<int64_t>(n, 1));
+      ax64.alloc((size_t)std::max<int64_t>(m, 1));
+    } else {
+      auto vec64 = [&](DevBuf<double>& d, const double* src) {
+        d.alloc((size_t)std::max<int64_t>(n, 1));
+        if (src) d.upload(src, (size_t)n, stream);
+        PDFP_HIP(hipStreamSynchronize(stream));  // src may be a temporary
+      };
+      vec64(lb64, var_lb);
+      vec64(ub64, var_ub);
+      vec64(w64, h_w.data());
+      std::vector<double> zeros((size_t)std::max<int64_t>(n, 1), 0.0);
+      vec64(xt64, zeros.data());
+      vec64(x64, zeros.data());
+    }
+    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
+    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
+    part_v.alloc((size_t)std::max<int64_t>(grid_for(std::max(m, n)), 1) * 2 + 8);
+    sums.alloc(64);
+    d_sp.alloc(1);
+    h_sp.alloc(1);
+    h_sums.alloc(64);
+    // terms of an empty operator are never written: start every slot at 0
+    std::memset(h_sums.p, 0, sizeof(double) * 64);
+    sums.upload(h_sums.p, 64, stream);
+    PDFP_HIP(hipStreamSynchronize(stream));
+  }
+
+  ~Projector() override {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+
+  int64_t device_bytes() const override {
+    size_t s = A.bytes() + At.bytes();
+    for (const DevBuf<T>* v : {&b, &lb, &ub, &w, &xt, &x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry,
+                               &z, &z0, &pz, &rz})
+      s += v->bytes();
+    for (const DevBuf<double>* v : {&row_unscale, &atz64, &ax64, &lb64, &ub64, &xt64, &w64, &x64,
+                                    &part_t, &part_a, &part_v, &sums})
+      s += v->bytes();
+    return (int64_t)(s + is_eq.bytes());
+  }
+
+  void sync() { PDFP_HIP(hipStreamSynchronize(stream)); }
+
+  void upload_T(DevBuf<T>& d, const double* src, int64_t len) {
+    std::vector<T> tmp((size_t)len);
+    for (int64_t k = 0; k < len; ++k) tmp[(size_t)k] = (T)src[k];
+    d.upload(tmp.data(), (size_t)len, stream);
+    sync();
+  }
+  void download_T(const DevBuf<T>& d, double* dst, int64_t len) {
+    std::vector<T> tmp((size_t)len);
+    d.download(tmp.data(), (size_t)len, stream);
+    sync();
+    for (int64_t k = 0; k < len; ++k) dst[k] = (double)tmp[(size_t)k];
+  }
+
+  void set_original_primal_residual(const double* D_r) override {
+    std::vector<double> inv((size_t)m);
+    double acc = 0.0;
+    for (int64_t i = 0; i < m; ++i) inv[(size_t)i] = 1.0 / D_r[i];
+    // b_norm_original = ‖b ./ D_r‖ of the FP64 b (src/pdfp.jl:87)
+    std::vector<double> hb((size_t)m);
+    download_T(b, hb.data(), m);
+    for (int64_t i = 0; i < m; ++i) {
+      const double v = hb[(size_t)i] / D_r[i];
+      acc += v * v;
+    }
+    b_norm_original = std::sqrt(acc);
+    row_unscale.alloc((size_t)std::max<int64_t>(m, 1));
+    row_unscale.upload(inv.data(), (size_t)m, stream);
+    sync();
+    orig_primal = true;
+  }
+
+  void set_x(const double* xs) override {
+    upload_T(x, xs, n);
+    upload_T(x0, xs, n);
+  }
+
+  void set_target(const double* xts) override {
+    upload_T(xt, xts, n);
+    std::copy(xts, xts + n, h_xt.begin());
+    if (kFused) {
+      xt64.upload(xts, (size_t)n, stream);
+      sync();
+    }
+    warm_start();
+  }
+
+  // warm_start!  src/pdfp.jl:104-114
+  void warm_start() override {
+    if (n) {
+      PDFP_HIP(hipMemcpyAsync(x0.p, x.p, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
+      PDFP_HIP(hipMemcpyAsync(y0.p, y.p, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
+    }
+    if (m) PDFP_HIP(hipMemcpyAsync(z0.p, z.p, sizeof(T) * m, hipMemcpyDeviceToDevice, stream));
+    sync();
+    r.inner_count = 0;
+    r.total_count = 0;
+    r.fixed_point_error = INFINITY;
+    r.last_trial_fixed_point_error = INFINITY;
+    termination = PDFP_UNSPECIFIED;
+    x64_valid = false;
+  }
+
+  // ---- one PDHG iteration ----------------------------------------------------
+
+  template <bool MAJOR>
+  void step_fused(int k_offset) {
+    EpiPrimalY<T, MAJOR> ex{d_sp.p, k_offset, x.p, x0.p, y.p, y0.p, lb.p, ub.p, xt.p, w.p,
+                            rx.p, px.p, ds.p, py.p, ry.p};
+    launch_pass<0>(At, Gather1<T>{z.p}, ex, nullptr, stream);
+    EpiDualZ<T, MAJOR> ez{d_sp.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    launch_pass<0>(A, Gather1<T>{rx.p}, ez, nullptr, stream);
+  }
+
+  // The reference's structure: SpMV, K1, K2, SpMV, K3 (src/pdfp.jl:158-163).
+  void step_plain(int k_offset) {
+    if constexpr (!kFused) {
+      launch_pass<0>(At, Gather1<double>{z.p}, EpiStore<double>{atz64.p}, nullptr, stream);
+      if (n) {
+        k_primal<<<grid_for(n), 256, 0, stream>>>(d_sp.p, k_offset, (int)n, px.p, ds.p, rx.p, x.p,
+                                                  x0.p, y.p, atz64.p, lb.p, ub.p);
+        k_dual_y<<<grid_for(n), 256, 0, stream>>>(d_sp.p, k_offset, (int)n, py.p, y.p, ry.p, y0.p,
+                                                  xt.p, rx.p, w.p);
+      }
+      launch_pass<0>(A, Gather1<double>{rx.p}, EpiStore<double>{ax64.p}, nullptr, stream);
+      if (m)
+        k_dual_z<<<grid_for(m), 256, 0, stream>>>(d_sp.p, k_offset, (int)m, pz.p, z.p, rz.p, z0.p,
+                                                  ax64.p, b.p, is_eq.p);
+    }
+  }
+
+  void step(int k_offset, bool major) {
+    if (kFused) {
+      if (major) step_fused<true>(k_offset);
+      else step_fused<false>(k_offset);
+    } else {
+      step_plain(k_offset);
+    }
+  }
+
+  // Evaluation: FP64 sums into sums[slot*16 ...]; see EpiEvalCol/EpiEvalRow.
+  void evaluate(int slot) {
+    EpiEvalCol<T> ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p};
+    launch_pass<kEvalColTerms>(At, GatherDelta<T>{pz.p, rz.p}, ec, part_t.p, stream);
+    EpiEvalRow<T> er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr};
+    launch_pass<kEvalRowTerms>(A, Gather1<T>{px.p}, er, part_a.p, stream);
+    if (At.nblocks)
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_t.p, At.nblocks, kEvalColTerms,
+                                                    sums.p + slot * 16);
+    if (A.nblocks)
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, kEvalRowTerms,
+                                                    sums.p + slot * 16 + 8);
+  }
+
+  void upload_step_params(double omega) {
+    h_sp.p[0].step_size = step_size;
+    h_sp.p[0].primal_weight = r.primal_weight;
+    h_sp.p[0].omega = omega;
+    h_sp.p[0].inner_count = (double)r.inner_count;
+    PDFP_HIP(hipMemcpyAsync(d_sp.p, h_sp.p, sizeof(StepParams), hipMemcpyHostToDevice, stream));
+  }
+
+  // ---- power iteration on the device -----------------------------------------
+
+  double reduce_scalar_vec(int nb, int nacc, double* out) {
+    finalize_partials<256><<<1, 256, 0, stream>>>(part_v.p, nb, nacc, sums.p + 48);
+    // slots 48..63: this reduction and the power iteration's σ² (slot 56)
+    PDFP_HIP(hipMemcpyAsync(h_sums.p + 48, sums.p + 48, sizeof(double) * 16,
+                            hipMemcpyDeviceToHost, stream));
+    sync();
+    for (int k = 0; k < nacc; ++k) out[k] = h_sums.p[48 + k];
+    return out[0];
+  }
+
+  // estimate_max_singular_value  src/preconditioner.jl:108-125
+  double max_singular_value(int max_iterations, double tolerance) override {
+    if (m == 0) return 0.0;
+    DevBuf<double> eig, nxt, dual;
+    eig.alloc((size_t)m);
+    nxt.alloc((size_t)m);
+    dual.alloc((size_t)std::max<int64_t>(n, 1));
+    std::vector<double> start((size_t)m);
+    start_vector(m, kSvSeed, start.data());
+    eig.upload(start.data(), (size_t)m, stream);
+    const int gm = grid_for(m);
+    double tmp[2];
+    k_sumsq<<<gm, 256, 0, stream>>>((int)m, eig.p, part_v.p);
+    double nrm = std::sqrt(reduce_scalar_vec(gm, 1, tmp));
+    double sigma_sq = 1.0;
+    for (int it = 0; it < max_iterations; ++it) {
+      if (nrm == 0.0) break;
+      k_scale_div<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, nrm);
+      launch_pass<0>(At, Gather1<double>{nxt.p}, EpiStore<double>{dual.p}, nullptr, stream);
+      launch_pass<1>(A, Gather1<double>{dual.p}, EpiStoreDot{eig.p, nxt.p}, part_a.p, stream);
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, 1, sums.p + 56);
+      k_pi_resid<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, sums.p + 56, part_v.p);
+      reduce_scalar_vec(gm, 2, tmp);
+      sigma_sq = h_sums.p[56];
+      if (std::sqrt(tmp[0]) < tolerance) break;
+      nrm = std::sqrt(tmp[1]);
+    }
+    return std::sqrt(jmax(sigma_sq, 0.0));
+  }
+
+  // initialize_step_size_and_primal_weight!  src/pdfp.jl:220-241
+  void initialize(const pdfp_params& p, double pw) {
+    if (nnz == 0) {
+      step_size = 1.0;
+    } else {
+      if (std::isnan(sv)) sv = max_singular_value(p.sv_max_iterations, p.sv_tolerance);
+      step_size = 0.998 / std::sqrt(1.0 + sv * sv);
+    }
+    r.primal_weight = pw;
+    r.best_primal_weight = pw;
+    r.primal_weight_error_sum = 0.0;
+    r.primal_weight_last_error = 0.0;
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; ++j) acc += h_w[(size_t)j] * h_w[(size_t)j];
+    dual_den = 1.0 + std::sqrt(acc);
+    acc = 0.0;
+    for (int64_t j = 0; j < n; ++j)
+      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h_xt[(size_t)j]) : 0.0;
+    l1_den = 1.0 + acc;
+  }
+
+  // Scalars of one evaluation slot (see evaluate()).
+  struct Eval {
+    double fpe, rel_primal, rel_dual, rel_gap, l1, primal_dist, dual_dist;
+  };
+  Eval read_eval(int slot) const {
+    const double* S = h_sums.p + slot * 16;
+    Eval e;
+    const double pw = r.primal_weight;
+    const double movement = S[0] * pw + (S[1] + S[8]) / pw;
+    const double interaction = 2.0 * step_size * S[2];
+    e.fpe = std::sqrt(jmax(movement + interaction, 0.0));
+    const double bn = orig_primal ? b_norm_original : b_norm;
+    e.rel_primal = std::sqrt(S[9]) / (1.0 + bn);
+    e.rel_dual = std::sqrt(S[3]) / dual_den;
+    e.rel_gap = std::fabs(S[5] + S[10] - S[4]) / l1_den;
+    e.l1 = S[4];
+    e.primal_dist = std::sqrt(S[6]);
+    e.dual_dist = std::sqrt(S[7] + S[11]);
+    return e;
+  }
+
+  // pdfp_solve!  src/pdfp.jl:252-288
+  int solve(const pdfp_params& p, double primal_weight) override {
+    const double t_start = wall_seconds();
+    if (p.termination_evaluation_frequency < 1)
+      throw Error(PDFP_ERR_ARGUMENT, "termination_evaluation_frequency must be >= 1");
+    initialize(p, primal_weight);
+    const double omega = p.reflection_coefficient;
+    const int freq = p.termination_evaluation_frequency;
+    bool do_restart = false;
+    n_restarts = 0;
+    n_evals = 0;
+    x64_valid = false;
+    while (r.total_count < p.iteration_limit) {
+      upload_step_params(omega);
+      const bool restart_eval = do_restart;
+      step(1, restart_eval || freq == 1);
+      if (restart_eval) evaluate(1);
+      for (int i = 2; i <= freq; ++i) step(i, i == freq);
+      evaluate(0);
+      PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 32, hipMemcpyDeviceToHost,
+                              stream));
+      sync();
+      ++n_evals;
+      if (restart_eval) {
+        r.fixed_point_error = read_eval(1).fpe;
+        r.initial_fixed_point_error = r.fixed_point_error;
+        do_restart = false;
+      }
+      const Eval e = read_eval(0);
+      r.fixed_point_error = e.fpe;
+      r.relative_primal_residual = e.rel_primal;
+      r.relative_dual_residual = e.rel_dual;
+      relative_l1_gap = e.rel_gap;
+      l1_distance = e.l1;
+      r.inner_count += freq;
+      r.total_count += freq;
+      if (r.relative_primal_residual <= p.eps_feasible_relative &&
+          r.relative_dual_residual <= p.eps_optimal_relative &&
+          relative_l1_gap <= p.eps_optimal_relative) {
+        termination = PDFP_OPTIMAL;
+        break;
+      }
+      if (wall_seconds() - t_start > p.time_limit) {
+        termination = PDFP_TIME_LIMIT;
+        break;
+      }
+      do_restart = should_restart(r, p);
+      if (do_restart) {
+        // perform_restart!  src/pdfp.jl:180-192
+        update_primal_weight(r, p, e.primal_dist, e.dual_dist);
+        const int64_t len = std::max(n, m);
+        if (len)
+          k_restart<T><<<grid_for(len), 256, 0, stream>>>((int)n, (int)m, px.p, py.p, pz.p, x.p,
+                                                          x0.p, y.p, y0.p, z.p, z0.p);
+        r.inner_count = 0;
+        r.last_trial_fixed_point_error = INFINITY;
+        ++n_restarts;
+      }
+    }
+    if (termination == PDFP_UNSPECIFIED) termination = PDFP_ITERATION_LIMIT;
+    sync();
+    solve_seconds = wall_seconds() - t_start;
+    return termination;
+  }
+
+  void polish() {
+    if constexpr (kFused) {
+      if (x64_valid || n == 0) return;
+      const int g = grid_for(n);
+      k_polish<<<g, 256, 0, stream>>>((int)n, px.p, lb.p, ub.p, lb64.p, ub64.p, xt64.p, w64.p,
+                                      x64.p, part_v.p);
+      double l1;
+      reduce_scalar_vec(g, 1, &l1);
+      l1_distance = l1;
+      x64_valid = true;
+    }
+  }
+
+  void get_x(double* out) override {
+    if (kFused) {
+      polish();
+      if (n) x64.download(out, (size_t)n, stream);
+      sync();
+    } else {
+      download_T(px, out, n);
+    }
+  }
+
+  void get_vector(int which, double* out) override {
+    switch (which) {
+      case 0: get_x(out); break;
+      case 1: download_T(py, out, n); break;
+      case 2: download_T(pz, out, m); break;
+      case 3: download_T(x, out, n); break;
+      case 4: download_T(y, out, n); break;
+      case 5: download_T(z, out, m); break;
+      case 6: download_T(ds, out, n); break;
+      case 7: download_T(xt, out, n); break;
+      default: throw Error(PDFP_ERR_ARGUMENT, "unknown vector id");
+    }
+  }
+
+  void info(pdfp_info* o) override {
+    if (kFused && termination != PDFP_UNSPECIFIED) polish();
+    std::memset(o, 0, sizeof(*o));
+    o->relative_primal_residual = r.relative_primal_residual;
+    o->relative_dual_residual = r.relative_dual_residual;
+    o->relative_l1_gap = relative_l1_gap;
+    o->l1_distance = l1_distance;
+    o->fixed_point_error = r.fixed_point_error;
+    o->initial_fixed_point_error = r.initial_fixed_point_error;
+    o->primal_weight = r.primal_weight;
+    o->step_size = step_size;
+    o->max_singular_value = sv;
+    o->solve_seconds = solve_seconds;
+    o->total_count = r.total_count;
+    o->termination_reason = termination;
+    o->n_restarts = n_restarts;
+    o->precision = kFused ? PDFP_PREC_FP32_MIXED : PDFP_PREC_FP64;
+    o->n_evaluations = n_evals;
+  }
+
+  void spmv(int transpose, const double* xin, double* yout) override {
+    const DevCsr<T>& M = transpose ? At : A;
+    const int64_t nin = transpose ? m : n, nout = transpose ? n : m;
+    DevBuf<T> xd;
+    DevBuf<double> yd;
+    xd.alloc((size_t)std::max<int64_t>(nin, 1));
+    yd.alloc((size_t)std::max<int64_t>(nout, 1));
+    upload_T(xd, xin, nin);
+    launch_pass<0>(M, Gather1<T>{xd.p}, EpiStore<double>{yd.p}, nullptr, stream);
+    if (nout) yd.download(yout, (size_t)nout, stream);
+    sync();
+  }
+
+  double time_iterations(int iters) override {
+    if (std::isnan(sv)) {
+      pdfp_params p;
+      pdfp_default_params(&p);
+      initialize(p, r.primal_weight);
+    }
+    upload_step_params(1.0);
+    step(1, false);
+    sync();
+    hipEvent_t e0, e1;
+    PDFP_HIP(hipEventCreate(&e0));
+    PDFP_HIP(hipEventCreate(&e1));
+    PDFP_HIP(hipEventRecord(e0, stream));
+    for (int i = 0; i < iters; ++i) step(2 + (i % 198), false);
+    PDFP_HIP(hipEventRecord(e1, stream));
+    PDFP_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / std::max(iters, 1);
+  }
+
+  // Mean duration of the transpose pass and of the A pass, each bracketed by
+  // its own events on the context's stream (fused form: the two launches of
+  // an iteration; plain form: the two SpMVs).
+  void time_kernels(int iters, double* ms_at, double* ms_a) override {
+    if (std::isnan(sv)) {
+      pdfp_params p;
+      pdfp_default_params(&p);
+      initialize(p, r.primal_weight);
+    }
+    upload_step_params(1.0);
+    std::vector<hipEvent_t> ev((size_t)iters * 4);
+    for (auto& e : ev) PDFP_HIP(hipEventCreate(&e));
+    for (int i = 0; i < iters; ++i) {
+      const int k = 2 + (i % 198);
+      if constexpr (kFused) {
+        EpiPrimalY<T, false> ex{d_sp.p, k, x.p, x0.p, y.p, y0.p, lb.p, ub.p, xt.p, w.p,
+                                rx.p, px.p, ds.p, py.p, ry.p};
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
+        launch_pass<0>(At, Gather1<T>{z.p}, ex, nullptr, stream);
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
+        EpiDualZ<T, false> ez{d_sp.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
+        launch_pass<0>(A, Gather1<T>{rx.p}, ez, nullptr, stream);
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
+      } else {
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
+        launch_pass<0>(At, Gather1<double>{z.p}, EpiStore<double>{atz64.p}, nullptr, stream);
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
+        if (n) {
+          k_primal<<<grid_for(n), 256, 0, stream>>>(d_sp.p, k, (int)n, px.p, ds.p, rx.p, x.p,
+                                                    x0.p, y.p, atz64.p, lb.p, ub.p);
+          k_dual_y<<<grid_for(n), 256, 0, stream>>>(d_sp.p, k, (int)n, py.p, y.p, ry.p, y0.p,
+                                                    xt.p, rx.p, w.p);
+        }
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
+        launch_pass<0>(A, Gather1<double>{rx.p}, EpiStore<double>{ax64.p}, nullptr, stream);
+        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
+        if (m)
+          k_dual_z<<<grid_for(m), 256, 0, stream>>>(d_sp.p, k, (int)m, pz.p, z.p, rz.p, z0.p,
+                                                    ax64.p, b.p, is_eq.p);
+      }
+    }
+    sync();
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < iters; ++i) {
+      float t0 = 0.f, t1 = 0.f;
+      PDFP_HIP(hipEventElapsedTime(&t0, ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]));
+      PDFP_HIP(hipEventElapsedTime(&t1, ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]));
+      sa += t0;
+      sb += t1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms_at = sa / std::max(iters, 1);
+    *ms_a = sb / std::max(iters, 1);
+  }
+};
+
+}  // namespace pdfp
+
+// ---- C ABI ---------------------------------------------------------------------
+
+struct pdfp_ctx {
+  std::unique_ptr<pdfp::ProjectorBase> impl;
+  std::string err = "no error";
+  bool target_set = false;
+};
+
+namespace {
+
+template <class F>
+int guarded(pdfp_ctx* ctx, F&& f) {
+  if (!ctx) return PDFP_ERR_ARGUMENT;
+  try {
+    f();
+    return PDFP_OK;
+  } catch (const pdfp::Error& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "host allocation failed";
+    return PDFP_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return PDFP_ERR_DEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void pdfp_default_params(pdfp_params* p) {
+  if (!p) return;
+  p->eps_optimal_relative = 1e-4;
+  p->eps_feasible_relative = 1e-4;
+  p->iteration_limit = 100000;
+  p->time_limit = 3600.0;
+  p->termination_evaluation_frequency = 200;
+  p->sv_max_iterations = 5000;
+  p->reflection_coefficient = 1.0;
+  p->sv_tolerance = 1e-4;
+  p->artificial_restart_threshold = 0.36;
+  p->sufficient_reduction_for_restart = 0.2;
+  p->necessary_reduction_for_restart = 0.5;
+  p->k_p = 0.99;
+  p->k_i = 0.01;
+  p->k_d = 0.0;
+  p->i_smooth = 0.3;
+}
+
+const char* pdfp_last_error(const pdfp_ctx* ctx) {
+  if (!ctx) return pdfp::g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+int pdfp_create(pdfp_ctx** out, int64_t m, int64_t n, int64_t nnz, const int64_t* colptr,
+                const int64_t* rowval, const double* nzval, int32_t index_base, const double* b,
+                const uint8_t* is_eq, const uint8_t* is_int, const double* var_lb,
+                const double* var_ub, const double* w, int32_t precision, int32_t device) {
+  if (!out) return PDFP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto fail = [&](int code, const std::string& msg) {
+    pdfp::g_create_error = msg;
+    return code;
+  };
+  if (m < 0 || n < 0 || nnz < 0) return fail(PDFP_ERR_ARGUMENT, "negative size");
+  if (!colptr || (nnz > 0 && (!rowval || !nzval)) || (m > 0 && (!b || !is_eq)) ||
+      (n > 0 && (!is_int || !var_lb || !var_ub || !w)))
+    return fail(PDFP_ERR_ARGUMENT, "null input array");
+  if (index_base != 0 && index_base != 1) return fail(PDFP_ERR_ARGUMENT, "index_base must be 0 or 1");
+  if (colptr[n] - index_base != nnz) return fail(PDFP_ERR_ARGUMENT, "colptr[n] does not match nnz");
+  if (precision != PDFP_PREC_FP64 && precision != PDFP_PREC_FP32_MIXED)
+    return fail(PDFP_ERR_ARGUMENT, "unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(PDFP_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(PDFP_ERR_ARGUMENT, "device index out of range");
+  auto* ctx = new (std::nothrow) pdfp_ctx();
+  if (!ctx) return fail(PDFP_ERR_NOMEM, "host allocation failed");
+  try {
+    if (precision == PDFP_PREC_FP64)
+      ctx->impl.reset(new pdfp::Projector<double>(m, n, colptr, rowval, nzval, index_base, b,
+                                                  is_eq, is_int, var_lb, var_ub, w, device));
+    else
+      ctx->impl.reset(new pdfp::Projector<float>(m, n, colptr, rowval, nzval, index_base, b,
+                                                 is_eq, is_int, var_lb, var_ub, w, device));
+  } catch (const pdfp::Error& e) {
+    delete ctx;
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    delete ctx;
+    return fail(PDFP_ERR_DEVICE, e.what());
+  }
+  *out = ctx;
+  return PDFP_OK;
+}
+
+int pdfp_set_original_primal_residual(pdfp_ctx* ctx, const double* D_r) {
+  return guarded(ctx, [&] {
+    if (!D_r) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null D_r");
+    ctx->impl->set_original_primal_residual(D_r);
+  });
+}
+
+int pdfp_set_x(pdfp_ctx* ctx, const double* x) {
+  return guarded(ctx, [&] {
+    if (!x) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null x");
+    ctx->impl->set_x(x);
+  });
+}
+
+int pdfp_set_target(pdfp_ctx* ctx, const double* xt) {
+  return guarded(ctx, [&] {
+    if (!xt) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null target");
+    ctx->impl->set_target(xt);
+    ctx->target_set = true;
+  });
+}
+
+int pdfp_warm_start(pdfp_ctx* ctx) {
+  return guarded(ctx, [&] { ctx->impl->warm_start(); });
+}
+
+int pdfp_solve(pdfp_ctx* ctx, const pdfp_params* params, double primal_weight,
+               int32_t* termination, int64_t* total_count) {
+  return guarded(ctx, [&] {
+    pdfp_params p;
+    if (params) p = *params;
+    else pdfp_default_params(&p);
+    if (!(primal_weight > 0.0)) throw pdfp::Error(PDFP_ERR_ARGUMENT, "primal_weight must be > 0");
+    const int t = ctx->impl->solve(p, primal_weight);
+    pdfp_info info;
+    ctx->impl->info(&info);
+    if (termination) *termination = t;
+    if (total_count) *total_count = info.total_count;
+  });
+}
+
+int pdfp_get_x(pdfp_ctx* ctx, double* out) {
+  return guarded(ctx, [&] {
+    if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    ctx->impl->get_x(out);
+  });
+}
+
+int pdfp_get_vector(pdfp_ctx* ctx, int32_t which, double* out) {
+  return guarded(ctx, [&] {
+    if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    ctx->impl->get_vector(which, out);
+  });
+}
+
+int pdfp_get_info(pdfp_ctx* ctx, pdfp_info* out) {
+  return guarded(ctx, [&] {
+    if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    ctx->impl->info(out);
+  });
+}
+
+void pdfp_destroy(pdfp_ctx* ctx) { delete ctx; }
+
+int pdfp_max_singular_value(pdfp_ctx* ctx, int32_t max_iterations, double tolerance,
+                            double* sigma_out) {
+  return guarded(ctx, [&] {
+    if (!sigma_out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    *sigma_out = ctx->impl->max_singular_value(max_iterations, tolerance);
+  });
+}
+
+void pdfp_start_vector(int64_t m, uint64_t seed, double* out) {
+  if (out && m > 0) pdfp::start_vector(m, seed, out);
+}
+
+int pdfp_spmv(pdfp_ctx* ctx, int32_t transpose, const double* x, double* y) {
+  return guarded(ctx, [&] {
+    if (!x || !y) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null vector");
+    ctx->impl->spmv(transpose, x, y);
+  });
+}
+
+int pdfp_time_iterations(pdfp_ctx* ctx, int32_t iterations, double* ms_out) {
+  return guarded(ctx, [&] {
+    if (!ms_out || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
+    *ms_out = ctx->impl->time_iterations(iterations);
+  });
+}
+
+int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at, double* ms_a) {
+  return guarded(ctx, [&] {
+    if (!ms_at || !ms_a || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
+    ctx->impl->time_kernels(iterations, ms_at, ms_a);
+  });
+}
+
+int64_t pdfp_device_bytes(const pdfp_ctx* ctx) {
+  return ctx && ctx->impl ? ctx->impl->device_bytes() : 0;
+}
+
+}  // extern "C"

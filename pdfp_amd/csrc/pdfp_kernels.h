@@ -1,0 +1,376 @@
+// Device kernels and fused epilogues of the PDFP projector.
+//
+// The saddle-point form of the pump's ℓ1 projection (src/pdfp.jl:1-13):
+//   min_{ℓ≤x≤u} max_{y,z}  y'(x̃ - x) + z'(A x - b),
+//   |y_j| ≤ w_j (j ∈ I), y_j = 0 (j ∉ I), z_i ≥ 0 on ≤ rows.
+// One restarted Halpern PDHG iteration with reflection (src/pdfp.jl:119-163):
+//   x⁺ = clamp(x - τ(Aᵀz - y), ℓ, u)                     (primal prox: box)
+//   y⁺ = clamp(y + σ(x̃ - x̄), -w, w) on I, 0 elsewhere   (ℓ1 prox, dual form)
+//   z⁺ = z + σ(A x̄ - b), projected to z ≥ 0 on ≤ rows
+// with x̄ = 2x⁺ - x, every block reflected and Halpern-averaged toward the
+// anchor: v ← w_k(ω v̄ + (1-ω) v) + (1 - w_k) v⁰, w_k = k/(k+1).
+//
+// Non-integer columns carry w_j = 0 in device memory (the reference's is_int
+// mask folded into the weight: y_j = 0 exactly where is_int is false).
+#pragma once
+
+#include "engine.h"
+
+namespace pdfp {
+
+// Scalars of the current evaluation block, in device memory so that a
+// captured graph of the block stays valid when they change.
+struct StepParams {
+  double step_size;
+  double primal_weight;
+  double omega;        // reflection coefficient
+  double inner_count;  // iterations since the last restart (exact in FP64)
+};
+
+struct StepScalars {
+  double tau, sigma, weight, omega, omw, omwt;
+};
+
+// τ = step/pw, σ = step·pw, k = inner_count + k_offset, w_k = k/(k+1)
+// (src/pdfp.jl:120-122, 135-137, 147-149) — the same FP64 operations.
+__device__ __forceinline__ StepScalars step_scalars(const StepParams* __restrict__ sp,
+                                                    int k_offset) {
+  StepScalars s;
+  s.tau = sp->step_size / sp->primal_weight;
+  s.sigma = sp->step_size * sp->primal_weight;
+  const double k = sp->inner_count + (double)k_offset;
+  s.weight = k / (k + 1.0);
+  s.omega = sp->omega;
+  s.omw = 1.0 - s.omega;
+  s.omwt = 1.0 - s.weight;
+  return s;
+}
+
+// Julia's clamp(x, lo, hi) = x > hi ? hi : (x < lo ? lo : x).
+__device__ __forceinline__ double jclamp(double x, double lo, double hi) {
+  return x > hi ? hi : (x < lo ? lo : x);
+}
+
+// ---- fused epilogues (precision-generic iterates T, FP64 arithmetic) ------
+
+// Column j of A (row j of Aᵀ): s[0] = (Aᵀz)_j.  Fuses K1 (primal step,
+// pdfp_kernels.jl:4-20) and K2 (dual-y step, pdfp_kernels.jl:33-48).
+// MAJOR steps also store pdhg_x, dual_slack, pdhg_y and reflected_y, which
+// only the evaluation and the restart read.
+template <typename T, bool MAJOR>
+struct EpiPrimalY {
+  const StepParams* sp;
+  int k_offset;
+  T* __restrict__ x;
+  const T* __restrict__ x0;
+  T* __restrict__ y;
+  const T* __restrict__ y0;
+  const T* __restrict__ lb;
+  const T* __restrict__ ub;
+  const T* __restrict__ xt;
+  const T* __restrict__ w;
+  T* __restrict__ rx;
+  T* __restrict__ px;
+  T* __restrict__ ds;
+  T* __restrict__ py;
+  T* __restrict__ ry;
+  __device__ __forceinline__ void operator()(int j, const double* s, double*) const {
+    const StepScalars c = step_scalars(sp, k_offset);
+    const double cx = (double)x[j];
+    const double cy = (double)y[j];
+    const double temp = cx - c.tau * (s[0] - cy);
+    const double pxj = jclamp(temp, (double)lb[j], (double)ub[j]);
+    const double rxj = 2.0 * pxj - cx;
+    x[j] = (T)(c.weight * (c.omega * rxj + c.omw * cx) + c.omwt * (double)x0[j]);
+    rx[j] = (T)rxj;
+    const double wj = (double)w[j];
+    const double pyj = wj > 0.0 ? jclamp(cy + c.sigma * ((double)xt[j] - rxj), -wj, wj) : 0.0;
+    const double ryj = 2.0 * pyj - cy;
+    y[j] = (T)(c.weight * (c.omega * ryj + c.omw * cy) + c.omwt * (double)y0[j]);
+    if (MAJOR) {
+      px[j] = (T)pxj;
+      ds[j] = (T)((pxj - temp) / c.tau);
+      py[j] = (T)pyj;
+      ry[j] = (T)ryj;
+    }
+  }
+};
+
+// Row i of A: s[0] = (A x̄)_i.  Fuses K3 (dual-z step, pdfp_kernels.jl:60-75).
+template <typename T, bool MAJOR>
+struct EpiDualZ {
+  const StepParams* sp;
+  int k_offset;
+  T* __restrict__ z;
+  const T* __restrict__ z0;
+  const T* __restrict__ b;
+  const uint8_t* __restrict__ is_eq;
+  T* __restrict__ pz;
+  T* __restrict__ rz;
+  __device__ __forceinline__ void operator()(int i, const double* s, double*) const {
+    const StepScalars c = step_scalars(sp, k_offset);
+    const double cz = (double)z[i];
+    const double raw = cz + c.sigma * (s[0] - (double)b[i]);
+    const double pzi = is_eq[i] ? raw : (raw > 0.0 ? raw : 0.0);
+    const double rzi = 2.0 * pzi - cz;
+    z[i] = (T)(c.weight * (c.omega * rzi + c.omw * cz) + c.omwt * (double)z0[i]);
+    if (MAJOR) {
+      pz[i] = (T)pzi;
+      rz[i] = (T)rzi;
+    }
+  }
+};
+
+// out[row] = s[0]
+template <typename TO>
+struct EpiStore {
+  TO* __restrict__ out;
+  __device__ __forceinline__ void operator()(int r, const double* s, double*) const {
+    out[r] = (TO)s[0];
+  }
+};
+
+// out[row] = s[0];  acc[0] += v[row] * s[0]   (power iteration: eig = A d, ⟨nxt, eig⟩)
+struct EpiStoreDot {
+  double* __restrict__ out;
+  const double* __restrict__ v;
+  __device__ __forceinline__ void operator()(int r, const double* s, double* acc) const {
+    out[r] = s[0];
+    acc[0] += v[r] * s[0];
+  }
+};
+
+// ---- evaluation epilogues (FP64 reductions) ---------------------------------
+// Column pass over Aᵀ with GatherDelta(pdhg_z, reflected_z):
+//   s[0] = (Aᵀ pdhg_z)_j, s[1] = (Aᵀ δz)_j, δ = pdhg - reflected.
+// Terms: 0 ‖δx‖², 1 ‖δy‖², 2 ⟨Aᵀδz - δy, δx⟩        (src/pdfp.jl:167-178)
+//        3 ‖Aᵀz - y - dual_slack‖²                    (src/pdfp.jl:207-209)
+//        4 Σ_I w|x - x̃|, 5 Σ_I y(x̃ - x)               (src/pdfp.jl:211-214)
+//        6 ‖x⁰ - x‖², 7 ‖y⁰ - y‖²                     (src/pdfp.jl:181-185)
+constexpr int kEvalColTerms = 8;
+template <typename T>
+struct EpiEvalCol {
+  const T* __restrict__ px;
+  const T* __restrict__ rx;
+  const T* __restrict__ py;
+  const T* __restrict__ ry;
+  const T* __restrict__ ds;
+  const T* __restrict__ xt;
+  const T* __restrict__ w;
+  const T* __restrict__ x0;
+  const T* __restrict__ y0;
+  __device__ __forceinline__ void operator()(int j, const double* s, double* acc) const {
+    const double pxj = (double)px[j], pyj = (double)py[j];
+    const double dx = pxj - (double)rx[j];
+    const double dy = pyj - (double)ry[j];
+    acc[0] += dx * dx;
+    acc[1] += dy * dy;
+    acc[2] += (s[1] - dy) * dx;
+    const double r = s[0] - pyj - (double)ds[j];
+    acc[3] += r * r;
+    const double wj = (double)w[j];
+    if (wj > 0.0) {
+      const double xtj = (double)xt[j];
+      acc[4] += wj * fabs(pxj - xtj);
+      acc[5] += pyj * (xtj - pxj);
+    }
+    const double ex = (double)x0[j] - pxj;
+    const double ey = (double)y0[j] - pyj;
+    acc[6] += ex * ex;
+    acc[7] += ey * ey;
+  }
+};
+
+// Row pass over A with Gather1(pdhg_x): s[0] = (A pdhg_x)_i.
+// Terms: 0 ‖δz‖², 1 ‖viol(Ax - b)‖² (scaled by 1/D_r when the primal residual
+//        is measured in original units), 2 ⟨z, Ax - b⟩, 3 ‖z⁰ - z‖².
+constexpr int kEvalRowTerms = 4;
+template <typename T>
+struct EpiEvalRow {
+  const T* __restrict__ pz;
+  const T* __restrict__ rz;
+  const T* __restrict__ b;
+  const T* __restrict__ z0;
+  const uint8_t* __restrict__ is_eq;
+  const double* __restrict__ row_unscale;  // null: scaled rows
+  __device__ __forceinline__ void operator()(int i, const double* s, double* acc) const {
+    const double pzi = (double)pz[i];
+    const double dz = pzi - (double)rz[i];
+    acc[0] += dz * dz;
+    const double d = s[0] - (double)b[i];
+    double r = is_eq[i] ? fabs(d) : (d > 0.0 ? d : 0.0);
+    if (row_unscale) r *= row_unscale[i];
+    acc[1] += r * r;
+    acc[2] += pzi * d;
+    const double ez = (double)z0[i] - pzi;
+    acc[3] += ez * ez;
+  }
+};
+
+// Primal residual of an FP64 point (the polished projection).
+template <typename T>
+struct EpiViolation {
+  const T* __restrict__ b;
+  const uint8_t* __restrict__ is_eq;
+  const double* __restrict__ row_unscale;
+  __device__ __forceinline__ void operator()(int i, const double* s, double* acc) const {
+    const double d = s[0] - (double)b[i];
+    double r = is_eq[i] ? fabs(d) : (d > 0.0 ? d : 0.0);
+    if (row_unscale) r *= row_unscale[i];
+    acc[0] += r * r;
+  }
+};
+
+// ---- element-wise kernels of the plain FP64 form (the reference's K1-K3) ---
+
+// K1, src/pdfp_kernels.jl:4-20
+__global__ __launch_bounds__(256) void k_primal(const StepParams* __restrict__ sp, int k_offset,
+                                                int n, double* __restrict__ px,
+                                                double* __restrict__ ds, double* __restrict__ rx,
+                                                double* __restrict__ x,
+                                                const double* __restrict__ x0,
+                                                const double* __restrict__ y,
+                                                const double* __restrict__ atz,
+                                                const double* __restrict__ lb,
+                                                const double* __restrict__ ub) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const StepScalars c = step_scalars(sp, k_offset);
+  const double cx = x[j];
+  const double temp = cx - c.tau * (atz[j] - y[j]);
+  const double p = jclamp(temp, lb[j], ub[j]);
+  px[j] = p;
+  ds[j] = (p - temp) / c.tau;
+  const double r = 2.0 * p - cx;
+  rx[j] = r;
+  x[j] = c.weight * (c.omega * r + c.omw * cx) + c.omwt * x0[j];
+}
+
+// K2, src/pdfp_kernels.jl:33-48 (w_j = 0 off I)
+__global__ __launch_bounds__(256) void k_dual_y(const StepParams* __restrict__ sp, int k_offset,
+                                                int n, double* __restrict__ py,
+                                                double* __restrict__ y, double* __restrict__ ry,
+                                                const double* __restrict__ y0,
+                                                const double* __restrict__ xt,
+                                                const double* __restrict__ rx,
+                                                const double* __restrict__ w) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const StepScalars c = step_scalars(sp, k_offset);
+  const double cy = y[j];
+  const double wj = w[j];
+  const double v = wj > 0.0 ? jclamp(cy + c.sigma * (xt[j] - rx[j]), -wj, wj) : 0.0;
+  py[j] = v;
+  const double r = 2.0 * v - cy;
+  ry[j] = r;
+  y[j] = c.weight * (c.omega * r + c.omw * cy) + c.omwt * y0[j];
+}
+
+// K3, src/pdfp_kernels.jl:60-75
+__global__ __launch_bounds__(256) void k_dual_z(const StepParams* __restrict__ sp, int k_offset,
+                                                int m, double* __restrict__ pz,
+                                                double* __restrict__ z, double* __restrict__ rz,
+                                                const double* __restrict__ z0,
+                                                const double* __restrict__ ax,
+                                                const double* __restrict__ b,
+                                                const uint8_t* __restrict__ is_eq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const StepScalars c = step_scalars(sp, k_offset);
+  const double cz = z[i];
+  const double raw = cz + c.sigma * (ax[i] - b[i]);
+  const double v = is_eq[i] ? raw : (raw > 0.0 ? raw : 0.0);
+  pz[i] = v;
+  const double r = 2.0 * v - cz;
+  rz[i] = r;
+  z[i] = c.weight * (c.omega * r + c.omw * cz) + c.omwt * z0[i];
+}
+
+// ---- restart / warm start / polish / conversions ----------------------------
+
+// perform_restart!: initial ← pdhg, current ← pdhg for x, y (n) and z (m)
+// (src/pdfp.jl:186-188).
+template <typename T>
+__global__ __launch_bounds__(256) void k_restart(int n, int m, const T* __restrict__ px,
+                                                 const T* __restrict__ py,
+                                                 const T* __restrict__ pz, T* __restrict__ x,
+                                                 T* __restrict__ x0, T* __restrict__ y,
+                                                 T* __restrict__ y0, T* __restrict__ z,
+                                                 T* __restrict__ z0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const T a = px[i], c = py[i];
+    x[i] = a; x0[i] = a;
+    y[i] = c; y0[i] = c;
+  }
+  if (i < m) {
+    const T a = pz[i];
+    z[i] = a; z0[i] = a;
+  }
+}
+
+template <typename TO, typename TI>
+__global__ __launch_bounds__(256) void k_convert(int n, TO* __restrict__ out,
+                                                 const TI* __restrict__ in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (TO)in[i];
+}
+
+// FP64 polish of an FP32 projection: a coordinate that the primal prox
+// clamped to a bound gets that bound's FP64 value (x = D_c·clamp(x̂, l/D_c,
+// u/D_c) then holds exactly as in FP64); interior coordinates are widened.
+// acc[0] = Σ_I w|x - x̃| of the polished point, in FP64.
+__global__ __launch_bounds__(256) void k_polish(int n, const float* __restrict__ px,
+                                                const float* __restrict__ lb,
+                                                const float* __restrict__ ub,
+                                                const double* __restrict__ lb64,
+                                                const double* __restrict__ ub64,
+                                                const double* __restrict__ xt64,
+                                                const double* __restrict__ w64,
+                                                double* __restrict__ x64,
+                                                double* __restrict__ partials) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (j < n) {
+    const float p = px[j];
+    const double v = p <= lb[j] ? lb64[j] : (p >= ub[j] ? ub64[j] : (double)p);
+    x64[j] = v;
+    const double wj = w64[j];
+    if (wj > 0.0) acc[0] = wj * fabs(v - xt64[j]);
+  }
+  block_store_partials<256, 1>(acc, partials + blockIdx.x);
+}
+
+// Power iteration helpers (src/preconditioner.jl:113-123).
+__global__ __launch_bounds__(256) void k_scale_div(int m, double* __restrict__ out,
+                                                   const double* __restrict__ in, double nrm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = in[i] / nrm;
+}
+
+// nxt = eig - σ² nxt;  acc0 = ‖nxt‖², acc1 = ‖eig‖²  (σ² read from device)
+__global__ __launch_bounds__(256) void k_pi_resid(int m, double* __restrict__ nxt,
+                                                  const double* __restrict__ eig,
+                                                  const double* __restrict__ sigma_sq,
+                                                  double* __restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < m) {
+    const double e = eig[i];
+    const double v = e - sigma_sq[0] * nxt[i];
+    nxt[i] = v;
+    acc[0] = v * v;
+    acc[1] = e * e;
+  }
+  block_store_partials<256, 2>(acc, partials + (size_t)blockIdx.x * 2);
+}
+
+__global__ __launch_bounds__(256) void k_sumsq(int m, const double* __restrict__ v,
+                                               double* __restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < m) acc[0] = v[i] * v[i];
+  block_store_partials<256, 1>(acc, partials + blockIdx.x);
+}
+
+}  // namespace pdfp

@@ -1,0 +1,90 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point that
+include/pdfp_hip.h declares, and its host-only logic (defaults, argument
+validation, the power-iteration start vector) behaves.  No device compute."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pdfp_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "pdfp_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b((?:pdfp|pdlp)_[a-z0-9_]+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, name
+
+
+def test_default_params_are_the_reference_defaults():
+    """src/pdlp.jl:10-31."""
+    p = _lib.PdfpParams()
+    _lib.load().pdfp_default_params(C.byref(p))
+    assert (p.eps_optimal_relative, p.eps_feasible_relative) == (1e-4, 1e-4)
+    assert p.iteration_limit == 100_000 and p.time_limit == 3600.0
+    assert p.termination_evaluation_frequency == 200 and p.reflection_coefficient == 1.0
+    assert p.sv_max_iterations == 5000 and p.sv_tolerance == 1e-4
+    assert (p.artificial_restart_threshold, p.sufficient_reduction_for_restart,
+            p.necessary_reduction_for_restart) == (0.36, 0.2, 0.5)
+    assert (p.k_p, p.k_i, p.k_d, p.i_smooth) == (0.99, 0.01, 0.0, 0.3)
+
+
+def test_start_vector_matches_oracle_bitwise():
+    lib = _lib.load()
+    for m in (1, 7, 1000):
+        out = np.empty(m)
+        lib.pdfp_start_vector(m, 20240517, _lib.dptr(out))
+        assert np.array_equal(out, O.start_vector(m))
+
+
+def test_create_rejects_bad_arguments_before_touching_a_device():
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    colptr = np.array([0, 1, 2], dtype=np.int64)
+    rowval = np.array([0, 0], dtype=np.int64)
+    vals = np.array([1.0, 1.0])
+    b = np.array([1.0])
+    eq = np.zeros(1, dtype=np.uint8)
+    ii = np.ones(2, dtype=np.uint8)
+    lb, ub, w = np.zeros(2), np.ones(2), np.ones(2)
+    args = [C.byref(ctx), 1, 2, 2, _lib.i64ptr(colptr), _lib.i64ptr(rowval), _lib.dptr(vals), 0,
+            _lib.dptr(b), _lib.u8ptr(eq), _lib.u8ptr(ii), _lib.dptr(lb), _lib.dptr(ub),
+            _lib.dptr(w), 0, 0]
+    bad = list(args)
+    bad[3] = 3                                  # nnz disagrees with colptr
+    assert lib.pdfp_create(*bad) == 1
+    assert b"nnz" in lib.pdfp_last_error(None)
+    bad = list(args)
+    bad[14] = 7                                 # unknown precision
+    assert lib.pdfp_create(*bad) == 1
+    bad = list(args)
+    bad[7] = 2                                  # index base
+    assert lib.pdfp_create(*bad) == 1
+    bad = list(args)
+    bad[8] = None                               # null b
+    assert lib.pdfp_create(*bad) == 1
+    assert not ctx.value
+    # calls on a null context fail cleanly
+    assert lib.pdfp_warm_start(None) == 1
+    lib.pdfp_destroy(None)
+
+
+def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(ImportError):
+        _lib.load()
