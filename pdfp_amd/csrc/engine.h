@@ -2,7 +2,7 @@
 // in a precomputed row-block schedule, forms each row's inner product with a
 // gathered vector (or two), and hands the sums to a fused per-row epilogue.
 //
-// Schedule (built on the host, see schedule.cpp): consecutive rows are packed
+// Schedule (built on the host by row_blocks in host_util.h): consecutive rows are packed
 // into a block while the block holds at most BLOCK rows and at most NPB
 // nonzeros.  A row longer than NPB sits alone in its block.
 //
@@ -78,7 +78,15 @@ __device__ __forceinline__ double block_sum_to0(double v, double* red) {
   return s;
 }
 
-template <int BLOCK, int NACC>
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Writes the block's NACC partials: terms [0, NACC-NMAX) are sums, the last
+// NMAX terms are maxima (of nonnegative values).
+template <int BLOCK, int NACC, int NMAX = 0>
 __device__ __forceinline__ void block_store_partials(const double (&acc)[NACC],
                                                      double* __restrict__ out) {
   constexpr int kW = BLOCK / kWave;
@@ -87,14 +95,15 @@ __device__ __forceinline__ void block_store_partials(const double (&acc)[NACC],
   const int wid = threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < NACC; ++k) {
-    const double v = wave_sum(acc[k]);
+    const double v = k < NACC - NMAX ? wave_sum(acc[k]) : wave_max(acc[k]);
     if (lane == 0) red[wid][k] = v;
   }
   __syncthreads();
   if (threadIdx.x < NACC) {
+    const bool is_max = (int)threadIdx.x >= NACC - NMAX;
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kW; ++w) s += red[w][threadIdx.x];
+    for (int w = 0; w < kW; ++w) s = is_max ? fmax(s, red[w][threadIdx.x]) : s + red[w][threadIdx.x];
     out[threadIdx.x] = s;
   }
 }
@@ -127,7 +136,7 @@ struct GatherDelta {
 // ---- the pass ---------------------------------------------------------------
 
 // Epi: __device__ void operator()(int row, const double* sums, double* acc)
-template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC>
+template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX = 0>
 __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
                                                   double* __restrict__ partials) {
   constexpr int NV = Gather::NV;
@@ -219,21 +228,36 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
     if (tid == 0) epi(r0, t, acc);
   }
   if constexpr (NACC > 0) {
-    block_store_partials<BLOCK, NACC>(acc, partials + (size_t)b * NACC);
+    block_store_partials<BLOCK, NACC, NMAX>(acc, partials + (size_t)b * NACC);
   }
 }
 
-// Adds per-block partials (nb blocks × nacc terms) in block order into
-// out[0..nacc).  One workgroup.
+// Combines per-block partials (nb blocks × nacc terms) in block order into
+// out[0..nacc): terms [0, nacc-nmax) are added, the last nmax are maxima.
+// One workgroup.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void finalize_partials(const double* __restrict__ partials,
-                                                           int nb, int nacc,
+                                                           int nb, int nacc, int nmax,
                                                            double* __restrict__ out) {
   __shared__ double red[BLOCK / kWave];
+  constexpr int kW = BLOCK / kWave;
   for (int k = 0; k < nacc; ++k) {
+    const bool is_max = k >= nacc - nmax;
     double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BLOCK) s += partials[(size_t)b * nacc + k];
-    s = block_sum_to0<BLOCK>(s, red);
+    for (int b = threadIdx.x; b < nb; b += BLOCK) {
+      const double v = partials[(size_t)b * nacc + k];
+      s = is_max ? fmax(s, v) : s + v;
+    }
+    if (!is_max) {
+      s = block_sum_to0<BLOCK>(s, red);
+    } else {
+      s = wave_max(s);
+      __syncthreads();
+      if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s;
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int w = 0; w < kW; ++w) s = fmax(s, red[w]);
+    }
     if (threadIdx.x == 0) out[k] = s;
   }
 }

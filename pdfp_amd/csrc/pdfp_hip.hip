@@ -22,135 +22,11 @@
 #include "engine.h"
 #include "host_util.h"
 #include "pdfp_kernels.h"
+#include "solver_common.h"
 
 namespace pdfp {
 
-constexpr int kBlock = 256;  // threads per workgroup of the CSR passes
-constexpr int kNpb = 2048;   // nonzeros staged per workgroup
-constexpr uint64_t kSvSeed = 20240517ull;
-constexpr double kJuliaEps = 2.220446049250313e-16;
-
 std::string g_create_error = "no error";
-
-// ---- restart logic shared with the root LP (src/pdlp.jl:167-205) ---------
-
-struct RestartState {
-  double primal_weight = 1.0, best_primal_weight = 1.0;
-  double best_primal_dual_residual_gap = INFINITY;
-  double primal_weight_error_sum = 0.0, primal_weight_last_error = 0.0;
-  double fixed_point_error = INFINITY, initial_fixed_point_error = INFINITY;
-  double last_trial_fixed_point_error = INFINITY;
-  int64_t inner_count = 0, total_count = 0;
-  double relative_primal_residual = INFINITY, relative_dual_residual = INFINITY;
-};
-
-// should_restart  src/pdlp.jl:167-181
-inline bool should_restart(RestartState& s, const pdfp_params& p) {
-  bool restart = false;
-  if (s.total_count == p.termination_evaluation_frequency) {
-    restart = true;
-  } else if (s.total_count > p.termination_evaluation_frequency) {
-    if (s.fixed_point_error <= p.sufficient_reduction_for_restart * s.initial_fixed_point_error)
-      restart = true;
-    if (s.fixed_point_error <= p.necessary_reduction_for_restart * s.initial_fixed_point_error &&
-        s.fixed_point_error > s.last_trial_fixed_point_error)
-      restart = true;
-    if ((double)s.inner_count >= p.artificial_restart_threshold * (double)s.total_count)
-      restart = true;
-  }
-  s.last_trial_fixed_point_error = s.fixed_point_error;
-  return restart;
-}
-
-inline double jmax(double a, double b) {
-  if (std::isnan(a) || std::isnan(b)) return NAN;
-  return a > b ? a : b;
-}
-
-// update_primal_weight!  src/pdlp.jl:185-205
-inline void update_primal_weight(RestartState& s, const pdfp_params& rp, double primal_dist,
-                                 double dual_dist) {
-  const double ratio = s.relative_dual_residual / jmax(s.relative_primal_residual, kJuliaEps);
-  if (1e-16 < primal_dist && primal_dist < 1e12 && 1e-16 < dual_dist && dual_dist < 1e12 &&
-      1e-8 < ratio && ratio < 1e8) {
-    const double err = std::log(dual_dist) - std::log(primal_dist) - std::log(s.primal_weight);
-    s.primal_weight_error_sum *= rp.i_smooth;
-    s.primal_weight_error_sum += err;
-    const double delta_err = err - s.primal_weight_last_error;
-    s.primal_weight *=
-        std::exp(rp.k_p * err + rp.k_i * s.primal_weight_error_sum + rp.k_d * delta_err);
-    s.primal_weight_last_error = err;
-  } else {
-    s.primal_weight = s.best_primal_weight;
-    s.primal_weight_error_sum = 0.0;
-    s.primal_weight_last_error = 0.0;
-  }
-  const double gap =
-      std::fabs(std::log10(s.relative_dual_residual / jmax(s.relative_primal_residual, kJuliaEps)));
-  if (gap < s.best_primal_dual_residual_gap) {
-    s.best_primal_dual_residual_gap = gap;
-    s.best_primal_weight = s.primal_weight;
-  }
-}
-
-// SplitMix64 + Box–Muller normals: the fixed start of the power iteration.
-inline uint64_t splitmix64(uint64_t& s) {
-  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-inline void start_vector(int64_t m, uint64_t seed, double* out) {
-  uint64_t s = seed;
-  const double two_pi = 6.283185307179586476925286766559;
-  for (int64_t i = 0; i < m; i += 2) {
-    const double u1 = ((double)(splitmix64(s) >> 11) + 1.0) * 0x1.0p-53;
-    const double u2 = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
-    const double r = std::sqrt(-2.0 * std::log(u1));
-    out[i] = r * std::cos(two_pi * u2);
-    if (i + 1 < m) out[i + 1] = r * std::sin(two_pi * u2);
-  }
-}
-
-inline int grid_for(int64_t n) { return (int)((n + 255) / 256); }
-
-// Device CSR + schedule of one operator (A or Aᵀ).
-template <typename TV>
-struct DevCsr {
-  DevBuf<int32_t> rowptr, col, blocks;
-  DevBuf<TV> val;
-  int32_t nblocks = 0, nrows = 0;
-  int64_t nnz = 0;
-  void upload(const HostCsr& h, hipStream_t s) {
-    nrows = (int32_t)h.nrows;
-    nnz = (int64_t)h.col.size();
-    std::vector<int32_t> blk = row_blocks(h.rowptr, kBlock, kNpb);
-    nblocks = (int32_t)blk.size() - 1;
-    rowptr.alloc(h.rowptr.size());
-    rowptr.upload(h.rowptr.data(), h.rowptr.size(), s);
-    col.alloc(std::max<size_t>(h.col.size(), 1));
-    col.upload(h.col.data(), h.col.size(), s);
-    std::vector<TV> v(h.val.size());
-    for (size_t k = 0; k < v.size(); ++k) v[k] = (TV)h.val[k];
-    val.alloc(std::max<size_t>(v.size(), 1));
-    val.upload(v.data(), v.size(), s);
-    blocks.alloc(blk.size());
-    blocks.upload(blk.data(), blk.size(), s);
-    PDFP_HIP(hipStreamSynchronize(s));  // the host vectors die with this scope
-  }
-  Csr<TV> view() const {
-    return Csr<TV>{rowptr.p, col.p, val.p, blocks.p, nblocks, nrows};
-  }
-  size_t bytes() const { return rowptr.bytes() + col.bytes() + blocks.bytes() + val.bytes(); }
-};
-
-// Launch one CSR pass (no-op on an empty operator).
-template <int NACC, typename TV, class G, class E>
-inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
-  if (A.nblocks == 0) return;
-  csr_pass<kBlock, kNpb, TV, G, E, NACC><<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
-}
 
 struct ProjectorBase {
   virtual ~ProjectorBase() = default;
@@ -398,10 +274,10 @@ struct Projector final : ProjectorBase {
     EpiEvalRow<T> er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr};
     launch_pass<kEvalRowTerms>(A, Gather1<T>{px.p}, er, part_a.p, stream);
     if (At.nblocks)
-      finalize_partials<256><<<1, 256, 0, stream>>>(part_t.p, At.nblocks, kEvalColTerms,
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_t.p, At.nblocks, kEvalColTerms, 0,
                                                     sums.p + slot * 16);
     if (A.nblocks)
-      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, kEvalRowTerms,
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, kEvalRowTerms, 0,
                                                     sums.p + slot * 16 + 8);
   }
 
@@ -413,11 +289,8 @@ struct Projector final : ProjectorBase {
     PDFP_HIP(hipMemcpyAsync(d_sp.p, h_sp.p, sizeof(StepParams), hipMemcpyHostToDevice, stream));
   }
 
-  // ---- power iteration on the device -----------------------------------------
-
   double reduce_scalar_vec(int nb, int nacc, double* out) {
-    finalize_partials<256><<<1, 256, 0, stream>>>(part_v.p, nb, nacc, sums.p + 48);
-    // slots 48..63: this reduction and the power iteration's σ² (slot 56)
+    finalize_partials<256><<<1, 256, 0, stream>>>(part_v.p, nb, nacc, 0, sums.p + 48);
     PDFP_HIP(hipMemcpyAsync(h_sums.p + 48, sums.p + 48, sizeof(double) * 16,
                             hipMemcpyDeviceToHost, stream));
     sync();
@@ -427,32 +300,7 @@ struct Projector final : ProjectorBase {
 
   // estimate_max_singular_value  src/preconditioner.jl:108-125
   double max_singular_value(int max_iterations, double tolerance) override {
-    if (m == 0) return 0.0;
-    DevBuf<double> eig, nxt, dual;
-    eig.alloc((size_t)m);
-    nxt.alloc((size_t)m);
-    dual.alloc((size_t)std::max<int64_t>(n, 1));
-    std::vector<double> start((size_t)m);
-    start_vector(m, kSvSeed, start.data());
-    eig.upload(start.data(), (size_t)m, stream);
-    const int gm = grid_for(m);
-    double tmp[2];
-    k_sumsq<<<gm, 256, 0, stream>>>((int)m, eig.p, part_v.p);
-    double nrm = std::sqrt(reduce_scalar_vec(gm, 1, tmp));
-    double sigma_sq = 1.0;
-    for (int it = 0; it < max_iterations; ++it) {
-      if (nrm == 0.0) break;
-      k_scale_div<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, nrm);
-      launch_pass<0>(At, Gather1<double>{nxt.p}, EpiStore<double>{dual.p}, nullptr, stream);
-      launch_pass<1>(A, Gather1<double>{dual.p}, EpiStoreDot{eig.p, nxt.p}, part_a.p, stream);
-      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, 1, sums.p + 56);
-      k_pi_resid<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, sums.p + 56, part_v.p);
-      reduce_scalar_vec(gm, 2, tmp);
-      sigma_sq = h_sums.p[56];
-      if (std::sqrt(tmp[0]) < tolerance) break;
-      nrm = std::sqrt(tmp[1]);
-    }
-    return std::sqrt(jmax(sigma_sq, 0.0));
+    return power_iteration(A, At, m, n, max_iterations, tolerance, stream);
   }
 
   // initialize_step_size_and_primal_weight!  src/pdfp.jl:220-241

@@ -15,6 +15,7 @@
 #pragma once
 
 #include "engine.h"
+#include "solver_common.h"
 
 namespace pdfp {
 
@@ -121,25 +122,6 @@ struct EpiDualZ {
   }
 };
 
-// out[row] = s[0]
-template <typename TO>
-struct EpiStore {
-  TO* __restrict__ out;
-  __device__ __forceinline__ void operator()(int r, const double* s, double*) const {
-    out[r] = (TO)s[0];
-  }
-};
-
-// out[row] = s[0];  acc[0] += v[row] * s[0]   (power iteration: eig = A d, ⟨nxt, eig⟩)
-struct EpiStoreDot {
-  double* __restrict__ out;
-  const double* __restrict__ v;
-  __device__ __forceinline__ void operator()(int r, const double* s, double* acc) const {
-    out[r] = s[0];
-    acc[0] += v[r] * s[0];
-  }
-};
-
 // ---- evaluation epilogues (FP64 reductions) ---------------------------------
 // Column pass over Aᵀ with GatherDelta(pdhg_z, reflected_z):
 //   s[0] = (Aᵀ pdhg_z)_j, s[1] = (Aᵀ δz)_j, δ = pdhg - reflected.
@@ -224,7 +206,7 @@ struct EpiViolation {
 // ---- element-wise kernels of the plain FP64 form (the reference's K1-K3) ---
 
 // K1, src/pdfp_kernels.jl:4-20
-__global__ __launch_bounds__(256) void k_primal(const StepParams* __restrict__ sp, int k_offset,
+static __global__ __launch_bounds__(256) void k_primal(const StepParams* __restrict__ sp, int k_offset,
                                                 int n, double* __restrict__ px,
                                                 double* __restrict__ ds, double* __restrict__ rx,
                                                 double* __restrict__ x,
@@ -247,7 +229,7 @@ __global__ __launch_bounds__(256) void k_primal(const StepParams* __restrict__ s
 }
 
 // K2, src/pdfp_kernels.jl:33-48 (w_j = 0 off I)
-__global__ __launch_bounds__(256) void k_dual_y(const StepParams* __restrict__ sp, int k_offset,
+static __global__ __launch_bounds__(256) void k_dual_y(const StepParams* __restrict__ sp, int k_offset,
                                                 int n, double* __restrict__ py,
                                                 double* __restrict__ y, double* __restrict__ ry,
                                                 const double* __restrict__ y0,
@@ -267,7 +249,7 @@ __global__ __launch_bounds__(256) void k_dual_y(const StepParams* __restrict__ s
 }
 
 // K3, src/pdfp_kernels.jl:60-75
-__global__ __launch_bounds__(256) void k_dual_z(const StepParams* __restrict__ sp, int k_offset,
+static __global__ __launch_bounds__(256) void k_dual_z(const StepParams* __restrict__ sp, int k_offset,
                                                 int m, double* __restrict__ pz,
                                                 double* __restrict__ z, double* __restrict__ rz,
                                                 const double* __restrict__ z0,
@@ -320,7 +302,7 @@ __global__ __launch_bounds__(256) void k_convert(int n, TO* __restrict__ out,
 // clamped to a bound gets that bound's FP64 value (x = D_c·clamp(x̂, l/D_c,
 // u/D_c) then holds exactly as in FP64); interior coordinates are widened.
 // acc[0] = Σ_I w|x - x̃| of the polished point, in FP64.
-__global__ __launch_bounds__(256) void k_polish(int n, const float* __restrict__ px,
+static __global__ __launch_bounds__(256) void k_polish(int n, const float* __restrict__ px,
                                                 const float* __restrict__ lb,
                                                 const float* __restrict__ ub,
                                                 const double* __restrict__ lb64,
@@ -338,38 +320,6 @@ __global__ __launch_bounds__(256) void k_polish(int n, const float* __restrict__
     const double wj = w64[j];
     if (wj > 0.0) acc[0] = wj * fabs(v - xt64[j]);
   }
-  block_store_partials<256, 1>(acc, partials + blockIdx.x);
-}
-
-// Power iteration helpers (src/preconditioner.jl:113-123).
-__global__ __launch_bounds__(256) void k_scale_div(int m, double* __restrict__ out,
-                                                   const double* __restrict__ in, double nrm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) out[i] = in[i] / nrm;
-}
-
-// nxt = eig - σ² nxt;  acc0 = ‖nxt‖², acc1 = ‖eig‖²  (σ² read from device)
-__global__ __launch_bounds__(256) void k_pi_resid(int m, double* __restrict__ nxt,
-                                                  const double* __restrict__ eig,
-                                                  const double* __restrict__ sigma_sq,
-                                                  double* __restrict__ partials) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[2] = {0.0, 0.0};
-  if (i < m) {
-    const double e = eig[i];
-    const double v = e - sigma_sq[0] * nxt[i];
-    nxt[i] = v;
-    acc[0] = v * v;
-    acc[1] = e * e;
-  }
-  block_store_partials<256, 2>(acc, partials + (size_t)blockIdx.x * 2);
-}
-
-__global__ __launch_bounds__(256) void k_sumsq(int m, const double* __restrict__ v,
-                                               double* __restrict__ partials) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double acc[1] = {0.0};
-  if (i < m) acc[0] = v[i] * v[i];
   block_store_partials<256, 1>(acc, partials + blockIdx.x);
 }
 

@@ -1,0 +1,228 @@
+// Device epilogues of the root-LP solver: restarted Halpern PDHG with
+// reflection for  min c'x  s.t.  con_lb ≤ A x ≤ con_ub,  var_lb ≤ x ≤ var_ub
+// (src/pdlp.jl:115-153, GPU form src/pdlp_kernels.jl:4-63), fused into the
+// two CSR passes of an iteration:
+//   transpose pass (rows of Aᵀ = columns j):  s = (Aᵀy)_j,
+//     x⁺ = clamp(x - τ(c - s), ℓ, u),  x̄ = 2x⁺ - x,  Halpern average
+//   A pass (rows i):                          s = (A x̄)_i,
+//     t = y/σ - s,  y⁺ = (t - clamp(t, -u_c, -ℓ_c))σ,  ȳ = 2y⁺ - y, Halpern
+// The evaluation every 200 iterations computes, in FP64, the fixed-point
+// error, the KKT residuals and objectives (src/pdlp.jl:157-165, 220-242) and,
+// under cuPDLPx termination, the infeasibility rays (src/pdlp.jl:247-285).
+#pragma once
+
+#include "engine.h"
+#include "pdfp_kernels.h"
+#include "solver_common.h"
+
+namespace pdfp {
+
+__device__ __forceinline__ bool finite(double v) { return fabs(v) < INFINITY; }
+
+// Transpose pass of one iteration (K4, src/pdlp_kernels.jl:4-21).
+template <typename T, bool MAJOR>
+struct EpiLpPrimal {
+  const StepParams* sp;
+  int k_offset;
+  T* __restrict__ x;
+  const T* __restrict__ x0;
+  const T* __restrict__ c;
+  const T* __restrict__ lb;
+  const T* __restrict__ ub;
+  T* __restrict__ rx;
+  T* __restrict__ px;
+  T* __restrict__ ds;
+  __device__ __forceinline__ void operator()(int j, const double* s, double*) const {
+    const StepScalars sc = step_scalars(sp, k_offset);
+    const double cx = (double)x[j];
+    const double temp = cx - sc.tau * ((double)c[j] - s[0]);
+    const double p = jclamp(temp, (double)lb[j], (double)ub[j]);
+    const double r = 2.0 * p - cx;
+    rx[j] = (T)r;
+    x[j] = (T)(sc.weight * (sc.omega * r + sc.omw * cx) + sc.omwt * (double)x0[j]);
+    if (MAJOR) {
+      px[j] = (T)p;
+      ds[j] = (T)((p - temp) / sc.tau);
+    }
+  }
+};
+
+// A pass of one iteration (K5, src/pdlp_kernels.jl:35-51).
+template <typename T, bool MAJOR>
+struct EpiLpDual {
+  const StepParams* sp;
+  int k_offset;
+  T* __restrict__ y;
+  const T* __restrict__ y0;
+  const T* __restrict__ clb;
+  const T* __restrict__ cub;
+  T* __restrict__ py;
+  T* __restrict__ ry;
+  __device__ __forceinline__ void operator()(int i, const double* s, double*) const {
+    const StepScalars sc = step_scalars(sp, k_offset);
+    const double cy = (double)y[i];
+    const double t = cy / sc.sigma - s[0];
+    const double p = (t - jclamp(t, -(double)cub[i], -(double)clb[i])) * sc.sigma;
+    const double r = 2.0 * p - cy;
+    y[i] = (T)(sc.weight * (sc.omega * r + sc.omw * cy) + sc.omwt * (double)y0[i]);
+    if (MAJOR) {
+      py[i] = (T)p;
+      ry[i] = (T)r;
+    }
+  }
+};
+
+// ---- evaluation ------------------------------------------------------------
+
+// The dual ray component of row i: (ȳ - y⁺) projected by the finiteness of
+// the row bounds (src/pdlp.jl:249-253).
+template <typename T>
+__device__ __forceinline__ double dual_ray(const T* py, const T* ry, const T* clb, const T* cub,
+                                           int i) {
+  double v = (double)ry[i] - (double)py[i];
+  if (!finite((double)clb[i])) v = fmin(v, 0.0);
+  if (!finite((double)cub[i])) v = fmax(v, 0.0);
+  return v;
+}
+
+// The primal ray component of column j: (x̄ - x⁺) projected by the
+// finiteness of the variable bounds (src/pdlp.jl:249-251).
+template <typename T>
+__device__ __forceinline__ double primal_ray(const T* px, const T* rx, const T* lb, const T* ub,
+                                             int j) {
+  double v = (double)rx[j] - (double)px[j];
+  if (finite((double)lb[j])) v = fmax(v, 0.0);
+  if (finite((double)ub[j])) v = fmin(v, 0.0);
+  return v;
+}
+
+// Transpose-pass gather: p0 = a·y⁺_i, p1 = a·(y⁺_i - ȳ_i), p2 = a·ray_i.
+template <typename T, bool RAYS>
+struct GatherLpCol {
+  static constexpr int NV = RAYS ? 3 : 2;
+  const T* __restrict__ py;
+  const T* __restrict__ ry;
+  const T* __restrict__ clb;
+  const T* __restrict__ cub;
+  __device__ __forceinline__ void operator()(int i, double a, double* p) const {
+    const double pi = (double)py[i];
+    p[0] = a * pi;
+    p[1] = a * (pi - (double)ry[i]);
+    if constexpr (RAYS) p[2] = a * dual_ray(py, ry, clb, cub, i);
+  }
+};
+
+// A-pass gather: p0 = a·x⁺_j, p1 = a·ray_j (unnormalised).
+template <typename T, bool RAYS>
+struct GatherLpRow {
+  static constexpr int NV = RAYS ? 2 : 1;
+  const T* __restrict__ px;
+  const T* __restrict__ rx;
+  const T* __restrict__ lb;
+  const T* __restrict__ ub;
+  __device__ __forceinline__ void operator()(int j, double a, double* p) const {
+    p[0] = a * (double)px[j];
+    if constexpr (RAYS) p[1] = a * primal_ray(px, rx, lb, ub, j);
+  }
+};
+
+// Column terms: 0 ‖δx‖², 1 ⟨Aᵀδy, δx⟩, 2 ‖(c - Aᵀy - ds)·col_unscale‖²,
+// 3 c·x, 4 ds·x, 5 ‖x⁰ - x‖², 6 c·d, 7 dual-slack term of the dual ray
+// objective; maxima: 8 max|d|, 9 max|dual ray infeasibility|.
+constexpr int kLpColTerms = 10;
+constexpr int kLpColMax = 2;
+template <typename T, bool RAYS>
+struct EpiLpEvalCol {
+  const T* __restrict__ px;
+  const T* __restrict__ rx;
+  const T* __restrict__ ds;
+  const T* __restrict__ c;
+  const T* __restrict__ x0;
+  const T* __restrict__ lb;
+  const T* __restrict__ ub;
+  const double* __restrict__ col_unscale;  // null: scaled units
+  __device__ __forceinline__ void operator()(int j, const double* s, double* acc) const {
+    const double pxj = (double)px[j];
+    const double dx = pxj - (double)rx[j];
+    acc[0] += dx * dx;
+    acc[1] += s[1] * dx;
+    const double cj = (double)c[j], dsj = (double)ds[j];
+    double r = cj - s[0] - dsj;
+    if (col_unscale) r *= col_unscale[j];
+    acc[2] += r * r;
+    acc[3] += cj * pxj;
+    acc[4] += dsj * pxj;
+    const double ex = (double)x0[j] - pxj;
+    acc[5] += ex * ex;
+    if constexpr (RAYS) {
+      const double d = primal_ray(px, rx, lb, ub, j);
+      acc[6] += cj * d;
+      acc[8] = fmax(acc[8], fabs(d));
+      const double lo = (double)lb[j], hi = (double)ub[j];
+      const double dp = s[2];
+      acc[7] += fmax(-dp, 0.0) * (finite(lo) ? lo : 0.0) + fmin(-dp, 0.0) * (finite(hi) ? hi : 0.0);
+      double v = (!finite(lo) ? fmax(0.0, -dp) : 0.0) - (!finite(hi) ? fmin(0.0, -dp) : 0.0);
+      v *= col_unscale[j];
+      acc[9] = fmax(acc[9], fabs(v));
+    }
+  }
+};
+
+// Row terms: 0 ‖δy‖², 1 ‖(Ax - clamp(Ax, ℓ_c, u_c))·row_unscale‖², 2 the
+// bound part of the dual objective, 3 ‖y⁰ - y‖², 4 primal-slack term of the
+// dual ray objective; maxima: 5 max|dual ray|, 6 max|primal ray
+// infeasibility| (unnormalised).
+constexpr int kLpRowTerms = 7;
+constexpr int kLpRowMax = 2;
+template <typename T, bool RAYS>
+struct EpiLpEvalRow {
+  const T* __restrict__ py;
+  const T* __restrict__ ry;
+  const T* __restrict__ y0;
+  const T* __restrict__ clb;
+  const T* __restrict__ cub;
+  const double* __restrict__ row_unscale;  // null: scaled units
+  __device__ __forceinline__ void operator()(int i, const double* s, double* acc) const {
+    const double pyi = (double)py[i];
+    const double dy = pyi - (double)ry[i];
+    acc[0] += dy * dy;
+    const double lo = (double)clb[i], hi = (double)cub[i];
+    double r = s[0] - jclamp(s[0], lo, hi);
+    if (row_unscale) r *= row_unscale[i];
+    acc[1] += r * r;
+    acc[2] += fmax(pyi, 0.0) * (fabs(lo) < 1e19 ? lo : 0.0) +
+              fmin(pyi, 0.0) * (fabs(hi) < 1e19 ? hi : 0.0);
+    const double ey = (double)y0[i] - pyi;
+    acc[3] += ey * ey;
+    if constexpr (RAYS) {
+      const double v = dual_ray(py, ry, clb, cub, i);
+      acc[4] += fmax(v, 0.0) * (finite(lo) ? lo : 0.0) + fmin(v, 0.0) * (finite(hi) ? hi : 0.0);
+      acc[5] = fmax(acc[5], fabs(v));
+      const double ad = s[1];
+      double q = (finite(lo) ? fmax(0.0, -ad) : 0.0) + (finite(hi) ? fmax(0.0, ad) : 0.0);
+      q *= row_unscale[i];
+      acc[6] = fmax(acc[6], fabs(q));
+    }
+  }
+};
+
+// perform_restart!: initial ← pdhg, current ← pdhg (src/pdlp.jl:211-212).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lp_restart(int n, int m, const T* __restrict__ px,
+                                                    const T* __restrict__ py, T* __restrict__ x,
+                                                    T* __restrict__ x0, T* __restrict__ y,
+                                                    T* __restrict__ y0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const T a = px[i];
+    x[i] = a;
+    x0[i] = a;
+  }
+  if (i < m) {
+    const T a = py[i];
+    y[i] = a;
+    y0[i] = a;
+  }
+}
+
+}  // namespace pdfp

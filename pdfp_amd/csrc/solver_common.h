@@ -1,0 +1,256 @@
+// Host-side pieces shared by the projector and the root-LP solver: the
+// restart rule and PID primal weight (src/pdlp.jl:167-205), the device CSR
+// with its row-block schedule, and the power iteration for ‖A‖₂
+// (src/preconditioner.jl:108-125).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pdfp_hip.h"
+#include "engine.h"
+#include "host_util.h"
+
+namespace pdfp {
+
+constexpr int kBlock = 256;  // threads per workgroup of the CSR passes
+constexpr int kNpb = 2048;   // nonzeros staged per workgroup
+constexpr uint64_t kSvSeed = 20240517ull;
+constexpr double kJuliaEps = 2.220446049250313e-16;
+
+extern std::string g_create_error;  // message of the last failed *_create
+
+inline double jmax(double a, double b) {
+  if (std::isnan(a) || std::isnan(b)) return NAN;
+  return a > b ? a : b;
+}
+
+inline int grid_for(int64_t n) { return (int)((n + 255) / 256); }
+
+struct RestartState {
+  double primal_weight = 1.0, best_primal_weight = 1.0;
+  double best_primal_dual_residual_gap = INFINITY;
+  double primal_weight_error_sum = 0.0, primal_weight_last_error = 0.0;
+  double fixed_point_error = INFINITY, initial_fixed_point_error = INFINITY;
+  double last_trial_fixed_point_error = INFINITY;
+  int64_t inner_count = 0, total_count = 0;
+  double relative_primal_residual = INFINITY, relative_dual_residual = INFINITY;
+};
+
+// should_restart  src/pdlp.jl:167-181
+inline bool should_restart(RestartState& s, const pdfp_params& p) {
+  bool restart = false;
+  if (s.total_count == p.termination_evaluation_frequency) {
+    restart = true;
+  } else if (s.total_count > p.termination_evaluation_frequency) {
+    if (s.fixed_point_error <= p.sufficient_reduction_for_restart * s.initial_fixed_point_error)
+      restart = true;
+    if (s.fixed_point_error <= p.necessary_reduction_for_restart * s.initial_fixed_point_error &&
+        s.fixed_point_error > s.last_trial_fixed_point_error)
+      restart = true;
+    if ((double)s.inner_count >= p.artificial_restart_threshold * (double)s.total_count)
+      restart = true;
+  }
+  s.last_trial_fixed_point_error = s.fixed_point_error;
+  return restart;
+}
+
+// update_primal_weight!  src/pdlp.jl:185-205
+inline void update_primal_weight(RestartState& s, const pdfp_params& rp, double primal_dist,
+                                 double dual_dist) {
+  const double ratio = s.relative_dual_residual / jmax(s.relative_primal_residual, kJuliaEps);
+  if (1e-16 < primal_dist && primal_dist < 1e12 && 1e-16 < dual_dist && dual_dist < 1e12 &&
+      1e-8 < ratio && ratio < 1e8) {
+    const double err = std::log(dual_dist) - std::log(primal_dist) - std::log(s.primal_weight);
+    s.primal_weight_error_sum *= rp.i_smooth;
+    s.primal_weight_error_sum += err;
+    const double delta_err = err - s.primal_weight_last_error;
+    s.primal_weight *=
+        std::exp(rp.k_p * err + rp.k_i * s.primal_weight_error_sum + rp.k_d * delta_err);
+    s.primal_weight_last_error = err;
+  } else {
+    s.primal_weight = s.best_primal_weight;
+    s.primal_weight_error_sum = 0.0;
+    s.primal_weight_last_error = 0.0;
+  }
+  const double gap =
+      std::fabs(std::log10(s.relative_dual_residual / jmax(s.relative_primal_residual, kJuliaEps)));
+  if (gap < s.best_primal_dual_residual_gap) {
+    s.best_primal_dual_residual_gap = gap;
+    s.best_primal_weight = s.primal_weight;
+  }
+}
+
+// SplitMix64 + Box–Muller normals: the fixed start of the power iteration
+// (the reference draws randn(MersenneTwister(20240517), m); this stream is
+// the library's own and documented here: u1 ∈ (0,1], u2 ∈ [0,1) from the top
+// 53 bits of consecutive SplitMix64 outputs, r = √(-2 ln u1),
+// (r cos 2πu2, r sin 2πu2)).
+inline uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline void start_vector(int64_t m, uint64_t seed, double* out) {
+  uint64_t s = seed;
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t i = 0; i < m; i += 2) {
+    const double u1 = ((double)(splitmix64(s) >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = (double)(splitmix64(s) >> 11) * 0x1.0p-53;
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    out[i] = r * std::cos(two_pi * u2);
+    if (i + 1 < m) out[i + 1] = r * std::sin(two_pi * u2);
+  }
+}
+
+// Device CSR + schedule of one operator (A or Aᵀ).
+template <typename TV>
+struct DevCsr {
+  DevBuf<int32_t> rowptr, col, blocks;
+  DevBuf<TV> val;
+  int32_t nblocks = 0, nrows = 0;
+  int64_t nnz = 0;
+  void upload(const HostCsr& h, hipStream_t s) {
+    nrows = (int32_t)h.nrows;
+    nnz = (int64_t)h.col.size();
+    std::vector<int32_t> blk = row_blocks(h.rowptr, kBlock, kNpb);
+    nblocks = (int32_t)blk.size() - 1;
+    rowptr.alloc(h.rowptr.size());
+    rowptr.upload(h.rowptr.data(), h.rowptr.size(), s);
+    col.alloc(std::max<size_t>(h.col.size(), 1));
+    col.upload(h.col.data(), h.col.size(), s);
+    std::vector<TV> v(h.val.size());
+    for (size_t k = 0; k < v.size(); ++k) v[k] = (TV)h.val[k];
+    val.alloc(std::max<size_t>(v.size(), 1));
+    val.upload(v.data(), v.size(), s);
+    blocks.alloc(blk.size());
+    blocks.upload(blk.data(), blk.size(), s);
+    PDFP_HIP(hipStreamSynchronize(s));  // the host vectors die with this scope
+  }
+  Csr<TV> view() const { return Csr<TV>{rowptr.p, col.p, val.p, blocks.p, nblocks, nrows}; }
+  size_t bytes() const { return rowptr.bytes() + col.bytes() + blocks.bytes() + val.bytes(); }
+};
+
+// Launch one CSR pass (no-op on an empty operator).
+template <int NACC, int NMAX = 0, typename TV, class G, class E>
+inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
+  if (A.nblocks == 0) return;
+  csr_pass<kBlock, kNpb, TV, G, E, NACC, NMAX>
+      <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+}
+
+// Power-iteration helper kernels (src/preconditioner.jl:113-123); static so
+// that each translation unit carries its own copy.
+static __global__ __launch_bounds__(256) void k_scale_div(int m, double* __restrict__ out,
+                                                          const double* __restrict__ in,
+                                                          double nrm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = in[i] / nrm;
+}
+
+// nxt = eig - σ² nxt;  acc0 = ‖nxt‖², acc1 = ‖eig‖²  (σ² read from device)
+static __global__ __launch_bounds__(256) void k_pi_resid(int m, double* __restrict__ nxt,
+                                                         const double* __restrict__ eig,
+                                                         const double* __restrict__ sigma_sq,
+                                                         double* __restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[2] = {0.0, 0.0};
+  if (i < m) {
+    const double e = eig[i];
+    const double v = e - sigma_sq[0] * nxt[i];
+    nxt[i] = v;
+    acc[0] = v * v;
+    acc[1] = e * e;
+  }
+  block_store_partials<256, 2>(acc, partials + (size_t)blockIdx.x * 2);
+}
+
+static __global__ __launch_bounds__(256) void k_sumsq(int m, const double* __restrict__ v,
+                                                      double* __restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[1] = {0.0};
+  if (i < m) acc[0] = v[i] * v[i];
+  block_store_partials<256, 1>(acc, partials + blockIdx.x);
+}
+
+// out[row] = s[0]
+template <typename TO>
+struct EpiStore {
+  TO* __restrict__ out;
+  __device__ __forceinline__ void operator()(int r, const double* s, double*) const {
+    out[r] = (TO)s[0];
+  }
+};
+
+// out[row] = s[0];  acc[0] += v[row] * s[0]   (power iteration: eig = A d, ⟨nxt, eig⟩)
+struct EpiStoreDot {
+  double* __restrict__ out;
+  const double* __restrict__ v;
+  __device__ __forceinline__ void operator()(int r, const double* s, double* acc) const {
+    out[r] = s[0];
+    acc[0] += v[r] * s[0];
+  }
+};
+
+// Scratch for the small reductions of one solver context.
+struct Reducer {
+  DevBuf<double> part;   // per-block partials
+  DevBuf<double> sums;   // 64 finalized values
+  PinnedBuf<double> host;
+  void alloc(size_t partial_slots) {
+    part.alloc(std::max<size_t>(partial_slots, 16));
+    sums.alloc(64);
+    host.alloc(64);
+  }
+};
+
+// estimate_max_singular_value(A, A_t, m; max_iterations, tolerance)
+// src/preconditioner.jl:108-125, on the device: one host synchronisation per
+// iteration (the norms of the residual and of the next vector).
+template <typename TV>
+double power_iteration(const DevCsr<TV>& A, const DevCsr<TV>& At, int64_t m, int64_t n,
+                       int max_iterations, double tolerance, hipStream_t stream) {
+  if (m == 0) return 0.0;
+  DevBuf<double> eig, nxt, dual, part, sums;
+  PinnedBuf<double> h;
+  eig.alloc((size_t)m);
+  nxt.alloc((size_t)m);
+  dual.alloc((size_t)std::max<int64_t>(n, 1));
+  const int gm = grid_for(m);
+  part.alloc((size_t)std::max(gm * 2, std::max(A.nblocks, 1)) + 8);
+  sums.alloc(8);
+  h.alloc(8);
+  std::vector<double> start((size_t)m);
+  start_vector(m, kSvSeed, start.data());
+  eig.upload(start.data(), (size_t)m, stream);
+  k_sumsq<<<gm, 256, 0, stream>>>((int)m, eig.p, part.p);
+  finalize_partials<256><<<1, 256, 0, stream>>>(part.p, gm, 1, 0, sums.p);
+  PDFP_HIP(hipMemcpyAsync(h.p, sums.p, sizeof(double) * 8, hipMemcpyDeviceToHost, stream));
+  PDFP_HIP(hipStreamSynchronize(stream));
+  double nrm = std::sqrt(h.p[0]);
+  double sigma_sq = 1.0;
+  for (int it = 0; it < max_iterations; ++it) {
+    if (nrm == 0.0) break;
+    k_scale_div<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, nrm);
+    launch_pass<0>(At, Gather1<double>{nxt.p}, EpiStore<double>{dual.p}, nullptr, stream);
+    launch_pass<1>(A, Gather1<double>{dual.p}, EpiStoreDot{eig.p, nxt.p}, part.p, stream);
+    finalize_partials<256><<<1, 256, 0, stream>>>(part.p, A.nblocks, 1, 0, sums.p + 4);
+    k_pi_resid<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, sums.p + 4, part.p);
+    finalize_partials<256><<<1, 256, 0, stream>>>(part.p, gm, 2, 0, sums.p);
+    PDFP_HIP(hipMemcpyAsync(h.p, sums.p, sizeof(double) * 8, hipMemcpyDeviceToHost, stream));
+    PDFP_HIP(hipStreamSynchronize(stream));
+    sigma_sq = h.p[4];
+    if (std::sqrt(h.p[0]) < tolerance) break;
+    nrm = std::sqrt(h.p[1]);
+  }
+  return std::sqrt(jmax(sigma_sq, 0.0));
+}
+
+}  // namespace pdfp

@@ -1,0 +1,73 @@
+"""GPU parity of the root-LP solver (src/pdlp.jl) against the oracle and the
+HiGHS LP optima (tests/golden/root_lp.json).
+
+Bars: the device FP64 trajectory is the oracle's (same termination, same
+iteration count, primal within 1e-8 in scaled units); the objective of the
+returned point matches HiGHS to the solve tolerance."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pdfp_amd import generators as G
+from pdfp_amd import pdlp as L
+from pdfp_amd.milp import num_continuous
+from pdfp_amd.pdfp import SolverParameters
+from pdfp_amd.preconditioner import scale_problem
+
+from .helpers import build_instance, root_lp_fixtures
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", root_lp_fixtures(), ids=lambda c: c["name"])
+def test_root_lp_matches_highs_and_oracle(case):
+    print("pdlp", case["name"], flush=True)
+    milp = build_instance(case["spec"])
+    mixed = num_continuous(milp) > 0
+    x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-6, time_limit=60.0,
+                                          cupdlpx_termination=mixed)
+    xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-6, cupdlpx_termination=mixed)
+    assert st == so == "OPTIMAL"
+    assert it == ito
+    np.testing.assert_allclose(x, xo, atol=1e-7 * (1 + np.abs(xo).max()))
+    obj = float(milp.c @ x) + milp.c0
+    ref = case["lp_optimum"]
+    assert abs(obj - ref) <= 1e-3 * (1.0 + abs(ref))
+    # the dual slack signs drive the pump's batch schedule (src/pump.jl:81-93)
+    big = np.abs(dso) > 1e-6
+    assert np.all(np.sign(ds[big]) == np.sign(dso[big]))
+
+
+def test_root_lp_p0201_at_pump_tolerance():
+    """README.md:38-39: p0201's root LP at tolerance 1e-2."""
+    milp = build_instance({"family": "example", "name": "p0201"})
+    x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-2, time_limit=60.0,
+                                          cupdlpx_termination=False)
+    xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-2)
+    assert st == so == "OPTIMAL" and it == ito
+    assert abs(float(milp.c @ x) - 6875.0) <= 0.05 * 6875.0
+
+
+def test_root_lp_infeasible_detected():
+    """An infeasible mixed LP ends PRIMAL_INFEASIBLE under cuPDLPx
+    termination (src/pdlp.jl:355-358), on the device as on the oracle."""
+    import scipy.sparse as sp
+    A = sp.csc_matrix(np.array([[1.0, 1.0], [-1.0, -1.0]]))
+    milp = G.MILP(c=np.array([1.0, 1.0]), c0=0.0, A=A, b=np.array([1.0, -3.0]),
+                  row_sense=np.array(["le", "le"]), l=np.zeros(2), u=np.array([10.0, np.inf]),
+                  int_idx=np.array([0]))
+    xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-4, cupdlpx_termination=True)
+    x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-4, time_limit=30.0,
+                                          cupdlpx_termination=True)
+    assert so == "PRIMAL_INFEASIBLE"
+    assert st == so and it == ito
+
+
+def test_root_lp_mixed_generated():
+    m = G.mixed(500, 1000, 8)
+    m.c = np.random.default_rng(3).standard_normal(1000)
+    x, ds, st, it = L.solve_lp_relaxation(m, tolerance=1e-4, time_limit=60.0,
+                                          cupdlpx_termination=True)
+    xo, dso, so, ito = O.solve_lp_relaxation(m, tolerance=1e-4, cupdlpx_termination=True)
+    assert st == so == "OPTIMAL" and it == ito
+    np.testing.assert_allclose(x, xo, atol=1e-6 * (1 + np.abs(xo).max()))

