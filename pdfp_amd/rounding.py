@@ -1,0 +1,248 @@
+"""Rounding: FP 2.0's constraint-propagation rounding (CPR) and the guarded
+two-phase CPR (host; the reference's arithmetic, unchanged).
+
+Restates src/rounding.jl.  CPR ranks the integer variables by fractionality
+(binaries first), then fixes them one at a time to their rounded LP value,
+propagating each fixing.  The guarded rounder first fixes, in one
+propagation, every variable whose LP value is already integral (the batch),
+then rounds the rest sequentially; if the batch makes a row infeasible its
+work is discarded and the call falls back to sequential CPR.
+
+Random numbers: the reference draws from Julia's MersenneTwister; here the
+caller's ``numpy.random.Generator`` is used at the same points
+(``rand(rng, 1:len)`` ↔ ``rng.integers(1, len + 1)``).  Every rounding that
+draws nothing (iteration < swap_iteration, no flips or perturbations) is
+therefore identical to the reference's.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from .milp import MILP
+from .propagation import BOUND_TOLERANCE, PropagationData, PropagationState
+
+
+def jround(v: float) -> float:
+    """Julia's round (ties to even, the sign of zero kept)."""
+    return float(np.rint(v))
+
+
+def jclamp(x: float, lo: float, hi: float) -> float:
+    return hi if x > hi else (lo if x < lo else x)
+
+
+class BatchPolicy:
+    """Online rule that decides, per call, whether to try the batch
+    (src/rounding.jl:22-63)."""
+
+    def __init__(self, xi: float = 0.3, nu: float = 0.9, reprobe: int = 20):
+        self.xi, self.nu, self.reprobe = xi, nu, reprobe
+        self.n_failed = 0.0
+        self.n_held = 0.0
+        self.s = math.nan
+        self.h = math.nan
+        self.w = math.nan
+        self.skipped = 0
+
+    def _ema(self, old: float, x: float) -> float:
+        return x if math.isnan(old) else (1.0 - self.xi) * old + self.xi * x
+
+    def wants_batch(self) -> bool:
+        if self.skipped >= self.reprobe:
+            return True
+        if math.isnan(self.s) or self.n_failed == 0.0:
+            return True
+        h = 0.0 if math.isnan(self.h) else self.h
+        p = self.n_failed / (self.n_failed + self.n_held)
+        return p * (self.w + self.s) + (1.0 - p) * h < self.s
+
+    def observe_batch(self, failed: bool, cost: float, wasted: float) -> None:
+        self.n_failed *= self.nu
+        self.n_held *= self.nu
+        self.skipped = 0
+        if failed:
+            self.n_failed += 1.0
+            self.w = self._ema(self.w, wasted)
+            self.s = self._ema(self.s, max(cost - wasted, 0.0))
+        else:
+            self.n_held += 1.0
+            self.h = self._ema(self.h, cost)
+
+    def observe_skip(self, cost: float) -> None:
+        self.s = self._ema(self.s, cost)
+        self.skipped += 1
+
+
+class Rounder:
+    """Rounding state that persists across the calls of one pump run
+    (src/rounding.jl:78-98)."""
+
+    def __init__(self, milp: MILP, *, schedule: str = "all", max_pops: int = 2**63 - 1,
+                 swap_iteration: int = 10, policy: BatchPolicy | None | str = "default"):
+        self.milp = milp
+        self.data: PropagationData | None = None
+        self.schedule = schedule                # "all" or "regroup"
+        self.direction: list | None = None      # violation-guided columns, or None
+        self.max_pops = max_pops
+        self.swap_iteration = swap_iteration
+        self.policy = BatchPolicy() if policy == "default" else policy
+        self.n_calls = 0
+        self.n_batched = 0
+        self.n_fallbacks = 0
+        self._int_idx = milp.int_idx.tolist()
+        self._binary = [(milp.l[j] == 0.0 and milp.u[j] == 1.0) for j in range(len(milp.c))]
+        self._b = milp.b.tolist()
+        self._is_le = [s == "le" for s in milp.row_sense.tolist()]
+
+    def _new_state(self) -> PropagationState:
+        if self.data is None:
+            self.data = PropagationData(self.milp)
+        return PropagationState(self.data, self.max_pops)
+
+    def _rank_key(self, j: int, x) -> float:
+        v = x[j]
+        return abs(v - jround(v)) - (10.0 if self._binary[j] else 0.0)
+
+    def ranking(self, x_star, rng, iteration: int) -> list:
+        """Ascending fractionality, binaries first; from swap_iteration on,
+        |I|/10 random transpositions (src/rounding.jl:105-115)."""
+        idx = self._int_idx
+        keys = [self._rank_key(j, x_star) for j in idx]
+        order = sorted(range(len(idx)), key=keys.__getitem__)      # stable, as sortperm
+        ranked = [idx[k] for k in order]
+        n = len(ranked)
+        if iteration >= self.swap_iteration and n >= 10:
+            for _ in range(max(1, n // 10)):
+                a = int(rng.integers(1, n + 1)) - 1
+                b = int(rng.integers(1, n + 1)) - 1
+                ranked[a], ranked[b] = ranked[b], ranked[a]
+        return ranked
+
+    def _row_violation(self, i: int, activity: float) -> float:
+        return max(0.0, activity - self._b[i]) if self._is_le[i] else abs(activity - self._b[i])
+
+    def _guided_value(self, s: PropagationState, j: int, target: float, activity) -> float:
+        """src/rounding.jl:123-141."""
+        if not s.u[j] - s.l[j] > BOUND_TOLERANCE:
+            return target
+        lo = float(math.ceil(s.l[j] - BOUND_TOLERANCE))
+        hi = float(math.floor(s.u[j] + BOUND_TOLERANCE))
+        if not hi > lo:
+            return target
+        d = self.data
+        rows, vals = d.col_rows[j], d.col_vals[j]
+
+        def score(v):
+            t = 0.0
+            for i, a in zip(rows, vals):
+                t += self._row_violation(i, activity[i] + a * v) - self._row_violation(i, activity[i])
+            return t
+
+        score_lo = score(lo)
+        score_hi = score(hi)
+        if score_lo < score_hi - 1e-12:
+            return lo
+        if score_hi < score_lo - 1e-12:
+            return hi
+        return target
+
+    def _add_activity(self, activity, j: int, v: float) -> None:
+        d = self.data
+        for i, a in zip(d.col_rows[j], d.col_vals[j]):
+            activity[i] += a * v
+
+    def _round_sequentially(self, s: PropagationState, xt: list, idx, x_star, activity) -> None:
+        """src/rounding.jl:152-174."""
+        for j in idx:
+            if s.is_fixed(j):
+                xt[j] = s.l[j]
+                continue
+            target = jround(jclamp(x_star[j], s.l[j], s.u[j]))
+            if target < s.l[j] - BOUND_TOLERANCE or target > s.u[j] + BOUND_TOLERANCE:
+                other = jround(jclamp(x_star[j] + (1.0 if target < s.l[j] else -1.0),
+                                      s.l[j], s.u[j]))
+                target = jclamp(other, s.l[j], s.u[j])
+            if activity is not None and self.direction[j]:
+                target = self._guided_value(s, j, target, activity)
+            s.fix(j, target)
+            s.propagate()
+            if activity is not None:
+                self._add_activity(activity, j, target)
+            xt[j] = target
+
+    def _read_integers(self, s: PropagationState, xt: list, x_star) -> list:
+        for j in self._int_idx:
+            xt[j] = s.l[j] if s.is_fixed(j) else jround(jclamp(x_star[j], s.l[j], s.u[j]))
+        return xt
+
+    def _activity(self):
+        return None if self.direction is None else [0.0] * len(self.milp.b)
+
+    def round_sequential(self, x_star, rng, iteration: int) -> np.ndarray:
+        """FP 2.0's CPR of x_star (src/rounding.jl:191-197)."""
+        s = self._new_state()
+        xs = [float(v) for v in x_star]
+        xt = list(xs)
+        ranked = self.ranking(xs, rng, iteration)
+        self._round_sequentially(s, xt, ranked, xs, self._activity())
+        return np.array(self._read_integers(s, xt, xs))
+
+    def round_guarded(self, x_star, rng, iteration: int) -> np.ndarray:
+        """Guarded two-phase CPR (src/rounding.jl:205-256)."""
+        self.n_calls += 1
+        P = self.policy
+        t0 = time.perf_counter_ns() if P is not None else 0
+        if P is not None and not P.wants_batch():
+            xt = self.round_sequential(x_star, rng, iteration)
+            P.observe_skip((time.perf_counter_ns() - t0) / 1e9)
+            return xt
+        s = self._new_state()
+        xs = [float(v) for v in x_star]
+        xt = list(xs)
+        activity = self._activity()
+        ranked = self.ranking(xs, rng, iteration)
+
+        def integral(j):
+            return abs(xs[j] - jround(xs[j])) == 0.0
+
+        if self.schedule == "regroup":
+            last = 0
+            for k, j in enumerate(ranked):
+                if integral(j):
+                    last = k + 1
+            before = [j for j in ranked[:last] if not integral(j)]
+            batch = [j for j in ranked[:last] if integral(j)]
+            after = ranked[last:]
+        else:
+            before = []
+            batch = [j for j in ranked if integral(j)]
+            after = [j for j in ranked if not integral(j)]
+
+        self._round_sequentially(s, xt, before, xs, activity)
+        for j in batch:
+            if s.is_fixed(j):
+                xt[j] = s.l[j]
+                continue
+            target = jround(jclamp(xs[j], s.l[j], s.u[j]))
+            s.fix(j, target)
+            if activity is not None:
+                self._add_activity(activity, j, target)
+            xt[j] = target
+        s.propagate()
+
+        if s.infeasible:                 # the guard: discard the batch
+            self.n_fallbacks += 1
+            wasted = 0.0 if P is None else (time.perf_counter_ns() - t0) / 1e9
+            out = self.round_sequential(x_star, rng, iteration)
+            if P is not None:
+                P.observe_batch(True, (time.perf_counter_ns() - t0) / 1e9, wasted)
+            return out
+        self.n_batched += 1
+        self._round_sequentially(s, xt, after, xs, activity)
+        self._read_integers(s, xt, xs)
+        if P is not None:
+            P.observe_batch(False, (time.perf_counter_ns() - t0) / 1e9, 0.0)
+        return np.array(xt)

@@ -58,3 +58,22 @@ def project_with(module, state_cls, milp, x_tilde, tol=1e-8, iteration_limit=1_0
 def l1_distance(milp, x, x_tilde) -> float:
     I = milp.int_idx
     return float(np.abs(np.asarray(x)[I] - np.asarray(x_tilde)[I]).sum())
+
+
+def oracle_projector_module():
+    """The oracle with the projector-module interface the pump expects (tests
+    only: it stands in for the device on the CPU)."""
+    from types import SimpleNamespace
+
+    from oracle import oracle as O
+
+    return SimpleNamespace(PDFPState=O.OraclePDFPState, set_target=O.set_target,
+                           warm_start=O.warm_start, pdfp_solve=O.pdfp_solve,
+                           original_primal_residual=O.original_primal_residual)
+
+
+def oracle_root_lp(milp, *, tolerance, time_limit, cupdlpx_termination):
+    from oracle import oracle as O
+
+    return O.solve_lp_relaxation(milp, tolerance=tolerance, time_limit=time_limit,
+                                 cupdlpx_termination=cupdlpx_termination)

@@ -1,0 +1,111 @@
+"""CPU tests of the host side of the pump: propagation, guarded CPR rounding
+and the pump loop (src/propagation.jl, src/rounding.jl, src/pump.jl).  The
+projector and the root LP are the oracle here (the device runs them in the
+GPU tests); the pump code under test is the product's."""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from pdfp_amd import generators as G
+from pdfp_amd.milp import MILP, is_feasible, read_mps
+from pdfp_amd.propagation import PropagationData, PropagationState, activities
+from pdfp_amd.pump import feasibility_pump, flip_farthest, round_to_nearest
+from pdfp_amd.rounding import Rounder
+
+from .conftest import example_path
+from .helpers import oracle_projector_module, oracle_root_lp
+
+
+def pump(milp, **kw):
+    return feasibility_pump(milp, projector=oracle_projector_module(), root_lp=oracle_root_lp,
+                            **kw)
+
+
+@pytest.mark.parametrize("name", ["p0201", "gen-ip054", "pk1", "markshare2"])
+def test_pump_on_examples(name):
+    """test/runtests.jl:73-79."""
+    m = read_mps(example_path(name))
+    r = pump(m, time_limit=120.0)
+    assert r.found
+    assert is_feasible(m, r.x)
+    assert abs(r.objective - (float(m.c @ r.x) + m.c0)) <= 1e-9 * (1 + abs(r.objective))
+    assert r.time >= r.root_lp_time and r.pdhg_iterations >= 0
+
+
+def test_pump_on_set_cover():
+    m = G.set_cover(133, 201, 1)
+    r = pump(m, time_limit=60.0, batch_policy="always")
+    assert r.found and is_feasible(m, r.x)
+    assert r.iterations >= 0 and r.log and "FEASIBLE" in r.log[-2]
+
+
+def test_pump_time_limit_reports_not_found_or_found():
+    m = read_mps(example_path("markshare2"))
+    r = pump(m, time_limit=0.0)
+    assert r.iterations == 0
+
+
+def test_rounding_invariants_p0201():
+    """test/runtests.jl:57-71."""
+    m = read_mps(example_path("p0201"))
+    r = pump(m, time_limit=60.0)
+    assert r.found
+    rng = np.random.Generator(np.random.PCG64(1))
+    assert np.array_equal(Rounder(m).round_guarded(r.x, rng, 0), r.x)
+    x = np.full(len(m.c), 0.5) + 0.01 * np.random.default_rng(2).random(len(m.c))
+    a = Rounder(m, policy=None).round_guarded(x, np.random.Generator(np.random.PCG64(3)), 20)
+    b = Rounder(m, policy=None).round_sequential(x, np.random.Generator(np.random.PCG64(3)), 20)
+    assert np.array_equal(a, b)
+    assert np.all(a == np.round(a))
+
+
+def _tiny_models():
+    rng = np.random.default_rng(5)
+    out = []
+    for k in range(12):
+        n = int(rng.integers(3, 7))
+        m = int(rng.integers(1, 4))
+        A = rng.integers(-3, 4, size=(m, n)).astype(float)
+        A[:, rng.integers(0, n)] += 1.0
+        u = rng.integers(1, 3, size=n).astype(float)
+        x0 = np.floor(rng.random(n) * (u + 1))
+        eq = rng.random(m) < 0.3
+        b = A @ x0 + np.where(eq, 0.0, rng.integers(0, 3, size=m))
+        out.append(MILP(c=np.zeros(n), c0=0.0, A=sp.csc_matrix(A), b=b,
+                        row_sense=np.where(eq, "eq", "le"), l=np.zeros(n), u=u,
+                        int_idx=np.arange(n)))
+    return out
+
+
+@pytest.mark.parametrize("idx", range(12))
+def test_propagation_is_sound_and_incremental(idx):
+    """Brute force on tiny models: after fixing x_0 to each value and
+    propagating, every feasible integer completion stays inside the domains,
+    and the incremental row activities equal a fresh recomputation."""
+    m = _tiny_models()[idx]
+    n = len(m.c)
+    d = PropagationData(m)
+    feasible = [np.array(p, float) for p in itertools.product(*[range(int(u) + 1) for u in m.u])
+                if is_feasible(m, np.array(p, float))]
+    for v in range(int(m.u[0]) + 1):
+        s = PropagationState(d, 10**9)
+        s.fix(0, float(v))
+        s.propagate()
+        Lmin, Lmax = activities(d, s.l, s.u)
+        np.testing.assert_allclose(s.L_min, Lmin, atol=1e-9)
+        np.testing.assert_allclose(s.L_max, Lmax, atol=1e-9)
+        for p in feasible:
+            if p[0] == v:
+                assert not s.infeasible
+                assert all(s.l[j] - 1e-9 <= p[j] <= s.u[j] + 1e-9 for j in range(n))
+
+
+def test_round_to_nearest_and_flip():
+    m = G.toy2()
+    assert list(round_to_nearest(np.array([2.5, 0.3]), m)) == [2.0, 0.3]   # ties to even
+    assert list(round_to_nearest(np.array([7.2, 0.0]), m)) == [5.0, 0.0]   # kept in bounds
+    xt = np.array([1.0, 0.0])
+    flip_farthest(xt, np.array([1.9, 0.0]), m, 5)
+    assert xt[0] == 2.0
