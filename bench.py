@@ -153,8 +153,10 @@ def main():
         it, dt, st = run_projections(s, pp, milp, targets, args.tol)
         ms_at, ms_a = s.time_kernels(200)
         ms_iter = s.time_iterations(400)
+        info = s.info()
         results[prec] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
-                             ms_iter=ms_iter, info=s.info())
+                             ms_iter=ms_iter, info=info,
+                             npb={"A": info.npb_a, "At": info.npb_at})
         print(f"[bench] {prec}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; iteration "
               f"{ms_iter * 1e3:.1f}us; {st}", file=sys.stderr, flush=True)
@@ -189,6 +191,8 @@ def main():
                      "traffic": None, "algorithmic_bytes": dom_bytes,
                      "kernel_ms": dom_ms},
         "iteration_us": r32["ms_iter"] * 1e3,
+        "schedule_nnz_per_block": r32["npb"],
+        "kernel_us": {"transpose_pass": r32["ms_at"] * 1e3, "row_pass": r32["ms_a"] * 1e3},
     }
     if "fp64" in results:
         r64 = results["fp64"]

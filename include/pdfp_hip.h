@@ -104,6 +104,8 @@ typedef struct pdfp_info {
   int32_t n_restarts;
   int32_t precision;
   int32_t n_evaluations;
+  int32_t npb_a;             /* nonzeros per workgroup chosen by the autotuner */
+  int32_t npb_at;            /*   for the A pass and for the Aᵀ pass            */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */

@@ -56,6 +56,8 @@ class PdfpInfo(C.Structure):
         ("n_restarts", C.c_int32),
         ("precision", C.c_int32),
         ("n_evaluations", C.c_int32),
+        ("npb_a", C.c_int32),
+        ("npb_at", C.c_int32),
     ]
 
 

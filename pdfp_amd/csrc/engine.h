@@ -1,27 +1,33 @@
 // CSR row-pass engine for gfx950: one launch walks the rows of a CSR matrix
 // in a precomputed row-block schedule, forms each row's inner product with a
-// gathered vector (or two), and hands the sums to a fused per-row epilogue.
+// gathered vector (or two, or three), and hands the sums to a fused per-row
+// epilogue.
 //
-// Schedule (built on the host by row_blocks in host_util.h): consecutive rows are packed
-// into a block while the block holds at most BLOCK rows and at most NPB
-// nonzeros.  A row longer than NPB sits alone in its block.
+// Schedule (built on the host by row_blocks in host_util.h): consecutive rows
+// are packed into a block while it holds at most BLOCK rows and at most NPB
+// nonzeros; a row longer than NPB sits alone in its block.  For every block
+// the host also picks `tpr`, the lanes that reduce one row (a power of two,
+// at most a wavefront), from the block's row lengths; tpr == 0 marks a
+// single long row.
 //
-//   stream mode  (block nnz <= NPB): the block streams its contiguous slice of
-//     (col, val) once, coalesced, forms the products a_ij * x_j in FP64 and
-//     stages them in LDS; then groups of `tpr` lanes (a power of two, chosen
-//     from the rows in the block, at most one wavefront) reduce one row each
-//     with xor-shuffles inside the group.
-//   vector mode  (one row, nnz > NPB): all BLOCK lanes stride the row and the
-//     row sum is a workgroup reduction.
+//   stream mode  (tpr > 0): the block's contiguous slice of (col, val) is
+//     streamed once, coalesced; the products a_ij·x_j are formed in FP64 and
+//     staged in LDS together with the row offsets.  Groups of tpr lanes then
+//     reduce the rows round-robin (xor-shuffles inside a group) into an LDS
+//     row-sum array, and finally lane t runs the epilogue of row t: the
+//     epilogue's operands were loaded into registers at the start of the
+//     kernel, so their latency overlaps the streaming.
+//   vector mode  (tpr == 0): all BLOCK lanes stride the one row and the sum
+//     is a workgroup reduction.
 //
 // Products of FP32 operands are exact in FP64 and every row sum accumulates
 // in FP64: on long power-law rows this replaces compensated FP32 summation.
-// With tpr == 1 each row is summed in ascending column order, the order of
-// the reference's CSC product.
+// With tpr == 1 a row is summed in ascending column order, the order of the
+// reference's CSC product.
 //
-// Per-block FP64 partial sums (NACC of them) of the epilogues' reduction
-// terms are written to `partials[block * NACC + k]`; a finalize kernel adds
-// them in block order, so the reductions are deterministic.
+// Per-block FP64 partials (NACC terms) of the epilogues' reductions are
+// written to partials[block·NACC + k] and combined by finalize_partials in
+// block order, so every reduction is deterministic run to run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,6 +43,7 @@ struct Csr {
   const int32_t* __restrict__ col;     // nnz
   const TV* __restrict__ val;          // nnz
   const int32_t* __restrict__ blocks;  // nblocks + 1 row boundaries
+  const uint8_t* __restrict__ tpr;     // nblocks: lanes per row (0: one long row)
   int32_t nblocks;
   int32_t nrows;
 };
@@ -60,6 +67,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
 // Sum of v over the block, returned on thread 0 (other threads: garbage).
 template <int BLOCK>
 __device__ __forceinline__ double block_sum_to0(double v, double* red) {
@@ -76,12 +89,6 @@ __device__ __forceinline__ double block_sum_to0(double v, double* red) {
     for (int w = 0; w < kW; ++w) s += red[w];
   }
   return s;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
-  return v;
 }
 
 // Writes the block's NACC partials: terms [0, NACC-NMAX) are sums, the last
@@ -103,12 +110,13 @@ __device__ __forceinline__ void block_store_partials(const double (&acc)[NACC],
     const bool is_max = (int)threadIdx.x >= NACC - NMAX;
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kW; ++w) s = is_max ? fmax(s, red[w][threadIdx.x]) : s + red[w][threadIdx.x];
+    for (int w = 0; w < kW; ++w)
+      s = is_max ? fmax(s, red[w][threadIdx.x]) : s + red[w][threadIdx.x];
     out[threadIdx.x] = s;
   }
 }
 
-// ---- gathers: the per-nonzero products ----------------------------------
+// ---- gathers: the per-nonzero products --------------------------------------
 
 // p0 = a * x[c]
 template <typename TX>
@@ -133,29 +141,48 @@ struct GatherDelta {
   }
 };
 
-// ---- the pass ---------------------------------------------------------------
+// ---- epilogue interface ------------------------------------------------------
+// struct Epi {
+//   struct Pre;                                        // per-row operands
+//   __device__ Pre pre(int row) const;                 // loads, issued early
+//   __device__ void operator()(int row, const Pre&, const double* sums,
+//                              double* acc) const;     // math + stores
+// };
 
-// Epi: __device__ void operator()(int row, const double* sums, double* acc)
+struct NoPre {};
+
+// ---- the pass ------------------------------------------------------------------
+
 template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX = 0>
 __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
                                                   double* __restrict__ partials) {
   constexpr int NV = Gather::NV;
-  constexpr int U = 4;  // nonzeros in flight per lane while staging
+  constexpr int U = (NPB / BLOCK) < 4 ? (NPB / BLOCK > 0 ? NPB / BLOCK : 1) : 4;
+  constexpr int NA = NACC > 0 ? NACC : 1;
   __shared__ double stage[NV][NPB];
+  __shared__ double rowsum[NV][BLOCK];
+  __shared__ int off[BLOCK + 1];
   __shared__ double red[BLOCK / kWave];
 
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int r0 = A.blocks[b];
   const int r1 = A.blocks[b + 1];
-  const int base = A.rowptr[r0];
-  const int cnt = A.rowptr[r1] - base;
+  const int tpr = A.tpr[b];
+  const int nrows = r1 - r0;
 
-  double acc[NACC > 0 ? NACC : 1];
+  // The epilogue's operands: in flight while the row slice streams in.
+  typename Epi::Pre pre{};
+  if (tid < nrows) pre = epi.pre(r0 + tid);
+
+  double acc[NA];
 #pragma unroll
-  for (int k = 0; k < (NACC > 0 ? NACC : 1); ++k) acc[k] = 0.0;
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
 
-  if (cnt <= NPB) {
+  if (tpr > 0) {
+    const int base = A.rowptr[r0];
+    for (int i = tid; i <= nrows; i += BLOCK) off[i] = A.rowptr[r0 + i] - base;
+    const int cnt = A.rowptr[r1] - base;
     for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
       int cc[U];
       double vv[U];
@@ -177,42 +204,53 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
       }
     }
     __syncthreads();
-    const int nrows = r1 - r0;
-    int tpr = 1;
-    while (tpr < kWave && tpr * 2 * nrows <= BLOCK) tpr *= 2;
+    const int groups = BLOCK / tpr;
     const int grp = tid / tpr;
     const int lane = tid & (tpr - 1);
-    const int row = r0 + grp;
-    double s[NV];
+    // Every lane of a group walks the same rows, so the group stays
+    // converged around its shuffles.
+    for (int lr = grp; lr < nrows; lr += groups) {
+      double s[NV];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) s[q] = 0.0;
-    if (row < r1) {
-      const int k1 = A.rowptr[row + 1] - base;
-      for (int k = A.rowptr[row] - base + lane; k < k1; k += tpr) {
+      for (int q = 0; q < NV; ++q) s[q] = 0.0;
+      const int k1 = off[lr + 1];
+      for (int k = off[lr] + lane; k < k1; k += tpr) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) s[q] += stage[q][k];
       }
-    }
-    for (int off = tpr >> 1; off > 0; off >>= 1) {
+      for (int o = tpr >> 1; o > 0; o >>= 1) {
 #pragma unroll
-      for (int q = 0; q < NV; ++q) s[q] += __shfl_xor(s[q], off, tpr);
+        for (int q = 0; q < NV; ++q) s[q] += __shfl_xor(s[q], o, tpr);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) rowsum[q][lr] = s[q];
+      }
     }
-    if (lane == 0 && row < r1) epi(row, s, acc);
+    __syncthreads();
+    if (tid < nrows) {
+      double s[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) s[q] = rowsum[q][tid];
+      epi(r0 + tid, pre, s, acc);
+    }
   } else {
+    const int base = A.rowptr[r0];
+    const int cnt = A.rowptr[r1] - base;
     double s[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = 0.0;
-    for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
-      int cc[U];
-      double vv[U];
+    for (int k0 = 0; k0 < cnt; k0 += BLOCK * 4) {
+      int cc[4];
+      double vv[4];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int k = k0 + u * BLOCK + tid;
         cc[u] = k < cnt ? A.col[base + k] : 0;
         vv[u] = k < cnt ? (double)A.val[base + k] : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int k = k0 + u * BLOCK + tid;
         if (k < cnt) {
           double p[NV];
@@ -225,7 +263,7 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
     double t[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) t[q] = block_sum_to0<BLOCK>(s[q], red);
-    if (tid == 0) epi(r0, t, acc);
+    if (tid == 0) epi(r0, pre, t, acc);
   }
   if constexpr (NACC > 0) {
     block_store_partials<BLOCK, NACC, NMAX>(acc, partials + (size_t)b * NACC);

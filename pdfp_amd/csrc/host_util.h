@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <stdexcept>
@@ -175,6 +176,45 @@ inline std::vector<int32_t> row_blocks(const std::vector<int32_t>& rowptr, int m
     blocks.push_back((int32_t)r);
   }
   return blocks;
+}
+
+// Lanes per row for each block of a schedule: the power of two tpr ≤ 64 that
+// minimises the block's longest group, where the BLOCK/tpr groups take the
+// rows round-robin and a row costs ⌈len/tpr⌉ adds plus log2(tpr)+1 shuffle
+// steps; ties go to the smaller tpr.  0 marks a block that is one row longer
+// than max_nnz (the whole workgroup reduces it).
+inline std::vector<uint8_t> block_lanes(const std::vector<int32_t>& rowptr,
+                                        const std::vector<int32_t>& blocks, int block,
+                                        int max_nnz) {
+  const size_t nb = blocks.size() - 1;
+  std::vector<uint8_t> tpr(nb, 1);
+  std::vector<int64_t> group_cost;
+  for (size_t k = 0; k < nb; ++k) {
+    const int32_t r0 = blocks[k], r1 = blocks[k + 1];
+    const int64_t cnt = rowptr[(size_t)r1] - rowptr[(size_t)r0];
+    if (r1 - r0 == 1 && cnt > max_nnz) {
+      tpr[k] = 0;
+      continue;
+    }
+    int64_t best_cost = -1;
+    int best = 1;
+    for (int t = 1, lg = 0; t <= 64 && t <= block; t *= 2, ++lg) {
+      const int groups = block / t;
+      group_cost.assign((size_t)groups, 0);
+      for (int32_t r = r0; r < r1; ++r) {
+        const int64_t len = rowptr[(size_t)r + 1] - rowptr[(size_t)r];
+        group_cost[(size_t)((r - r0) % groups)] += (len + t - 1) / t + lg + 1;
+      }
+      int64_t c = 0;
+      for (int64_t v : group_cost) c = std::max(c, v);
+      if (best_cost < 0 || c < best_cost) {
+        best_cost = c;
+        best = t;
+      }
+    }
+    tpr[k] = (uint8_t)best;
+  }
+  return tpr;
 }
 
 }  // namespace pdfp

@@ -66,7 +66,7 @@ struct Projector final : ProjectorBase {
   PinnedBuf<StepParams> h_sp;
   PinnedBuf<double> h_sums;
 
-  std::vector<double> h_w, h_xt;  // w (0 off I) and x̃, FP64, for denominators
+  std::vector<double> h_w, h_xt, h_b;  // w (0 off I), x̃ and b in FP64
   bool orig_primal = false;
   double b_norm = 0.0, b_norm_original = NAN;
 
@@ -91,6 +91,9 @@ struct Projector final : ProjectorBase {
     nnz = (int64_t)ha.col.size();
     A.upload(ha, stream);
     At.upload(ht, stream);
+    // pick the nonzeros-per-block of each operator's schedule by timing
+    autotune_schedule(A, n, stream);
+    autotune_schedule(At, m, stream);
 
     h_w.assign((size_t)n, 0.0);
     for (int64_t j = 0; j < n; ++j) h_w[(size_t)j] = is_int_[j] ? w_[j] : 0.0;
@@ -98,6 +101,7 @@ struct Projector final : ProjectorBase {
       if (is_int_[j] && !(w_[j] > 0.0))
         throw Error(PDFP_ERR_ARGUMENT, "weights of integer columns must be positive");
     h_xt.assign((size_t)n, 0.0);
+    h_b.assign(b_, b_ + m);
     double acc = 0.0;
     for (int64_t i = 0; i < m; ++i) acc += b_[i] * b_[i];
     b_norm = std::sqrt(acc);
@@ -186,10 +190,8 @@ struct Projector final : ProjectorBase {
     double acc = 0.0;
     for (int64_t i = 0; i < m; ++i) inv[(size_t)i] = 1.0 / D_r[i];
     // b_norm_original = ‖b ./ D_r‖ of the FP64 b (src/pdfp.jl:87)
-    std::vector<double> hb((size_t)m);
-    download_T(b, hb.data(), m);
     for (int64_t i = 0; i < m; ++i) {
-      const double v = hb[(size_t)i] / D_r[i];
+      const double v = h_b[(size_t)i] / D_r[i];
       acc += v * v;
     }
     b_norm_original = std::sqrt(acc);
@@ -465,6 +467,8 @@ struct Projector final : ProjectorBase {
     o->n_restarts = n_restarts;
     o->precision = kFused ? PDFP_PREC_FP32_MIXED : PDFP_PREC_FP64;
     o->n_evaluations = n_evals;
+    o->npb_a = A.npb;
+    o->npb_at = At.npb;
   }
 
   void spmv(int transpose, const double* xin, double* yout) override {

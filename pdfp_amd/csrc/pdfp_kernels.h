@@ -53,6 +53,8 @@ __device__ __forceinline__ double jclamp(double x, double lo, double hi) {
 }
 
 // ---- fused epilogues (precision-generic iterates T, FP64 arithmetic) ------
+// Each epilogue loads its per-row operands in pre() (issued by the engine at
+// the start of the pass) and does the math and the stores in operator().
 
 // Column j of A (row j of Aᵀ): s[0] = (Aᵀz)_j.  Fuses K1 (primal step,
 // pdfp_kernels.jl:4-20) and K2 (dual-y step, pdfp_kernels.jl:33-48).
@@ -60,6 +62,9 @@ __device__ __forceinline__ double jclamp(double x, double lo, double hi) {
 // only the evaluation and the restart read.
 template <typename T, bool MAJOR>
 struct EpiPrimalY {
+  struct Pre {
+    T x, y, x0, y0, lb, ub, xt, w;
+  };
   const StepParams* sp;
   int k_offset;
   T* __restrict__ x;
@@ -75,19 +80,23 @@ struct EpiPrimalY {
   T* __restrict__ ds;
   T* __restrict__ py;
   T* __restrict__ ry;
-  __device__ __forceinline__ void operator()(int j, const double* s, double*) const {
+  __device__ __forceinline__ Pre pre(int j) const {
+    return {x[j], y[j], x0[j], y0[j], lb[j], ub[j], xt[j], w[j]};
+  }
+  __device__ __forceinline__ void operator()(int j, const Pre& p, const double* s,
+                                             double*) const {
     const StepScalars c = step_scalars(sp, k_offset);
-    const double cx = (double)x[j];
-    const double cy = (double)y[j];
+    const double cx = (double)p.x;
+    const double cy = (double)p.y;
     const double temp = cx - c.tau * (s[0] - cy);
-    const double pxj = jclamp(temp, (double)lb[j], (double)ub[j]);
+    const double pxj = jclamp(temp, (double)p.lb, (double)p.ub);
     const double rxj = 2.0 * pxj - cx;
-    x[j] = (T)(c.weight * (c.omega * rxj + c.omw * cx) + c.omwt * (double)x0[j]);
+    x[j] = (T)(c.weight * (c.omega * rxj + c.omw * cx) + c.omwt * (double)p.x0);
     rx[j] = (T)rxj;
-    const double wj = (double)w[j];
-    const double pyj = wj > 0.0 ? jclamp(cy + c.sigma * ((double)xt[j] - rxj), -wj, wj) : 0.0;
+    const double wj = (double)p.w;
+    const double pyj = wj > 0.0 ? jclamp(cy + c.sigma * ((double)p.xt - rxj), -wj, wj) : 0.0;
     const double ryj = 2.0 * pyj - cy;
-    y[j] = (T)(c.weight * (c.omega * ryj + c.omw * cy) + c.omwt * (double)y0[j]);
+    y[j] = (T)(c.weight * (c.omega * ryj + c.omw * cy) + c.omwt * (double)p.y0);
     if (MAJOR) {
       px[j] = (T)pxj;
       ds[j] = (T)((pxj - temp) / c.tau);
@@ -100,6 +109,10 @@ struct EpiPrimalY {
 // Row i of A: s[0] = (A x̄)_i.  Fuses K3 (dual-z step, pdfp_kernels.jl:60-75).
 template <typename T, bool MAJOR>
 struct EpiDualZ {
+  struct Pre {
+    T z, z0, b;
+    uint8_t eq;
+  };
   const StepParams* sp;
   int k_offset;
   T* __restrict__ z;
@@ -108,13 +121,15 @@ struct EpiDualZ {
   const uint8_t* __restrict__ is_eq;
   T* __restrict__ pz;
   T* __restrict__ rz;
-  __device__ __forceinline__ void operator()(int i, const double* s, double*) const {
+  __device__ __forceinline__ Pre pre(int i) const { return {z[i], z0[i], b[i], is_eq[i]}; }
+  __device__ __forceinline__ void operator()(int i, const Pre& p, const double* s,
+                                             double*) const {
     const StepScalars c = step_scalars(sp, k_offset);
-    const double cz = (double)z[i];
-    const double raw = cz + c.sigma * (s[0] - (double)b[i]);
-    const double pzi = is_eq[i] ? raw : (raw > 0.0 ? raw : 0.0);
+    const double cz = (double)p.z;
+    const double raw = cz + c.sigma * (s[0] - (double)p.b);
+    const double pzi = p.eq ? raw : (raw > 0.0 ? raw : 0.0);
     const double rzi = 2.0 * pzi - cz;
-    z[i] = (T)(c.weight * (c.omega * rzi + c.omw * cz) + c.omwt * (double)z0[i]);
+    z[i] = (T)(c.weight * (c.omega * rzi + c.omw * cz) + c.omwt * (double)p.z0);
     if (MAJOR) {
       pz[i] = (T)pzi;
       rz[i] = (T)rzi;
@@ -132,6 +147,9 @@ struct EpiDualZ {
 constexpr int kEvalColTerms = 8;
 template <typename T>
 struct EpiEvalCol {
+  struct Pre {
+    T px, rx, py, ry, ds, xt, w, x0, y0;
+  };
   const T* __restrict__ px;
   const T* __restrict__ rx;
   const T* __restrict__ py;
@@ -141,23 +159,27 @@ struct EpiEvalCol {
   const T* __restrict__ w;
   const T* __restrict__ x0;
   const T* __restrict__ y0;
-  __device__ __forceinline__ void operator()(int j, const double* s, double* acc) const {
-    const double pxj = (double)px[j], pyj = (double)py[j];
-    const double dx = pxj - (double)rx[j];
-    const double dy = pyj - (double)ry[j];
+  __device__ __forceinline__ Pre pre(int j) const {
+    return {px[j], rx[j], py[j], ry[j], ds[j], xt[j], w[j], x0[j], y0[j]};
+  }
+  __device__ __forceinline__ void operator()(int, const Pre& p, const double* s,
+                                             double* acc) const {
+    const double pxj = (double)p.px, pyj = (double)p.py;
+    const double dx = pxj - (double)p.rx;
+    const double dy = pyj - (double)p.ry;
     acc[0] += dx * dx;
     acc[1] += dy * dy;
     acc[2] += (s[1] - dy) * dx;
-    const double r = s[0] - pyj - (double)ds[j];
+    const double r = s[0] - pyj - (double)p.ds;
     acc[3] += r * r;
-    const double wj = (double)w[j];
+    const double wj = (double)p.w;
     if (wj > 0.0) {
-      const double xtj = (double)xt[j];
+      const double xtj = (double)p.xt;
       acc[4] += wj * fabs(pxj - xtj);
       acc[5] += pyj * (xtj - pxj);
     }
-    const double ex = (double)x0[j] - pxj;
-    const double ey = (double)y0[j] - pyj;
+    const double ex = (double)p.x0 - pxj;
+    const double ey = (double)p.y0 - pyj;
     acc[6] += ex * ex;
     acc[7] += ey * ey;
   }
@@ -169,37 +191,32 @@ struct EpiEvalCol {
 constexpr int kEvalRowTerms = 4;
 template <typename T>
 struct EpiEvalRow {
+  struct Pre {
+    T pz, rz, b, z0;
+    uint8_t eq;
+    double ru;
+  };
   const T* __restrict__ pz;
   const T* __restrict__ rz;
   const T* __restrict__ b;
   const T* __restrict__ z0;
   const uint8_t* __restrict__ is_eq;
   const double* __restrict__ row_unscale;  // null: scaled rows
-  __device__ __forceinline__ void operator()(int i, const double* s, double* acc) const {
-    const double pzi = (double)pz[i];
-    const double dz = pzi - (double)rz[i];
+  __device__ __forceinline__ Pre pre(int i) const {
+    return {pz[i], rz[i], b[i], z0[i], is_eq[i], row_unscale ? row_unscale[i] : 1.0};
+  }
+  __device__ __forceinline__ void operator()(int, const Pre& p, const double* s,
+                                             double* acc) const {
+    const double pzi = (double)p.pz;
+    const double dz = pzi - (double)p.rz;
     acc[0] += dz * dz;
-    const double d = s[0] - (double)b[i];
-    double r = is_eq[i] ? fabs(d) : (d > 0.0 ? d : 0.0);
-    if (row_unscale) r *= row_unscale[i];
+    const double d = s[0] - (double)p.b;
+    double r = p.eq ? fabs(d) : (d > 0.0 ? d : 0.0);
+    if (row_unscale) r *= p.ru;
     acc[1] += r * r;
     acc[2] += pzi * d;
-    const double ez = (double)z0[i] - pzi;
+    const double ez = (double)p.z0 - pzi;
     acc[3] += ez * ez;
-  }
-};
-
-// Primal residual of an FP64 point (the polished projection).
-template <typename T>
-struct EpiViolation {
-  const T* __restrict__ b;
-  const uint8_t* __restrict__ is_eq;
-  const double* __restrict__ row_unscale;
-  __device__ __forceinline__ void operator()(int i, const double* s, double* acc) const {
-    const double d = s[0] - (double)b[i];
-    double r = is_eq[i] ? fabs(d) : (d > 0.0 ? d : 0.0);
-    if (row_unscale) r *= row_unscale[i];
-    acc[0] += r * r;
   }
 };
 

@@ -69,6 +69,9 @@ struct LpSolver final : LpBase {
     nnz = (int64_t)ha.col.size();
     A.upload(ha, stream);
     At.upload(ht, stream);
+    // pick the nonzeros-per-block of each operator's schedule by timing
+    autotune_schedule(A, n, stream);
+    autotune_schedule(At, m, stream);
     h_c.assign(c_, c_ + n);
     h_clb.assign(clb_, clb_ + m);
     h_cub.assign(cub_, cub_ + m);

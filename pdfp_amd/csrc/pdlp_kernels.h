@@ -22,6 +22,9 @@ __device__ __forceinline__ bool finite(double v) { return fabs(v) < INFINITY; }
 // Transpose pass of one iteration (K4, src/pdlp_kernels.jl:4-21).
 template <typename T, bool MAJOR>
 struct EpiLpPrimal {
+  struct Pre {
+    T x, x0, c, lb, ub;
+  };
   const StepParams* sp;
   int k_offset;
   T* __restrict__ x;
@@ -32,14 +35,16 @@ struct EpiLpPrimal {
   T* __restrict__ rx;
   T* __restrict__ px;
   T* __restrict__ ds;
-  __device__ __forceinline__ void operator()(int j, const double* s, double*) const {
+  __device__ __forceinline__ Pre pre(int j) const { return {x[j], x0[j], c[j], lb[j], ub[j]}; }
+  __device__ __forceinline__ void operator()(int j, const Pre& q, const double* s,
+                                             double*) const {
     const StepScalars sc = step_scalars(sp, k_offset);
-    const double cx = (double)x[j];
-    const double temp = cx - sc.tau * ((double)c[j] - s[0]);
-    const double p = jclamp(temp, (double)lb[j], (double)ub[j]);
+    const double cx = (double)q.x;
+    const double temp = cx - sc.tau * ((double)q.c - s[0]);
+    const double p = jclamp(temp, (double)q.lb, (double)q.ub);
     const double r = 2.0 * p - cx;
     rx[j] = (T)r;
-    x[j] = (T)(sc.weight * (sc.omega * r + sc.omw * cx) + sc.omwt * (double)x0[j]);
+    x[j] = (T)(sc.weight * (sc.omega * r + sc.omw * cx) + sc.omwt * (double)q.x0);
     if (MAJOR) {
       px[j] = (T)p;
       ds[j] = (T)((p - temp) / sc.tau);
@@ -50,6 +55,9 @@ struct EpiLpPrimal {
 // A pass of one iteration (K5, src/pdlp_kernels.jl:35-51).
 template <typename T, bool MAJOR>
 struct EpiLpDual {
+  struct Pre {
+    T y, y0, clb, cub;
+  };
   const StepParams* sp;
   int k_offset;
   T* __restrict__ y;
@@ -58,13 +66,15 @@ struct EpiLpDual {
   const T* __restrict__ cub;
   T* __restrict__ py;
   T* __restrict__ ry;
-  __device__ __forceinline__ void operator()(int i, const double* s, double*) const {
+  __device__ __forceinline__ Pre pre(int i) const { return {y[i], y0[i], clb[i], cub[i]}; }
+  __device__ __forceinline__ void operator()(int i, const Pre& q, const double* s,
+                                             double*) const {
     const StepScalars sc = step_scalars(sp, k_offset);
-    const double cy = (double)y[i];
+    const double cy = (double)q.y;
     const double t = cy / sc.sigma - s[0];
-    const double p = (t - jclamp(t, -(double)cub[i], -(double)clb[i])) * sc.sigma;
+    const double p = (t - jclamp(t, -(double)q.cub, -(double)q.clb)) * sc.sigma;
     const double r = 2.0 * p - cy;
-    y[i] = (T)(sc.weight * (sc.omega * r + sc.omw * cy) + sc.omwt * (double)y0[i]);
+    y[i] = (T)(sc.weight * (sc.omega * r + sc.omw * cy) + sc.omwt * (double)q.y0);
     if (MAJOR) {
       py[i] = (T)p;
       ry[i] = (T)r;
@@ -133,6 +143,10 @@ constexpr int kLpColTerms = 10;
 constexpr int kLpColMax = 2;
 template <typename T, bool RAYS>
 struct EpiLpEvalCol {
+  struct Pre {
+    T px, rx, ds, c, x0, lb, ub;
+    double cu;
+  };
   const T* __restrict__ px;
   const T* __restrict__ rx;
   const T* __restrict__ ds;
@@ -141,28 +155,34 @@ struct EpiLpEvalCol {
   const T* __restrict__ lb;
   const T* __restrict__ ub;
   const double* __restrict__ col_unscale;  // null: scaled units
-  __device__ __forceinline__ void operator()(int j, const double* s, double* acc) const {
-    const double pxj = (double)px[j];
-    const double dx = pxj - (double)rx[j];
+  __device__ __forceinline__ Pre pre(int j) const {
+    return {px[j], rx[j], ds[j], c[j], x0[j], lb[j], ub[j], col_unscale ? col_unscale[j] : 1.0};
+  }
+  __device__ __forceinline__ void operator()(int, const Pre& q, const double* s,
+                                             double* acc) const {
+    const double pxj = (double)q.px;
+    const double dx = pxj - (double)q.rx;
     acc[0] += dx * dx;
     acc[1] += s[1] * dx;
-    const double cj = (double)c[j], dsj = (double)ds[j];
+    const double cj = (double)q.c, dsj = (double)q.ds;
     double r = cj - s[0] - dsj;
-    if (col_unscale) r *= col_unscale[j];
+    if (col_unscale) r *= q.cu;
     acc[2] += r * r;
     acc[3] += cj * pxj;
     acc[4] += dsj * pxj;
-    const double ex = (double)x0[j] - pxj;
+    const double ex = (double)q.x0 - pxj;
     acc[5] += ex * ex;
     if constexpr (RAYS) {
-      const double d = primal_ray(px, rx, lb, ub, j);
+      const double lo = (double)q.lb, hi = (double)q.ub;
+      double d = (double)q.rx - pxj;
+      if (finite(lo)) d = fmax(d, 0.0);
+      if (finite(hi)) d = fmin(d, 0.0);
       acc[6] += cj * d;
       acc[8] = fmax(acc[8], fabs(d));
-      const double lo = (double)lb[j], hi = (double)ub[j];
       const double dp = s[2];
       acc[7] += fmax(-dp, 0.0) * (finite(lo) ? lo : 0.0) + fmin(-dp, 0.0) * (finite(hi) ? hi : 0.0);
       double v = (!finite(lo) ? fmax(0.0, -dp) : 0.0) - (!finite(hi) ? fmin(0.0, -dp) : 0.0);
-      v *= col_unscale[j];
+      v *= q.cu;
       acc[9] = fmax(acc[9], fabs(v));
     }
   }
@@ -176,32 +196,42 @@ constexpr int kLpRowTerms = 7;
 constexpr int kLpRowMax = 2;
 template <typename T, bool RAYS>
 struct EpiLpEvalRow {
+  struct Pre {
+    T py, ry, y0, clb, cub;
+    double ru;
+  };
   const T* __restrict__ py;
   const T* __restrict__ ry;
   const T* __restrict__ y0;
   const T* __restrict__ clb;
   const T* __restrict__ cub;
   const double* __restrict__ row_unscale;  // null: scaled units
-  __device__ __forceinline__ void operator()(int i, const double* s, double* acc) const {
-    const double pyi = (double)py[i];
-    const double dy = pyi - (double)ry[i];
+  __device__ __forceinline__ Pre pre(int i) const {
+    return {py[i], ry[i], y0[i], clb[i], cub[i], row_unscale ? row_unscale[i] : 1.0};
+  }
+  __device__ __forceinline__ void operator()(int, const Pre& q, const double* s,
+                                             double* acc) const {
+    const double pyi = (double)q.py;
+    const double dy = pyi - (double)q.ry;
     acc[0] += dy * dy;
-    const double lo = (double)clb[i], hi = (double)cub[i];
+    const double lo = (double)q.clb, hi = (double)q.cub;
     double r = s[0] - jclamp(s[0], lo, hi);
-    if (row_unscale) r *= row_unscale[i];
+    if (row_unscale) r *= q.ru;
     acc[1] += r * r;
     acc[2] += fmax(pyi, 0.0) * (fabs(lo) < 1e19 ? lo : 0.0) +
               fmin(pyi, 0.0) * (fabs(hi) < 1e19 ? hi : 0.0);
-    const double ey = (double)y0[i] - pyi;
+    const double ey = (double)q.y0 - pyi;
     acc[3] += ey * ey;
     if constexpr (RAYS) {
-      const double v = dual_ray(py, ry, clb, cub, i);
+      double v = (double)q.ry - pyi;
+      if (!finite(lo)) v = fmin(v, 0.0);
+      if (!finite(hi)) v = fmax(v, 0.0);
       acc[4] += fmax(v, 0.0) * (finite(lo) ? lo : 0.0) + fmin(v, 0.0) * (finite(hi) ? hi : 0.0);
       acc[5] = fmax(acc[5], fabs(v));
       const double ad = s[1];
-      double q = (finite(lo) ? fmax(0.0, -ad) : 0.0) + (finite(hi) ? fmax(0.0, ad) : 0.0);
-      q *= row_unscale[i];
-      acc[6] = fmax(acc[6], fabs(q));
+      double qv = (finite(lo) ? fmax(0.0, -ad) : 0.0) + (finite(hi) ? fmax(0.0, ad) : 0.0);
+      qv *= q.ru;
+      acc[6] = fmax(acc[6], fabs(qv));
     }
   }
 };

@@ -1,7 +1,7 @@
 // Host-side pieces shared by the projector and the root-LP solver: the
 // restart rule and PID primal weight (src/pdlp.jl:167-205), the device CSR
-// with its row-block schedule, and the power iteration for ‖A‖₂
-// (src/preconditioner.jl:108-125).
+// with its row-block schedule and the schedule autotuner, and the power
+// iteration for ‖A‖₂ (src/preconditioner.jl:108-125).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,9 +19,10 @@
 namespace pdfp {
 
 constexpr int kBlock = 256;  // threads per workgroup of the CSR passes
-constexpr int kNpb = 2048;   // nonzeros staged per workgroup
 constexpr uint64_t kSvSeed = 20240517ull;
 constexpr double kJuliaEps = 2.220446049250313e-16;
+// Nonzeros staged per workgroup: the autotuner picks one per operator.
+constexpr int kNpbChoices[] = {256, 512, 1024, 2048};
 
 extern std::string g_create_error;  // message of the last failed *_create
 
@@ -88,9 +89,8 @@ inline void update_primal_weight(RestartState& s, const pdfp_params& rp, double 
 
 // SplitMix64 + Box–Muller normals: the fixed start of the power iteration
 // (the reference draws randn(MersenneTwister(20240517), m); this stream is
-// the library's own and documented here: u1 ∈ (0,1], u2 ∈ [0,1) from the top
-// 53 bits of consecutive SplitMix64 outputs, r = √(-2 ln u1),
-// (r cos 2πu2, r sin 2πu2)).
+// the library's own: u1 ∈ (0,1], u2 ∈ [0,1) from the top 53 bits of
+// consecutive SplitMix64 outputs, r = √(-2 ln u1), (r cos 2πu2, r sin 2πu2)).
 inline uint64_t splitmix64(uint64_t& s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -114,14 +114,17 @@ inline void start_vector(int64_t m, uint64_t seed, double* out) {
 template <typename TV>
 struct DevCsr {
   DevBuf<int32_t> rowptr, col, blocks;
+  DevBuf<uint8_t> tpr;
   DevBuf<TV> val;
   int32_t nblocks = 0, nrows = 0;
+  int32_t npb = 512;  // nonzeros per stream-mode block of the current schedule
   int64_t nnz = 0;
+  std::vector<int32_t> h_rowptr;  // kept for re-scheduling
+
   void upload(const HostCsr& h, hipStream_t s) {
     nrows = (int32_t)h.nrows;
     nnz = (int64_t)h.col.size();
-    std::vector<int32_t> blk = row_blocks(h.rowptr, kBlock, kNpb);
-    nblocks = (int32_t)blk.size() - 1;
+    h_rowptr = h.rowptr;
     rowptr.alloc(h.rowptr.size());
     rowptr.upload(h.rowptr.data(), h.rowptr.size(), s);
     col.alloc(std::max<size_t>(h.col.size(), 1));
@@ -130,20 +133,54 @@ struct DevCsr {
     for (size_t k = 0; k < v.size(); ++k) v[k] = (TV)h.val[k];
     val.alloc(std::max<size_t>(v.size(), 1));
     val.upload(v.data(), v.size(), s);
+    PDFP_HIP(hipStreamSynchronize(s));  // the host vectors die with this scope
+    schedule(npb, s);
+  }
+
+  // Row blocks of at most kBlock rows and `max_nnz` nonzeros, with their
+  // lanes per row.
+  void schedule(int max_nnz, hipStream_t s) {
+    npb = max_nnz;
+    std::vector<int32_t> blk = row_blocks(h_rowptr, kBlock, max_nnz);
+    std::vector<uint8_t> lanes = block_lanes(h_rowptr, blk, kBlock, max_nnz);
+    nblocks = (int32_t)blk.size() - 1;
     blocks.alloc(blk.size());
     blocks.upload(blk.data(), blk.size(), s);
-    PDFP_HIP(hipStreamSynchronize(s));  // the host vectors die with this scope
+    tpr.alloc(std::max<size_t>(lanes.size(), 1));
+    tpr.upload(lanes.data(), lanes.size(), s);
+    PDFP_HIP(hipStreamSynchronize(s));
   }
-  Csr<TV> view() const { return Csr<TV>{rowptr.p, col.p, val.p, blocks.p, nblocks, nrows}; }
-  size_t bytes() const { return rowptr.bytes() + col.bytes() + blocks.bytes() + val.bytes(); }
+
+  Csr<TV> view() const {
+    return Csr<TV>{rowptr.p, col.p, val.p, blocks.p, tpr.p, nblocks, nrows};
+  }
+  size_t bytes() const {
+    return rowptr.bytes() + col.bytes() + blocks.bytes() + tpr.bytes() + val.bytes();
+  }
 };
 
-// Launch one CSR pass (no-op on an empty operator).
+// Launch one CSR pass with the operator's schedule (no-op if it is empty).
 template <int NACC, int NMAX = 0, typename TV, class G, class E>
 inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
   if (A.nblocks == 0) return;
-  csr_pass<kBlock, kNpb, TV, G, E, NACC, NMAX>
-      <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+  switch (A.npb) {
+    case 256:
+      csr_pass<kBlock, 256, TV, G, E, NACC, NMAX>
+          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+      break;
+    case 512:
+      csr_pass<kBlock, 512, TV, G, E, NACC, NMAX>
+          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+      break;
+    case 1024:
+      csr_pass<kBlock, 1024, TV, G, E, NACC, NMAX>
+          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+      break;
+    default:
+      csr_pass<kBlock, 2048, TV, G, E, NACC, NMAX>
+          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+      break;
+  }
 }
 
 // Power-iteration helper kernels (src/preconditioner.jl:113-123); static so
@@ -183,33 +220,67 @@ static __global__ __launch_bounds__(256) void k_sumsq(int m, const double* __res
 // out[row] = s[0]
 template <typename TO>
 struct EpiStore {
+  using Pre = NoPre;
   TO* __restrict__ out;
-  __device__ __forceinline__ void operator()(int r, const double* s, double*) const {
+  __device__ __forceinline__ Pre pre(int) const { return {}; }
+  __device__ __forceinline__ void operator()(int r, const Pre&, const double* s, double*) const {
     out[r] = (TO)s[0];
   }
 };
 
 // out[row] = s[0];  acc[0] += v[row] * s[0]   (power iteration: eig = A d, ⟨nxt, eig⟩)
 struct EpiStoreDot {
+  struct Pre {
+    double v;
+  };
   double* __restrict__ out;
   const double* __restrict__ v;
-  __device__ __forceinline__ void operator()(int r, const double* s, double* acc) const {
+  __device__ __forceinline__ Pre pre(int r) const { return {v[r]}; }
+  __device__ __forceinline__ void operator()(int r, const Pre& p, const double* s,
+                                             double* acc) const {
     out[r] = s[0];
-    acc[0] += v[r] * s[0];
+    acc[0] += p.v * s[0];
   }
 };
 
-// Scratch for the small reductions of one solver context.
-struct Reducer {
-  DevBuf<double> part;   // per-block partials
-  DevBuf<double> sums;   // 64 finalized values
-  PinnedBuf<double> host;
-  void alloc(size_t partial_slots) {
-    part.alloc(std::max<size_t>(partial_slots, 16));
-    sums.alloc(64);
-    host.alloc(64);
+// Schedule autotuner: for each nonzeros-per-block choice, rebuild the
+// schedule and time a plain pass y = M x (FP64 out) with HIP events; keep
+// the fastest.  Returns the chosen value.
+template <typename TV>
+int autotune_schedule(DevCsr<TV>& M, int64_t ncols, hipStream_t s) {
+  if (M.nblocks == 0 || M.nnz == 0) return M.npb;
+  DevBuf<double> x, y;
+  x.alloc((size_t)std::max<int64_t>(ncols, 1));
+  y.alloc((size_t)std::max<int32_t>(M.nrows, 1));
+  std::vector<double> ones((size_t)std::max<int64_t>(ncols, 1), 1.0);
+  x.upload(ones.data(), ones.size(), s);
+  hipEvent_t e0, e1;
+  PDFP_HIP(hipEventCreate(&e0));
+  PDFP_HIP(hipEventCreate(&e1));
+  int best = M.npb;
+  float best_ms = 1e30f;
+  for (int npb : kNpbChoices) {
+    M.schedule(npb, s);
+    for (int w = 0; w < 3; ++w)
+      launch_pass<0>(M, Gather1<double>{x.p}, EpiStore<double>{y.p}, nullptr, s);
+    PDFP_HIP(hipEventRecord(e0, s));
+    constexpr int kReps = 10;
+    for (int r = 0; r < kReps; ++r)
+      launch_pass<0>(M, Gather1<double>{x.p}, EpiStore<double>{y.p}, nullptr, s);
+    PDFP_HIP(hipEventRecord(e1, s));
+    PDFP_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = npb;
+    }
   }
-};
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  M.schedule(best, s);
+  return best;
+}
 
 // estimate_max_singular_value(A, A_t, m; max_iterations, tolerance)
 // src/preconditioner.jl:108-125, on the device: one host synchronisation per
