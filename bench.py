@@ -112,6 +112,58 @@ def cpu_baseline(pp, milp, target, iterations: int):
                       f"(FP64 C oracle, single thread, ||A||_2 excluded), {dt:.1f} s"}
 
 
+def _oracle_pump_backends():
+    """The oracle as the pump's projector and root LP (CPU baseline leg)."""
+    from types import SimpleNamespace
+
+    from oracle import oracle as O
+
+    proj = SimpleNamespace(PDFPState=O.OraclePDFPState, set_target=O.set_target,
+                           warm_start=O.warm_start, pdfp_solve=O.pdfp_solve,
+                           original_primal_residual=O.original_primal_residual)
+
+    def root_lp(milp, *, tolerance, time_limit, cupdlpx_termination):
+        return O.solve_lp_relaxation(milp, tolerance=tolerance, time_limit=time_limit,
+                                     cupdlpx_termination=cupdlpx_termination)
+
+    return proj, root_lp
+
+
+def time_to_first_feasible(runs: int, cpu: bool):
+    """The full pump (root LP, projections, rounding) on the 133 × 201 binary
+    set-covering instance (seed 1) and on p0201: median PumpResult.time over
+    `runs` runs after one warm-up run, as the reference times it
+    (scripts/solve.jl:88-97)."""
+    from pdfp_amd import generators as G
+    from pdfp_amd.milp import is_feasible, read_mps
+    from pdfp_amd.pump import feasibility_pump
+
+    out = {}
+    cases = [("set_cover_133x201_seed1", G.set_cover(133, 201, 1)),
+             ("p0201", read_mps(os.path.join(ROOT, "tests", "golden", "examples",
+                                             "p0201.mps.gz")))]
+    feasibility_pump(G.set_cover(133, 201, 7), time_limit=60.0)        # warm-up
+    for name, milp in cases:
+        rec = {}
+        for prec in ("fp32", "fp64"):
+            rs = [feasibility_pump(milp, time_limit=60.0, precision=prec) for _ in range(runs)]
+            ok = all(r.found and is_feasible(milp, r.x) for r in rs)
+            ts = sorted(r.time for r in rs)
+            r0 = rs[0]
+            rec[prec] = {"seconds": ts[len(ts) // 2], "found": ok, "iterations": r0.iterations,
+                         "pdhg_iterations": r0.pdhg_iterations,
+                         "root_lp_s": r0.root_lp_time, "projection_s": r0.projection_time,
+                         "rounding_s": r0.rounding_time}
+        if cpu:
+            proj, root_lp = _oracle_pump_backends()
+            r = feasibility_pump(milp, time_limit=120.0, projector=proj, root_lp=root_lp)
+            rec["cpu_oracle"] = {"seconds": r.time, "found": bool(r.found), "cores": 1,
+                                 "kind": "port"}
+        out[name] = rec
+    out["p0201"]["reference_h100_seconds"] = 0.12   # README.md:47-48
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +174,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ttff", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the projector has no multi-GPU path in this build "
@@ -203,6 +256,9 @@ def main():
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pp, milp, targets[0], args.cpu_iterations)
+    if not args.no_ttff:
+        print("[bench] time to first feasible", file=sys.stderr, flush=True)
+        out["time_to_first_feasible"] = time_to_first_feasible(3, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
