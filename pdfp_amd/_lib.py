@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpdfp_hip.so")
+# PDFP_HIP_LIB selects another build of the library (e.g. a variant built
+# with `python -m pdfp_amd.build --out PATH -D NAME=VALUE`).
+LIB_PATH = os.environ.get("PDFP_HIP_LIB", os.path.join(HERE, "libpdfp_hip.so"))
 
 TERMINATION = {0: "UNSPECIFIED", 1: "OPTIMAL", 2: "ITERATION_LIMIT", 3: "TIME_LIMIT",
                4: "PRIMAL_INFEASIBLE", 5: "DUAL_INFEASIBLE"}

@@ -35,18 +35,19 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *srcs, "-o", tmp]
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-I",
+           os.path.join(ROOT, "include"), *srcs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def exported_symbols(path: str = LIB) -> list[str]:
@@ -59,4 +60,7 @@ if __name__ == "__main__":
     if "--check" in sys.argv:
         print("\n".join(s for s in exported_symbols() if s.startswith(("pdfp_", "pdlp_"))))
     else:
-        print(build(verbose=True))
+        args = sys.argv[1:]
+        out = args[args.index("--out") + 1] if "--out" in args else LIB
+        defs = tuple(args[i + 1] for i, a in enumerate(args) if a == "-D")
+        print(build(verbose=True, out=out, defines=defs))

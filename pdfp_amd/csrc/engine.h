@@ -8,7 +8,8 @@
 // nonzeros; a row longer than NPB sits alone in its block.  For every block
 // the host also picks `tpr`, the lanes that reduce one row (a power of two,
 // at most a wavefront), from the block's row lengths; tpr == 0 marks a
-// single long row.
+// single long row.  Each block's record carries its first row, its first
+// nonzero and tpr, so the streaming starts after one dependent load.
 //
 //   stream mode  (tpr > 0): the block's contiguous slice of (col, val) is
 //     streamed once, coalesced; the products a_ij·x_j are formed in FP64 and
@@ -42,8 +43,10 @@ struct Csr {
   const int32_t* __restrict__ rowptr;  // nrows + 1
   const int32_t* __restrict__ col;     // nnz
   const TV* __restrict__ val;          // nnz
-  const int32_t* __restrict__ blocks;  // nblocks + 1 row boundaries
-  const uint8_t* __restrict__ tpr;     // nblocks: lanes per row (0: one long row)
+  // nblocks + 1 records {first row, first nonzero, lanes per row, 0}; the
+  // last record is {nrows, nnz, 0, 0}.  One 16-byte load gives a block
+  // everything it needs to start streaming.
+  const int4* __restrict__ sched;
   int32_t nblocks;
   int32_t nrows;
 };
@@ -59,6 +62,20 @@ __device__ __forceinline__ int xcd_block(int bid, int nb) {
   // XCDs 0..rem-1 own per+1 blocks, the others per.
   const int start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
   return start + idx;
+}
+
+// Loads of data that a pass reads exactly once (the matrix slice): marked
+// non-temporal so that they do not evict the gathered vector from L2.
+#ifndef PDFP_STREAM_NT
+#define PDFP_STREAM_NT 1
+#endif
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+#if PDFP_STREAM_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -166,9 +183,12 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
 
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  const int r0 = A.blocks[b];
-  const int r1 = A.blocks[b + 1];
-  const int tpr = A.tpr[b];
+  const int4 sb = A.sched[b];
+  const int4 se = A.sched[b + 1];
+  const int r0 = sb.x, r1 = se.x;
+  const int base = sb.y;
+  const int cnt = se.y - sb.y;
+  const int tpr = sb.z;
   const int nrows = r1 - r0;
 
   // The epilogue's operands: in flight while the row slice streams in.
@@ -180,17 +200,15 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
   for (int k = 0; k < NA; ++k) acc[k] = 0.0;
 
   if (tpr > 0) {
-    const int base = A.rowptr[r0];
     for (int i = tid; i <= nrows; i += BLOCK) off[i] = A.rowptr[r0 + i] - base;
-    const int cnt = A.rowptr[r1] - base;
     for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
       int cc[U];
       double vv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = k0 + u * BLOCK + tid;
-        cc[u] = k < cnt ? A.col[base + k] : 0;
-        vv[u] = k < cnt ? (double)A.val[base + k] : 0.0;
+        cc[u] = k < cnt ? ld_stream(A.col + base + k) : 0;
+        vv[u] = k < cnt ? (double)ld_stream(A.val + base + k) : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -235,8 +253,6 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
       epi(r0 + tid, pre, s, acc);
     }
   } else {
-    const int base = A.rowptr[r0];
-    const int cnt = A.rowptr[r1] - base;
     double s[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = 0.0;
@@ -246,8 +262,8 @@ __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int k = k0 + u * BLOCK + tid;
-        cc[u] = k < cnt ? A.col[base + k] : 0;
-        vv[u] = k < cnt ? (double)A.val[base + k] : 0.0;
+        cc[u] = k < cnt ? ld_stream(A.col + base + k) : 0;
+        vv[u] = k < cnt ? (double)ld_stream(A.val + base + k) : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {

@@ -113,8 +113,8 @@ inline void start_vector(int64_t m, uint64_t seed, double* out) {
 // Device CSR + schedule of one operator (A or Aᵀ).
 template <typename TV>
 struct DevCsr {
-  DevBuf<int32_t> rowptr, col, blocks;
-  DevBuf<uint8_t> tpr;
+  DevBuf<int32_t> rowptr, col;
+  DevBuf<int4> sched;  // per-block {first row, first nonzero, lanes per row, 0}
   DevBuf<TV> val;
   int32_t nblocks = 0, nrows = 0;
   int32_t npb = 512;  // nonzeros per stream-mode block of the current schedule
@@ -144,19 +144,20 @@ struct DevCsr {
     std::vector<int32_t> blk = row_blocks(h_rowptr, kBlock, max_nnz);
     std::vector<uint8_t> lanes = block_lanes(h_rowptr, blk, kBlock, max_nnz);
     nblocks = (int32_t)blk.size() - 1;
-    blocks.alloc(blk.size());
-    blocks.upload(blk.data(), blk.size(), s);
-    tpr.alloc(std::max<size_t>(lanes.size(), 1));
-    tpr.upload(lanes.data(), lanes.size(), s);
+    std::vector<int4> rec(blk.size());
+    for (size_t k = 0; k < blk.size(); ++k) {
+      rec[k].x = blk[k];
+      rec[k].y = h_rowptr[(size_t)blk[k]];
+      rec[k].z = k < lanes.size() ? (int)lanes[k] : 0;
+      rec[k].w = 0;
+    }
+    sched.alloc(rec.size());
+    sched.upload(rec.data(), rec.size(), s);
     PDFP_HIP(hipStreamSynchronize(s));
   }
 
-  Csr<TV> view() const {
-    return Csr<TV>{rowptr.p, col.p, val.p, blocks.p, tpr.p, nblocks, nrows};
-  }
-  size_t bytes() const {
-    return rowptr.bytes() + col.bytes() + blocks.bytes() + tpr.bytes() + val.bytes();
-  }
+  Csr<TV> view() const { return Csr<TV>{rowptr.p, col.p, val.p, sched.p, nblocks, nrows}; }
+  size_t bytes() const { return rowptr.bytes() + col.bytes() + sched.bytes() + val.bytes(); }
 };
 
 // Launch one CSR pass with the operator's schedule (no-op if it is empty).
