@@ -209,7 +209,9 @@ def main():
         info = s.info()
         results[prec] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
                              ms_iter=ms_iter, info=info,
-                             npb={"A": info.npb_a, "At": info.npb_at})
+                             npb={"A": info.npb_a, "At": info.npb_at,
+                                  "A_persistent_wg_per_cu": info.persist_a,
+                                  "At_persistent_wg_per_cu": info.persist_at})
         print(f"[bench] {prec}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; iteration "
               f"{ms_iter * 1e3:.1f}us; {st}", file=sys.stderr, flush=True)

@@ -106,6 +106,8 @@ typedef struct pdfp_info {
   int32_t n_evaluations;
   int32_t npb_a;             /* nonzeros per workgroup chosen by the autotuner */
   int32_t npb_at;            /*   for the A pass and for the Aᵀ pass            */
+  int32_t persist_a;         /* 0: one workgroup per row block; k: a persistent */
+  int32_t persist_at;        /*   grid of k workgroups per CU (autotuned)       */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */

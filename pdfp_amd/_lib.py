@@ -60,6 +60,8 @@ class PdfpInfo(C.Structure):
         ("n_evaluations", C.c_int32),
         ("npb_a", C.c_int32),
         ("npb_at", C.c_int32),
+        ("persist_a", C.c_int32),
+        ("persist_at", C.c_int32),
     ]
 
 

@@ -469,6 +469,8 @@ struct Projector final : ProjectorBase {
     o->n_evaluations = n_evals;
     o->npb_a = A.npb;
     o->npb_at = At.npb;
+    o->persist_a = A.persist;
+    o->persist_at = At.persist;
   }
 
   void spmv(int transpose, const double* xin, double* yout) override {

@@ -118,6 +118,7 @@ struct DevCsr {
   DevBuf<TV> val;
   int32_t nblocks = 0, nrows = 0;
   int32_t npb = 512;  // nonzeros per stream-mode block of the current schedule
+  int32_t persist = 0;  // 0: one workgroup per block; k: k workgroups per CU, looping
   int64_t nnz = 0;
   std::vector<int32_t> h_rowptr;  // kept for re-scheduling
 
@@ -160,27 +161,29 @@ struct DevCsr {
   size_t bytes() const { return rowptr.bytes() + col.bytes() + sched.bytes() + val.bytes(); }
 };
 
+// Launch one pass of a given NPB: one workgroup per row block, or (when the
+// operator's `persist` > 0) a persistent grid of persist × 256 workgroups
+// (256 CUs) that walks the blocks.
+template <int NPB, int NACC, int NMAX, typename TV, class G, class E>
+inline void launch_pass_npb(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
+  const int grid = A.persist * 256;
+  if (A.persist > 0 && grid < A.nblocks)
+    csr_pass_persistent<kBlock, NPB, TV, G, E, NACC, NMAX>
+        <<<grid, kBlock, 0, s>>>(A.view(), g, e, partials);
+  else
+    csr_pass<kBlock, NPB, TV, G, E, NACC, NMAX>
+        <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+}
+
 // Launch one CSR pass with the operator's schedule (no-op if it is empty).
 template <int NACC, int NMAX = 0, typename TV, class G, class E>
 inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
   if (A.nblocks == 0) return;
   switch (A.npb) {
-    case 256:
-      csr_pass<kBlock, 256, TV, G, E, NACC, NMAX>
-          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
-      break;
-    case 512:
-      csr_pass<kBlock, 512, TV, G, E, NACC, NMAX>
-          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
-      break;
-    case 1024:
-      csr_pass<kBlock, 1024, TV, G, E, NACC, NMAX>
-          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
-      break;
-    default:
-      csr_pass<kBlock, 2048, TV, G, E, NACC, NMAX>
-          <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
-      break;
+    case 256: launch_pass_npb<256, NACC, NMAX>(A, g, e, partials, s); break;
+    case 512: launch_pass_npb<512, NACC, NMAX>(A, g, e, partials, s); break;
+    case 1024: launch_pass_npb<1024, NACC, NMAX>(A, g, e, partials, s); break;
+    default: launch_pass_npb<2048, NACC, NMAX>(A, g, e, partials, s); break;
   }
 }
 
@@ -250,36 +253,42 @@ struct EpiStoreDot {
 template <typename TV>
 int autotune_schedule(DevCsr<TV>& M, int64_t ncols, hipStream_t s) {
   if (M.nblocks == 0 || M.nnz == 0) return M.npb;
-  DevBuf<double> x, y;
+  DevBuf<TV> x, y;
   x.alloc((size_t)std::max<int64_t>(ncols, 1));
   y.alloc((size_t)std::max<int32_t>(M.nrows, 1));
-  std::vector<double> ones((size_t)std::max<int64_t>(ncols, 1), 1.0);
+  std::vector<TV> ones((size_t)std::max<int64_t>(ncols, 1), (TV)1);
   x.upload(ones.data(), ones.size(), s);
   hipEvent_t e0, e1;
   PDFP_HIP(hipEventCreate(&e0));
   PDFP_HIP(hipEventCreate(&e1));
-  int best = M.npb;
+  int best = M.npb, best_persist = 0;
   float best_ms = 1e30f;
+  constexpr int kPersist[] = {0, 2, 4, 8};
   for (int npb : kNpbChoices) {
     M.schedule(npb, s);
-    for (int w = 0; w < 3; ++w)
-      launch_pass<0>(M, Gather1<double>{x.p}, EpiStore<double>{y.p}, nullptr, s);
-    PDFP_HIP(hipEventRecord(e0, s));
-    constexpr int kReps = 10;
-    for (int r = 0; r < kReps; ++r)
-      launch_pass<0>(M, Gather1<double>{x.p}, EpiStore<double>{y.p}, nullptr, s);
-    PDFP_HIP(hipEventRecord(e1, s));
-    PDFP_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = npb;
+    for (int persist : kPersist) {
+      M.persist = persist;
+      for (int w = 0; w < 3; ++w)
+        launch_pass<0>(M, Gather1<TV>{x.p}, EpiStore<TV>{y.p}, nullptr, s);
+      PDFP_HIP(hipEventRecord(e0, s));
+      constexpr int kReps = 10;
+      for (int r = 0; r < kReps; ++r)
+        launch_pass<0>(M, Gather1<TV>{x.p}, EpiStore<TV>{y.p}, nullptr, s);
+      PDFP_HIP(hipEventRecord(e1, s));
+      PDFP_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = npb;
+        best_persist = persist;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   M.schedule(best, s);
+  M.persist = best_persist;
   return best;
 }
 
