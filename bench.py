@@ -91,6 +91,39 @@ def run_projections(s, pp, milp, targets, tol):
     return iters, t_total, statuses
 
 
+def kernel_bytes_batch(m: int, n: int, nnz: int, K: int = 8) -> tuple[float, float]:
+    """Algorithmic bytes of the two batched launches (8 trajectory slots,
+    FP32).  Transpose pass: 8·nnz + 4(n+1), z of the 8 slots read once 32m,
+    per column x, y, x⁰, y⁰, x̃ (32 B each) and ℓ, u, w (4 B each) 172n,
+    write x, y, x̄ 96n.  A pass: 8·nnz + 4(m+1), x̄ read once 32n, per row
+    z, z⁰ (32 B each), b, is_eq 69m, write z 32m."""
+    at = 8 * nnz + 4 * (n + 1) + 32 * m + 172 * n + 96 * n
+    a = 8 * nnz + 4 * (m + 1) + 32 * n + 69 * m + 32 * m
+    return float(at), float(a)
+
+
+def run_batch(bt, pp, milp, target_sets, tol):
+    """Each step: K warm-started projections, one per trajectory, solved
+    together.  Returns (Σ PDHG iterations, seconds, statuses)."""
+    from pdfp_amd.pdfp import SolverParameters
+
+    iters = 0
+    t_total = 0.0
+    statuses = []
+    pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=500_000)
+    for targets in target_sets:
+        t0 = time.perf_counter()
+        for k, xt in enumerate(targets):
+            bt.set_target(k, xt / pp.D_c)
+        st = bt.solve(p, pw)
+        t_total += time.perf_counter() - t0
+        iters += sum(bt.total_count(k) for k in range(bt.K))
+        statuses.append(st)
+    return iters, t_total, statuses
+
+
 def cpu_baseline(pp, milp, target, iterations: int):
     """The FP64 C oracle (the reference's CPU path restated) on one core:
     PDHG iterations of one projection of the same workload, ‖A‖₂ excluded."""
@@ -175,6 +208,8 @@ def main():
     ap.add_argument("--cpu-iterations", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttff", action="store_true")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="trajectories per step of the headline measurement (1: single)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the projector has no multi-GPU path in this build "
@@ -217,11 +252,43 @@ def main():
               f"{ms_iter * 1e3:.1f}us; {st}", file=sys.stderr, flush=True)
         s.close()
 
+    K = args.batch
+    if K > 1:
+        from pdfp_amd.batch import PDFPBatch
+
+        bt = PDFPBatch(K, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+        for k in range(K):
+            bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
+        sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]
+                  for s in range(args.warmup)]
+        sets = [[G.binary_targets(milp, 4000 + 100 * s + k) for k in range(K)]
+                for s in range(args.steps)]
+        t0 = time.perf_counter()
+        run_batch(bt, pp, milp, sets_w, args.tol)
+        print(f"[bench] batch{K} warmup {time.perf_counter() - t0:.2f}s", file=sys.stderr,
+              flush=True)
+        it, dt, st = run_batch(bt, pp, milp, sets, args.tol)
+        ms_at, ms_a = bt.time_kernels(100)
+        ms_iter = bt.time_iterations(200)
+        results["batch"] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
+                                ms_iter=ms_iter)
+        print(f"[bench] fp32 batch of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
+              f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; batched iteration "
+              f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)
+        bt.close()
+        head = results["batch"]
+        bat, ba = kernel_bytes_batch(mm, nn, nnz)
+        steps_desc = (f"{K} warm-started l1 projections (one per trajectory, 8-slot batch "
+                      f"sharing A) to new 0/1 targets at relative KKT tolerance {args.tol:g}")
+    else:
+        head = results["fp32"]
+        bat, ba = kernel_bytes_fp32(mm, nn, nnz)
+        steps_desc = (f"one warm-started l1 projection to a new 0/1 target at relative KKT "
+                      f"tolerance {args.tol:g}")
     r32 = results["fp32"]
-    value = r32["iters"] / r32["seconds"]
-    bat, ba = kernel_bytes_fp32(mm, nn, nnz)
-    dom_bytes, dom_ms, dom_name = max((bat, r32["ms_at"], "transpose pass (A^T z + x/y update)"),
-                                      (ba, r32["ms_a"], "row pass (A xbar + z update)"),
+    value = head["iters"] / head["seconds"]
+    dom_bytes, dom_ms, dom_name = max((bat, head["ms_at"], "transpose pass (A^T z + x/y update)"),
+                                      (ba, head["ms_a"], "row pass (A xbar + z update)"),
                                       key=lambda t: t[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     out = {
@@ -231,29 +298,33 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+        "ms_per_step": 1e3 * head["seconds"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": desc, "step": "one warm-started l1 projection to a new 0/1 "
-                   f"target at relative KKT tolerance {args.tol:g}",
+        "config": {"workload": desc, "step": steps_desc, "trajectories": K,
                    "precision": "FP32 data/iterates, FP64 reductions and polish",
-                   "iterations_timed": r32["iters"]},
+                   "iterations_timed": head["iters"]},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes": dom_bytes,
                      "kernel_ms": dom_ms},
-        "iteration_us": r32["ms_iter"] * 1e3,
+        "iteration_us": head["ms_iter"] * 1e3,
+        "kernel_us": {"transpose_pass": head["ms_at"] * 1e3, "row_pass": head["ms_a"] * 1e3},
+        "fp32_single_value": r32["iters"] / r32["seconds"],
+        "fp32_single_iteration_us": r32["ms_iter"] * 1e3,
+        "fp32_single_kernel_us": {"transpose_pass": r32["ms_at"] * 1e3,
+                                  "row_pass": r32["ms_a"] * 1e3},
         "schedule_nnz_per_block": r32["npb"],
-        "kernel_us": {"transpose_pass": r32["ms_at"] * 1e3, "row_pass": r32["ms_a"] * 1e3},
     }
     if "fp64" in results:
         r64 = results["fp64"]
         v64 = r64["iters"] / r64["seconds"]
         out["fp64_plain_value"] = v64
-        out["speedup_fp32_vs_fp64"] = value / v64
+        out["speedup_vs_fp64_plain"] = value / v64
+        out["speedup_fp32_single_vs_fp64_plain"] = out["fp32_single_value"] / v64
         out["fp64_iteration_us"] = r64["ms_iter"] * 1e3
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

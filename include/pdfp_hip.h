@@ -165,6 +165,50 @@ int pdfp_get_info(pdfp_ctx* ctx, pdfp_info* out);
 /* Frees every device buffer of the context. NULL is a no-op. */
 void pdfp_destroy(pdfp_ctx* ctx);
 
+/* ---- batch of trajectories on one instance ----------------------------- */
+/* The reference runs K concurrent pump trajectories as K OS processes, each
+ * with its own PDFPState on the same GPU (scripts/solve.jl:121-148).  A
+ * pdfp_batch holds one instance's scaled matrix once and up to 8
+ * trajectories (own target, iterates, primal weight, restart state) that
+ * iterate together in the same fused launches, in the FP32 form; each
+ * trajectory follows pdfp_solve! (src/pdfp.jl:252-288) on its own. */
+
+typedef struct pdfp_batch pdfp_batch;
+
+const char* pdfp_batch_last_error(const pdfp_batch* ctx);
+
+/* As pdfp_create, for K ∈ 1..8 trajectories (FP32 form). */
+int pdfp_batch_create(pdfp_batch** out, int32_t K, int64_t m, int64_t n,
+                      int64_t nnz, const int64_t* colptr, const int64_t* rowval,
+                      const double* nzval, int32_t index_base, const double* b,
+                      const uint8_t* is_eq, const uint8_t* is_int,
+                      const double* var_lb, const double* var_ub,
+                      const double* w, int32_t device);
+/* original_primal_residual! for every trajectory (src/pdfp.jl:85-90). */
+int pdfp_batch_set_original_primal_residual(pdfp_batch* ctx, const double* D_r);
+/* Trajectory k's copyto!(current_x/initial_x) (src/pump.jl:305-306),
+ * set_target! (src/pdfp.jl:98-101) and warm_start! (src/pdfp.jl:104-114). */
+int pdfp_batch_set_x(pdfp_batch* ctx, int32_t k, const double* x_scaled);
+int pdfp_batch_set_target(pdfp_batch* ctx, int32_t k, const double* x_tilde_scaled);
+int pdfp_batch_warm_start(pdfp_batch* ctx, int32_t k);
+/* pdfp_solve! for every k with active[k] (all when active is NULL), with
+ * primal_weight[k]; returns when each of them has terminated.  An inactive
+ * trajectory is left unchanged.  termination / total_count: K entries. */
+int pdfp_batch_solve(pdfp_batch* ctx, const pdfp_params* params,
+                     const double* primal_weight, const uint8_t* active,
+                     int32_t* termination, int64_t* total_count);
+/* Trajectory k's projection (FP64, polished), vectors (ids as in
+ * pdfp_get_vector) and scalars. */
+int pdfp_batch_get_x(pdfp_batch* ctx, int32_t k, double* x_scaled_out);
+int pdfp_batch_get_vector(pdfp_batch* ctx, int32_t k, int32_t which, double* out);
+int pdfp_batch_get_info(pdfp_batch* ctx, int32_t k, pdfp_info* out);
+/* Mean milliseconds of one batched iteration (all K trajectories). */
+int pdfp_batch_time_iterations(pdfp_batch* ctx, int32_t iterations, double* ms_out);
+/* Mean milliseconds of the batched transpose pass and of the batched A pass. */
+int pdfp_batch_time_kernels(pdfp_batch* ctx, int32_t iterations, double* ms_at,
+                            double* ms_a);
+void pdfp_batch_destroy(pdfp_batch* ctx);
+
 /* ---- root LP (src/pdlp.jl) -------------------------------------------- */
 
 typedef struct pdlp_ctx pdlp_ctx;

@@ -115,6 +115,22 @@ SIGNATURES = {
     "pdfp_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_device_bytes": (C.c_int64, [_P]),
+    "pdfp_batch_last_error": (C.c_char_p, [_P]),
+    "pdfp_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int64, C.c_int64,
+                                    _I64, _I64, _D, C.c_int32, _D, _U8, _U8, _D, _D, _D,
+                                    C.c_int32]),
+    "pdfp_batch_set_original_primal_residual": (C.c_int, [_P, _D]),
+    "pdfp_batch_set_x": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_batch_set_target": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_batch_warm_start": (C.c_int, [_P, C.c_int32]),
+    "pdfp_batch_solve": (C.c_int, [_P, C.POINTER(PdfpParams), _D, _U8, C.POINTER(C.c_int32),
+                                   _I64]),
+    "pdfp_batch_get_x": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_batch_get_vector": (C.c_int, [_P, C.c_int32, C.c_int32, _D]),
+    "pdfp_batch_get_info": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpInfo)]),
+    "pdfp_batch_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_batch_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "pdfp_batch_destroy": (None, [_P]),
 }
 
 _lib = None

@@ -1,0 +1,119 @@
+"""Batch of up to 8 projector trajectories of one instance on the device.
+
+The reference runs concurrent trajectories as separate OS processes, each
+with its own projector state on the GPU (scripts/solve.jl:121-148).
+``PDFPBatch`` keeps the instance's scaled matrix in HBM once and runs the K
+trajectories' PDHG iterations in the same fused launches (FP32 iterates,
+FP64 reductions and polish).  Each trajectory has its own target, iterates,
+primal weight, restart state and termination, and follows ``pdfp_solve!``
+(src/pdfp.jl:252-288) on its own.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import check, dptr, f64, i64ptr, u8ptr
+from .pdfp import SolverParameters, _csc_arrays
+
+
+class PDFPBatch:
+    """K ∈ 1..8 trajectories on the scaled data of one instance."""
+
+    MAX = 8
+
+    def __init__(self, K: int, A: sp.csc_matrix, b, is_eq, is_int, var_lb, var_ub, w, *,
+                 device: int = 0):
+        lib = _lib.load()
+        self._lib = lib
+        m, n = A.shape
+        self.K, self.m, self.n = int(K), int(m), int(n)
+        colptr, rowval, nzval = _csc_arrays(A)
+        eq = np.ascontiguousarray(is_eq, dtype=np.uint8)
+        ii = np.ascontiguousarray(is_int, dtype=np.uint8)
+        ctx = C.c_void_p()
+        st = lib.pdfp_batch_create(C.byref(ctx), self.K, m, n, len(nzval), i64ptr(colptr),
+                                   i64ptr(rowval), dptr(nzval), 0, dptr(f64(b)), u8ptr(eq),
+                                   u8ptr(ii), dptr(f64(var_lb)), dptr(f64(var_ub)), dptr(f64(w)),
+                                   device)
+        check(st, lib.pdfp_batch_last_error, None)
+        self._ctx = ctx
+        self.termination_reason = ["UNSPECIFIED"] * self.K
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            self._lib.pdfp_batch_destroy(ctx)
+            self._ctx = None
+
+    def close(self):
+        self.__del__()
+
+    def _check(self, st):
+        check(st, self._lib.pdfp_batch_last_error, self._ctx)
+
+    def original_primal_residual(self, D_r) -> "PDFPBatch":
+        d = f64(D_r)
+        self._check(self._lib.pdfp_batch_set_original_primal_residual(self._ctx, dptr(d)))
+        return self
+
+    def set_x(self, k: int, x_scaled) -> None:
+        x = f64(x_scaled)
+        self._check(self._lib.pdfp_batch_set_x(self._ctx, k, dptr(x)))
+
+    def set_target(self, k: int, x_tilde_scaled) -> None:
+        x = f64(x_tilde_scaled)
+        self._check(self._lib.pdfp_batch_set_target(self._ctx, k, dptr(x)))
+        self.termination_reason[k] = "UNSPECIFIED"
+
+    def warm_start(self, k: int) -> None:
+        self._check(self._lib.pdfp_batch_warm_start(self._ctx, k))
+        self.termination_reason[k] = "UNSPECIFIED"
+
+    def solve(self, p: SolverParameters, primal_weight, active=None) -> list:
+        """pdfp_solve! for each active trajectory; returns their termination
+        reasons (the inactive ones keep theirs)."""
+        pw = np.broadcast_to(np.asarray(primal_weight, dtype=np.float64), (self.K,)).copy()
+        act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+        term = (C.c_int32 * self.K)()
+        total = (C.c_int64 * self.K)()
+        cp = p.to_c()
+        self._check(self._lib.pdfp_batch_solve(self._ctx, C.byref(cp), dptr(pw),
+                                               u8ptr(act) if act is not None else None, term,
+                                               total))
+        self.termination_reason = [_lib.TERMINATION[int(t)] for t in term]
+        return list(self.termination_reason)
+
+    def pdhg_x(self, k: int) -> np.ndarray:
+        out = np.empty(self.n)
+        self._check(self._lib.pdfp_batch_get_x(self._ctx, k, dptr(out)))
+        return out
+
+    def vector(self, k: int, name: str) -> np.ndarray:
+        ids = {"pdhg_x": 0, "pdhg_y": 1, "pdhg_z": 2, "current_x": 3, "current_y": 4,
+               "current_z": 5, "dual_slack": 6, "x_tilde": 7}
+        out = np.empty(self.m if name in ("pdhg_z", "current_z") else self.n)
+        self._check(self._lib.pdfp_batch_get_vector(self._ctx, k, ids[name], dptr(out)))
+        return out
+
+    def info(self, k: int) -> _lib.PdfpInfo:
+        out = _lib.PdfpInfo()
+        self._check(self._lib.pdfp_batch_get_info(self._ctx, k, C.byref(out)))
+        return out
+
+    def total_count(self, k: int) -> int:
+        return int(self.info(k).total_count)
+
+    def time_kernels(self, iterations: int) -> tuple[float, float]:
+        a, b = C.c_double(), C.c_double()
+        self._check(self._lib.pdfp_batch_time_kernels(self._ctx, iterations, C.byref(a),
+                                                      C.byref(b)))
+        return a.value, b.value
+
+    def time_iterations(self, iterations: int) -> float:
+        ms = C.c_double()
+        self._check(self._lib.pdfp_batch_time_iterations(self._ctx, iterations, C.byref(ms)))
+        return ms.value

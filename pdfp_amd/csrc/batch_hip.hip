@@ -1,0 +1,652 @@
+// libpdfp_hip: a batch of up to 8 projector trajectories sharing one matrix.
+//
+// The reference runs concurrent trajectories as separate OS processes, each
+// with its own PDFPState on the same GPU (scripts/solve.jl:121-148).  Here
+// the K trajectories of one instance share the scaled A and Aᵀ in HBM and
+// run their PDHG iterations in the same two fused launches per iteration;
+// each trajectory keeps its own target, iterates, primal weight, restart
+// state and termination, with exactly the per-trajectory logic of
+// pdfp_solve! (src/pdfp.jl:252-288).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pdfp_hip.h"
+#include "batch_kernels.h"
+#include "engine.h"
+#include "host_util.h"
+#include "pdfp_kernels.h"
+#include "solver_common.h"
+
+namespace pdfp {
+
+struct Trajectory {
+  RestartState r;
+  bool orig_primal = false;
+  double relative_l1_gap = INFINITY, l1_den = 1.0, l1_distance = NAN;
+  int termination = PDFP_UNSPECIFIED;
+  int n_restarts = 0, n_evals = 0;
+  double solve_seconds = 0.0;
+  bool x64_valid = false;
+  bool do_restart = false;
+  std::vector<double> h_xt;
+};
+
+struct Batch {
+  int K;
+  int64_t m, n, nnz;
+  hipStream_t stream = nullptr;
+  DevCsr<float> A, At;
+  DevBuf<float> b, lb, ub, w;                          // shared
+  DevBuf<float> xt, x, x0, y, y0, rx, px, ds, py, ry;  // n × 8
+  DevBuf<float> z, z0, pz, rz;                         // m × 8
+  DevBuf<uint8_t> is_eq;
+  DevBuf<double> row_unscale;
+  DevBuf<double> lb64, ub64, w64, xt64, x64;
+  DevBuf<float> tmpf;
+  DevBuf<double> part_t, part_a, part_v, sums;
+  DevBuf<BatchStep> d_sp;
+  PinnedBuf<BatchStep> h_sp;
+  PinnedBuf<double> h_sums;
+  std::vector<double> h_w, h_b;
+  double b_norm = 0.0, b_norm_original = NAN, dual_den = 1.0;
+  bool orig_primal = false;
+  double sv = NAN, step_size = 0.0;
+  std::vector<Trajectory> tr;
+  std::string err = "no error";
+
+  Batch(int K_, int64_t m_, int64_t n_, const int64_t* colptr, const int64_t* rowval,
+        const double* nzval, int32_t base, const double* b_, const uint8_t* is_eq_,
+        const uint8_t* is_int_, const double* var_lb, const double* var_ub, const double* w_,
+        int device)
+      : K(K_), m(m_), n(n_) {
+    PDFP_HIP(hipSetDevice(device));
+    PDFP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HostCsr ha = csc_to_csr(m, n, colptr, rowval, nzval, base);
+    HostCsr ht = csc_as_transpose_csr(m, n, colptr, rowval, nzval, base);
+    nnz = (int64_t)ha.col.size();
+    A.upload(ha, stream);
+    At.upload(ht, stream);
+    // eight gathered values per nonzero: the staged products need 64 bytes
+    // each, so the schedule keeps blocks at 256 nonzeros
+    A.schedule(256, stream);
+    At.schedule(256, stream);
+    tr.resize((size_t)K);
+    h_w.assign((size_t)n, 0.0);
+    for (int64_t j = 0; j < n; ++j) {
+      if (is_int_[j] && !(w_[j] > 0.0))
+        throw Error(PDFP_ERR_ARGUMENT, "weights of integer columns must be positive");
+      h_w[(size_t)j] = is_int_[j] ? w_[j] : 0.0;
+    }
+    h_b.assign(b_, b_ + m);
+    double acc = 0.0;
+    for (int64_t i = 0; i < m; ++i) acc += b_[i] * b_[i];
+    b_norm = std::sqrt(acc);
+    acc = 0.0;
+    for (int64_t j = 0; j < n; ++j) acc += h_w[(size_t)j] * h_w[(size_t)j];
+    dual_den = 1.0 + std::sqrt(acc);
+    auto vecf = [&](DevBuf<float>& d, int64_t len, const double* src) {
+      d.alloc((size_t)std::max<int64_t>(len, 1));
+      std::vector<float> tmp((size_t)std::max<int64_t>(len, 1), 0.f);
+      if (src)
+        for (int64_t k = 0; k < len; ++k) tmp[(size_t)k] = (float)src[k];
+      d.upload(tmp.data(), tmp.size(), stream);
+      PDFP_HIP(hipStreamSynchronize(stream));
+    };
+    vecf(b, m, b_);
+    vecf(lb, n, var_lb);
+    vecf(ub, n, var_ub);
+    vecf(w, n, h_w.data());
+    for (DevBuf<float>* v : {&xt, &x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry})
+      vecf(*v, n * kBatch, nullptr);
+    for (DevBuf<float>* v : {&z, &z0, &pz, &rz}) vecf(*v, m * kBatch, nullptr);
+    vecf(tmpf, std::max(m, n), nullptr);
+    is_eq.alloc((size_t)std::max<int64_t>(m, 1));
+    is_eq.upload(is_eq_, (size_t)m, stream);
+    auto vec64 = [&](DevBuf<double>& d, const double* src) {
+      d.alloc((size_t)std::max<int64_t>(n, 1));
+      std::vector<double> tmp((size_t)std::max<int64_t>(n, 1), 0.0);
+      if (src) std::copy(src, src + n, tmp.begin());
+      d.upload(tmp.data(), tmp.size(), stream);
+      PDFP_HIP(hipStreamSynchronize(stream));
+    };
+    vec64(lb64, var_lb);
+    vec64(ub64, var_ub);
+    vec64(w64, h_w.data());
+    vec64(xt64, nullptr);
+    vec64(x64, nullptr);
+    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
+    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
+    part_v.alloc((size_t)std::max(grid_for(std::max(m, n)), 1) + 8);
+    sums.alloc((size_t)kBatch * 32 + 16);
+    h_sums.alloc((size_t)kBatch * 32 + 16);
+    std::memset(h_sums.p, 0, sizeof(double) * ((size_t)kBatch * 32 + 16));
+    sums.upload(h_sums.p, (size_t)kBatch * 32 + 16, stream);
+    d_sp.alloc(kBatch);
+    h_sp.alloc(kBatch);
+    for (int q = 0; q < kBatch; ++q) h_sp.p[q] = BatchStep{1.0, 1.0, 1.0, 0.0, 0.0, {0, 0, 0}};
+    for (auto& t : tr) t.h_xt.assign((size_t)n, 0.0);
+    PDFP_HIP(hipStreamSynchronize(stream));
+  }
+
+  ~Batch() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+
+  void sync() { PDFP_HIP(hipStreamSynchronize(stream)); }
+  void check_k(int k) const {
+    if (k < 0 || k >= K) throw Error(PDFP_ERR_ARGUMENT, "trajectory index out of range");
+  }
+
+  // Column k of an interleaved vector from / to FP64 host data.
+  void scatter(DevBuf<float>& dst, int64_t len, int k, const double* src) {
+    std::vector<float> tmp((size_t)std::max<int64_t>(len, 1));
+    for (int64_t i = 0; i < len; ++i) tmp[(size_t)i] = (float)src[i];
+    tmpf.upload(tmp.data(), (size_t)len, stream);
+    if (len) k_batch_scatter<<<grid_for(len), 256, 0, stream>>>((int)len, k, tmpf.p, dst.p);
+    sync();
+  }
+  void gather(const DevBuf<float>& src, int64_t len, int k, double* out) {
+    if (len) k_batch_gather<<<grid_for(len), 256, 0, stream>>>((int)len, k, src.p, tmpf.p);
+    std::vector<float> tmp((size_t)std::max<int64_t>(len, 1));
+    tmpf.download(tmp.data(), (size_t)len, stream);
+    sync();
+    for (int64_t i = 0; i < len; ++i) out[i] = (double)tmp[(size_t)i];
+  }
+
+  void set_original_primal_residual(const double* D_r) {
+    std::vector<double> inv((size_t)m);
+    double acc = 0.0;
+    for (int64_t i = 0; i < m; ++i) {
+      inv[(size_t)i] = 1.0 / D_r[i];
+      const double v = h_b[(size_t)i] / D_r[i];
+      acc += v * v;
+    }
+    b_norm_original = std::sqrt(acc);
+    row_unscale.alloc((size_t)std::max<int64_t>(m, 1));
+    row_unscale.upload(inv.data(), (size_t)m, stream);
+    sync();
+    orig_primal = true;
+  }
+
+  void set_x(int k, const double* xs) {
+    check_k(k);
+    scatter(x, n, k, xs);
+    scatter(x0, n, k, xs);
+  }
+
+  void warm_start(int k) {
+    check_k(k);
+    const int64_t len = std::max(n, m);
+    if (len) k_batch_warm<<<grid_for(len), 256, 0, stream>>>((int)n, (int)m, k, x.p, y.p, z.p,
+                                                              x0.p, y0.p, z0.p);
+    sync();
+    Trajectory& t = tr[(size_t)k];
+    t.r.inner_count = 0;
+    t.r.total_count = 0;
+    t.r.fixed_point_error = INFINITY;
+    t.r.last_trial_fixed_point_error = INFINITY;
+    t.termination = PDFP_UNSPECIFIED;
+    t.x64_valid = false;
+  }
+
+  void set_target(int k, const double* xts) {
+    check_k(k);
+    scatter(xt, n, k, xts);
+    std::copy(xts, xts + n, tr[(size_t)k].h_xt.begin());
+    warm_start(k);
+  }
+
+  template <bool MAJOR>
+  void step_t(int k_offset) {
+    EpiBatchPrimalY<MAJOR> ex{d_sp.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+                              rx.p, px.p, ds.p, py.p, ry.p};
+    launch_pass_npb<256, 0, 0>(At, GatherBatch{z.p}, ex, nullptr, stream);
+    EpiBatchDualZ<MAJOR> ez{d_sp.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    launch_pass_npb<256, 0, 0>(A, GatherBatch{rx.p}, ez, nullptr, stream);
+  }
+  void step(int k_offset, bool major) {
+    if (major) step_t<true>(k_offset);
+    else step_t<false>(k_offset);
+  }
+
+  void evaluate(int k, int slot) {
+    EpiBatchEvalCol ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, k};
+    launch_pass<kEvalColTerms>(At, GatherDeltaStrided{pz.p, rz.p, k}, ec, part_t.p, stream);
+    EpiBatchEvalRow er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr, k};
+    launch_pass<kEvalRowTerms>(A, Gather1Strided{px.p, k}, er, part_a.p, stream);
+    double* out = sums.p + (size_t)k * 32 + (size_t)slot * 16;
+    if (At.nblocks)
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_t.p, At.nblocks, kEvalColTerms, 0, out);
+    if (A.nblocks)
+      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, kEvalRowTerms, 0,
+                                                    out + 8);
+  }
+
+  struct Eval {
+    double fpe, rel_primal, rel_dual, rel_gap, l1, primal_dist, dual_dist;
+  };
+  Eval read_eval(int k, int slot) const {
+    const double* S = h_sums.p + (size_t)k * 32 + (size_t)slot * 16;
+    const Trajectory& t = tr[(size_t)k];
+    const double pw = t.r.primal_weight;
+    Eval e;
+    const double movement = S[0] * pw + (S[1] + S[8]) / pw;
+    e.fpe = std::sqrt(jmax(movement + 2.0 * step_size * S[2], 0.0));
+    const double bn = orig_primal ? b_norm_original : b_norm;
+    e.rel_primal = std::sqrt(S[9]) / (1.0 + bn);
+    e.rel_dual = std::sqrt(S[3]) / dual_den;
+    e.rel_gap = std::fabs(S[5] + S[10] - S[4]) / t.l1_den;
+    e.l1 = S[4];
+    e.primal_dist = std::sqrt(S[6]);
+    e.dual_dist = std::sqrt(S[7] + S[11]);
+    return e;
+  }
+
+  void upload_steps(double omega, const std::vector<bool>& active) {
+    for (int q = 0; q < kBatch; ++q) {
+      BatchStep& s = h_sp.p[q];
+      s.step_size = step_size;
+      s.omega = omega;
+      if (q < K) {
+        s.primal_weight = tr[(size_t)q].r.primal_weight;
+        s.inner_count = (double)tr[(size_t)q].r.inner_count;
+      }
+      s.active = (q < K && active[(size_t)q]) ? 1.0 : 0.0;
+    }
+    PDFP_HIP(hipMemcpyAsync(d_sp.p, h_sp.p, sizeof(BatchStep) * kBatch, hipMemcpyHostToDevice,
+                            stream));
+  }
+
+  // pdfp_solve! (src/pdfp.jl:252-288) for every trajectory with active[k],
+  // in lockstep blocks of `freq` iterations.
+  void solve(const pdfp_params& p, const double* pw, const uint8_t* act) {
+    const double t_start = wall_seconds();
+    if (p.termination_evaluation_frequency < 1)
+      throw Error(PDFP_ERR_ARGUMENT, "termination_evaluation_frequency must be >= 1");
+    if (nnz > 0 && std::isnan(sv)) sv = power_iteration(A, At, m, n, p.sv_max_iterations,
+                                                       p.sv_tolerance, stream);
+    step_size = nnz == 0 ? 1.0 : 0.998 / std::sqrt(1.0 + sv * sv);
+    std::vector<bool> active((size_t)K, false);
+    for (int k = 0; k < K; ++k) {
+      if (act && !act[k]) continue;
+      if (!(pw[k] > 0.0)) throw Error(PDFP_ERR_ARGUMENT, "primal_weight must be > 0");
+      active[(size_t)k] = true;
+      Trajectory& t = tr[(size_t)k];
+      // initialize_step_size_and_primal_weight!  src/pdfp.jl:220-241
+      t.r.primal_weight = pw[k];
+      t.r.best_primal_weight = pw[k];
+      t.r.primal_weight_error_sum = 0.0;
+      t.r.primal_weight_last_error = 0.0;
+      double acc = 0.0;
+      for (int64_t j = 0; j < n; ++j)
+        acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(t.h_xt[(size_t)j]) : 0.0;
+      t.l1_den = 1.0 + acc;
+      t.do_restart = false;
+      t.n_restarts = 0;
+      t.n_evals = 0;
+      t.x64_valid = false;
+      if (t.r.total_count >= p.iteration_limit) {
+        t.termination = PDFP_ITERATION_LIMIT;
+        active[(size_t)k] = false;
+      }
+    }
+    const double omega = p.reflection_coefficient;
+    const int freq = p.termination_evaluation_frequency;
+    auto any = [&] {
+      for (bool a : active)
+        if (a) return true;
+      return false;
+    };
+    while (any()) {
+      upload_steps(omega, active);
+      bool any_restart = false;
+      for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && tr[(size_t)k].do_restart;
+      step(1, any_restart || freq == 1);
+      for (int k = 0; k < K; ++k)
+        if (active[(size_t)k] && tr[(size_t)k].do_restart) evaluate(k, 1);
+      for (int i = 2; i <= freq; ++i) step(i, i == freq);
+      for (int k = 0; k < K; ++k)
+        if (active[(size_t)k]) evaluate(k, 0);
+      PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
+                              hipMemcpyDeviceToHost, stream));
+      sync();
+      const double now = wall_seconds();
+      for (int k = 0; k < K; ++k) {
+        if (!active[(size_t)k]) continue;
+        Trajectory& t = tr[(size_t)k];
+        ++t.n_evals;
+        if (t.do_restart) {
+          t.r.fixed_point_error = read_eval(k, 1).fpe;
+          t.r.initial_fixed_point_error = t.r.fixed_point_error;
+          t.do_restart = false;
+        }
+        const Eval e = read_eval(k, 0);
+        t.r.fixed_point_error = e.fpe;
+        t.r.relative_primal_residual = e.rel_primal;
+        t.r.relative_dual_residual = e.rel_dual;
+        t.relative_l1_gap = e.rel_gap;
+        t.l1_distance = e.l1;
+        t.r.inner_count += freq;
+        t.r.total_count += freq;
+        bool stop = false;
+        if (t.r.relative_primal_residual <= p.eps_feasible_relative &&
+            t.r.relative_dual_residual <= p.eps_optimal_relative &&
+            t.relative_l1_gap <= p.eps_optimal_relative) {
+          t.termination = PDFP_OPTIMAL;
+          stop = true;
+        } else if (now - t_start > p.time_limit) {
+          t.termination = PDFP_TIME_LIMIT;
+          stop = true;
+        } else {
+          t.do_restart = should_restart(t.r, p);
+          if (t.do_restart) {
+            update_primal_weight(t.r, p, e.primal_dist, e.dual_dist);
+            const int64_t len = std::max(n, m);
+            if (len)
+              k_batch_restart<<<grid_for(len), 256, 0, stream>>>(
+                  (int)n, (int)m, k, px.p, py.p, pz.p, x.p, x0.p, y.p, y0.p, z.p, z0.p);
+            t.r.inner_count = 0;
+            t.r.last_trial_fixed_point_error = INFINITY;
+            ++t.n_restarts;
+          }
+          if (t.r.total_count >= p.iteration_limit) {
+            t.termination = PDFP_ITERATION_LIMIT;
+            stop = true;
+          }
+        }
+        if (stop) {
+          active[(size_t)k] = false;
+          t.solve_seconds = now - t_start;
+        }
+      }
+    }
+    sync();
+  }
+
+  void polish(int k) {
+    Trajectory& t = tr[(size_t)k];
+    if (t.x64_valid || n == 0) return;
+    xt64.upload(t.h_xt.data(), (size_t)n, stream);
+    const int g = grid_for(n);
+    k_batch_polish<<<g, 256, 0, stream>>>((int)n, k, px.p, lb.p, ub.p, lb64.p, ub64.p, xt64.p,
+                                          w64.p, x64.p, part_v.p);
+    finalize_partials<256><<<1, 256, 0, stream>>>(part_v.p, g, 1, 0, sums.p + kBatch * 32);
+    PDFP_HIP(hipMemcpyAsync(h_sums.p + kBatch * 32, sums.p + kBatch * 32, sizeof(double),
+                            hipMemcpyDeviceToHost, stream));
+    sync();
+    t.l1_distance = h_sums.p[kBatch * 32];
+    t.x64_valid = true;
+  }
+
+  void get_x(int k, double* out) {
+    check_k(k);
+    polish(k);
+    if (n) x64.download(out, (size_t)n, stream);
+    sync();
+    // the polish buffer is shared: the next polish recomputes
+    tr[(size_t)k].x64_valid = false;
+  }
+
+  void get_vector(int k, int which, double* out) {
+    check_k(k);
+    switch (which) {
+      case 0: get_x(k, out); break;
+      case 1: gather(py, n, k, out); break;
+      case 2: gather(pz, m, k, out); break;
+      case 3: gather(x, n, k, out); break;
+      case 4: gather(y, n, k, out); break;
+      case 5: gather(z, m, k, out); break;
+      case 6: gather(ds, n, k, out); break;
+      case 7: gather(xt, n, k, out); break;
+      default: throw Error(PDFP_ERR_ARGUMENT, "unknown vector id");
+    }
+  }
+
+  void info(int k, pdfp_info* o) {
+    check_k(k);
+    Trajectory& t = tr[(size_t)k];
+    if (t.termination != PDFP_UNSPECIFIED) {
+      polish(k);
+      t.x64_valid = false;
+    }
+    std::memset(o, 0, sizeof(*o));
+    o->relative_primal_residual = t.r.relative_primal_residual;
+    o->relative_dual_residual = t.r.relative_dual_residual;
+    o->relative_l1_gap = t.relative_l1_gap;
+    o->l1_distance = t.l1_distance;
+    o->fixed_point_error = t.r.fixed_point_error;
+    o->initial_fixed_point_error = t.r.initial_fixed_point_error;
+    o->primal_weight = t.r.primal_weight;
+    o->step_size = step_size;
+    o->max_singular_value = sv;
+    o->solve_seconds = t.solve_seconds;
+    o->total_count = t.r.total_count;
+    o->termination_reason = t.termination;
+    o->n_restarts = t.n_restarts;
+    o->precision = PDFP_PREC_FP32_MIXED;
+    o->n_evaluations = t.n_evals;
+    o->npb_a = A.npb;
+    o->npb_at = At.npb;
+    o->persist_a = A.persist;
+    o->persist_at = At.persist;
+  }
+
+  // Mean duration of the transpose pass and of the A pass of a batched
+  // iteration, each bracketed by its own events on the batch's stream.
+  void time_kernels(int iters, double* ms_at, double* ms_a) {
+    time_iterations(1);
+    std::vector<hipEvent_t> ev((size_t)iters * 4);
+    for (auto& e : ev) PDFP_HIP(hipEventCreate(&e));
+    for (int i = 0; i < iters; ++i) {
+      const int k = 2 + (i % 198);
+      EpiBatchPrimalY<false> ex{d_sp.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+                                rx.p, px.p, ds.p, py.p, ry.p};
+      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
+      launch_pass_npb<256, 0, 0>(At, GatherBatch{z.p}, ex, nullptr, stream);
+      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
+      EpiBatchDualZ<false> ez{d_sp.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
+      launch_pass_npb<256, 0, 0>(A, GatherBatch{rx.p}, ez, nullptr, stream);
+      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
+    }
+    sync();
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < iters; ++i) {
+      float t0 = 0.f, t1 = 0.f;
+      PDFP_HIP(hipEventElapsedTime(&t0, ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]));
+      PDFP_HIP(hipEventElapsedTime(&t1, ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]));
+      sa += t0;
+      sb += t1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *ms_at = sa / std::max(iters, 1);
+    *ms_a = sb / std::max(iters, 1);
+  }
+
+  double time_iterations(int iters) {
+    if (std::isnan(sv) && nnz > 0) {
+      pdfp_params p;
+      pdfp_default_params(&p);
+      sv = power_iteration(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
+      step_size = 0.998 / std::sqrt(1.0 + sv * sv);
+    }
+    std::vector<bool> active((size_t)K, true);
+    upload_steps(1.0, active);
+    step(1, false);
+    sync();
+    hipEvent_t e0, e1;
+    PDFP_HIP(hipEventCreate(&e0));
+    PDFP_HIP(hipEventCreate(&e1));
+    PDFP_HIP(hipEventRecord(e0, stream));
+    for (int i = 0; i < iters; ++i) step(2 + (i % 198), false);
+    PDFP_HIP(hipEventRecord(e1, stream));
+    PDFP_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / std::max(iters, 1);
+  }
+};
+
+}  // namespace pdfp
+
+struct pdfp_batch {
+  std::unique_ptr<pdfp::Batch> impl;
+  std::string err = "no error";
+};
+
+namespace {
+
+template <class F>
+int batch_guarded(pdfp_batch* ctx, F&& f) {
+  if (!ctx || !ctx->impl) return PDFP_ERR_ARGUMENT;
+  try {
+    f();
+    return PDFP_OK;
+  } catch (const pdfp::Error& e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "host allocation failed";
+    return PDFP_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return PDFP_ERR_DEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pdfp_batch_last_error(const pdfp_batch* ctx) {
+  if (!ctx) return pdfp::g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+int pdfp_batch_create(pdfp_batch** out, int32_t K, int64_t m, int64_t n, int64_t nnz,
+                      const int64_t* colptr, const int64_t* rowval, const double* nzval,
+                      int32_t index_base, const double* b, const uint8_t* is_eq,
+                      const uint8_t* is_int, const double* var_lb, const double* var_ub,
+                      const double* w, int32_t device) {
+  if (!out) return PDFP_ERR_ARGUMENT;
+  *out = nullptr;
+  auto fail = [&](int code, const std::string& msg) {
+    pdfp::g_create_error = msg;
+    return code;
+  };
+  if (K < 1 || K > pdfp::kBatch) return fail(PDFP_ERR_ARGUMENT, "K must be in 1..8");
+  if (m < 0 || n < 0 || nnz < 0) return fail(PDFP_ERR_ARGUMENT, "negative size");
+  if (!colptr || (nnz > 0 && (!rowval || !nzval)) || (m > 0 && (!b || !is_eq)) ||
+      (n > 0 && (!is_int || !var_lb || !var_ub || !w)))
+    return fail(PDFP_ERR_ARGUMENT, "null input array");
+  if (index_base != 0 && index_base != 1) return fail(PDFP_ERR_ARGUMENT, "index_base must be 0 or 1");
+  if (colptr[n] - index_base != nnz) return fail(PDFP_ERR_ARGUMENT, "colptr[n] does not match nnz");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(PDFP_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(PDFP_ERR_ARGUMENT, "device index out of range");
+  auto* ctx = new (std::nothrow) pdfp_batch();
+  if (!ctx) return fail(PDFP_ERR_NOMEM, "host allocation failed");
+  try {
+    ctx->impl.reset(new pdfp::Batch(K, m, n, colptr, rowval, nzval, index_base, b, is_eq, is_int,
+                                    var_lb, var_ub, w, device));
+  } catch (const pdfp::Error& e) {
+    delete ctx;
+    return fail(e.code, e.what());
+  } catch (const std::exception& e) {
+    delete ctx;
+    return fail(PDFP_ERR_DEVICE, e.what());
+  }
+  *out = ctx;
+  return PDFP_OK;
+}
+
+int pdfp_batch_set_original_primal_residual(pdfp_batch* ctx, const double* D_r) {
+  return batch_guarded(ctx, [&] {
+    if (!D_r) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null D_r");
+    ctx->impl->set_original_primal_residual(D_r);
+  });
+}
+
+int pdfp_batch_set_x(pdfp_batch* ctx, int32_t k, const double* x) {
+  return batch_guarded(ctx, [&] {
+    if (!x) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null x");
+    ctx->impl->set_x(k, x);
+  });
+}
+
+int pdfp_batch_set_target(pdfp_batch* ctx, int32_t k, const double* xt) {
+  return batch_guarded(ctx, [&] {
+    if (!xt) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null target");
+    ctx->impl->set_target(k, xt);
+  });
+}
+
+int pdfp_batch_warm_start(pdfp_batch* ctx, int32_t k) {
+  return batch_guarded(ctx, [&] { ctx->impl->warm_start(k); });
+}
+
+int pdfp_batch_solve(pdfp_batch* ctx, const pdfp_params* params, const double* primal_weight,
+                     const uint8_t* active, int32_t* termination, int64_t* total_count) {
+  return batch_guarded(ctx, [&] {
+    if (!primal_weight) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null primal weights");
+    pdfp_params p;
+    if (params) p = *params;
+    else pdfp_default_params(&p);
+    ctx->impl->solve(p, primal_weight, active);
+    for (int k = 0; k < ctx->impl->K; ++k) {
+      if (termination) termination[k] = ctx->impl->tr[(size_t)k].termination;
+      if (total_count) total_count[k] = ctx->impl->tr[(size_t)k].r.total_count;
+    }
+  });
+}
+
+int pdfp_batch_get_x(pdfp_batch* ctx, int32_t k, double* out) {
+  return batch_guarded(ctx, [&] {
+    if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    ctx->impl->get_x(k, out);
+  });
+}
+
+int pdfp_batch_get_vector(pdfp_batch* ctx, int32_t k, int32_t which, double* out) {
+  return batch_guarded(ctx, [&] {
+    if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    ctx->impl->get_vector(k, which, out);
+  });
+}
+
+int pdfp_batch_get_info(pdfp_batch* ctx, int32_t k, pdfp_info* out) {
+  return batch_guarded(ctx, [&] {
+    if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
+    ctx->impl->info(k, out);
+  });
+}
+
+int pdfp_batch_time_iterations(pdfp_batch* ctx, int32_t iterations, double* ms_out) {
+  return batch_guarded(ctx, [&] {
+    if (!ms_out || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
+    *ms_out = ctx->impl->time_iterations(iterations);
+  });
+}
+
+int pdfp_batch_time_kernels(pdfp_batch* ctx, int32_t iterations, double* ms_at, double* ms_a) {
+  return batch_guarded(ctx, [&] {
+    if (!ms_at || !ms_a || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
+    ctx->impl->time_kernels(iterations, ms_at, ms_a);
+  });
+}
+
+void pdfp_batch_destroy(pdfp_batch* ctx) { delete ctx; }
+
+}  // extern "C"

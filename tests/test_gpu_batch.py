@@ -1,0 +1,74 @@
+"""GPU tests of the shared-matrix trajectory batch (pdfp_batch_*).
+
+Each trajectory must behave as an independent FP32 projection: its KKT
+residuals, re-evaluated in FP64 by the oracle, meet the tolerance (×10 for
+FP32 rounding), its ℓ1 distance matches the single FP32 projector's to
+1e-4 relative, and trajectories that are not active are left unchanged."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pdfp_amd import generators as G
+from pdfp_amd import pdfp as P
+from pdfp_amd.batch import PDFPBatch
+from pdfp_amd.pdfp import SolverParameters, projector_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(pp, K):
+    return PDFPBatch(K, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+
+
+def _oracle_kkt(pp, xt_scaled, x, y, z, ds):
+    o = O.OraclePDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    O.set_target(o, xt_scaled)
+    o.initialize(None, 1.0)
+    return o.residual_of(x, y, z, ds)
+
+
+@pytest.mark.parametrize("K", [8, 3])
+def test_batch_trajectories_are_independent_projections(K):
+    print("batch", K, flush=True)
+    m = G.mixed(400, 800, 31)
+    pp = projector_problem(m)
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    bt = _batch(pp, K)
+    targets = [G.binary_targets(m, 500 + k) / pp.D_c for k in range(K)]
+    for k in range(K):
+        bt.set_target(k, targets[k])
+    terms = bt.solve(p, np.linspace(0.8, 1.5, K))
+    assert terms == ["OPTIMAL"] * K
+    for k in range(K):
+        x = bt.pdhg_x(k)
+        r = _oracle_kkt(pp, targets[k], x, bt.vector(k, "pdhg_y"), bt.vector(k, "pdhg_z"),
+                        bt.vector(k, "dual_slack"))
+        assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (k, r)
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, targets[k])
+        assert P.pdfp_solve(s, p, float(np.linspace(0.8, 1.5, K)[k])) == "OPTIMAL"
+        l1b, l1s = bt.info(k).l1_distance, s.info().l1_distance
+        assert abs(l1b - l1s) <= 1e-4 * max(1.0, l1s), (k, l1b, l1s)
+
+
+def test_batch_inactive_trajectory_is_untouched_and_warm_starts():
+    m = G.set_cover(133, 201, 4)
+    pp = projector_problem(m)
+    bt = _batch(pp, 4)
+    p = SolverParameters(eps_optimal_relative=1e-6, eps_feasible_relative=1e-6)
+    for k in range(4):
+        bt.set_target(k, np.zeros(201))
+    assert bt.solve(p, 1.0) == ["OPTIMAL"] * 4
+    before = [bt.pdhg_x(k) for k in range(4)]
+    bt.set_target(1, np.ones(201) / pp.D_c)        # feasible target: distance 0
+    terms = bt.solve(p, 1.0, active=[0, 1, 0, 0])
+    assert terms[1] == "OPTIMAL"
+    for k in (0, 2, 3):
+        assert np.array_equal(bt.pdhg_x(k), before[k])
+    assert bt.info(1).l1_distance <= 1e-5
+    # the same targets in all slots give the same projection
+    l1 = [bt.info(k).l1_distance for k in (0, 2, 3)]
+    assert max(l1) - min(l1) <= 1e-6 * max(1.0, max(l1))
