@@ -73,7 +73,7 @@ struct Batch {
     A.upload(ha, stream);
     At.upload(ht, stream);
     // eight gathered values per nonzero: the staged products need 64 bytes
-    // each, so the schedule keeps blocks at 256 nonzeros
+    // each, so blocks hold 256 or 512 nonzeros (autotune() picks)
     A.schedule(256, stream);
     At.schedule(256, stream);
     tr.resize((size_t)K);
@@ -120,8 +120,6 @@ struct Batch {
     vec64(w64, h_w.data());
     vec64(xt64, nullptr);
     vec64(x64, nullptr);
-    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
-    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
     part_v.alloc((size_t)std::max(grid_for(std::max(m, n)), 1) + 8);
     sums.alloc((size_t)kBatch * 32 + 16);
     h_sums.alloc((size_t)kBatch * 32 + 16);
@@ -131,6 +129,9 @@ struct Batch {
     h_sp.alloc(kBatch);
     for (int q = 0; q < kBatch; ++q) h_sp.p[q] = BatchStep{1.0, 1.0, 1.0, 0.0, 0.0, {0, 0, 0}};
     for (auto& t : tr) t.h_xt.assign((size_t)n, 0.0);
+    autotune();
+    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
+    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
     PDFP_HIP(hipStreamSynchronize(stream));
   }
 
@@ -205,13 +206,61 @@ struct Batch {
     warm_start(k);
   }
 
+  // The batched passes stage 8 products per nonzero: 256 or 512 nonzeros
+  // per block (48 KB of LDS at 512).
+  template <class G, class E>
+  void launch_batch(const DevCsr<float>& M, G g, E e) {
+    if (M.nblocks == 0) return;
+    if (M.npb == 512) launch_pass_npb<512, 0, 0>(M, g, e, nullptr, stream);
+    else launch_pass_npb<256, 0, 0>(M, g, e, nullptr, stream);
+  }
+
   template <bool MAJOR>
   void step_t(int k_offset) {
     EpiBatchPrimalY<MAJOR> ex{d_sp.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                               rx.p, px.p, ds.p, py.p, ry.p};
-    launch_pass_npb<256, 0, 0>(At, GatherBatch{z.p}, ex, nullptr, stream);
+    launch_batch(At, GatherBatch{z.p}, ex);
     EpiBatchDualZ<MAJOR> ez{d_sp.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-    launch_pass_npb<256, 0, 0>(A, GatherBatch{rx.p}, ez, nullptr, stream);
+    launch_batch(A, GatherBatch{rx.p}, ez);
+  }
+
+  // Schedule autotuner of the batched passes: time whole iterations with
+  // every trajectory inactive (the iterates keep their values) for each
+  // nonzeros-per-block of A and Aᵀ and each grid form; keep the fastest.
+  void autotune() {
+    if (nnz == 0) return;
+    std::vector<bool> none((size_t)K, false);
+    upload_steps(1.0, none);
+    hipEvent_t e0, e1;
+    PDFP_HIP(hipEventCreate(&e0));
+    PDFP_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bA = 256, bAt = 256, bP = 0;
+    for (int na : {256, 512})
+      for (int nt : {256, 512})
+        for (int persist : {0, 4}) {
+          A.schedule(na, stream);
+          At.schedule(nt, stream);
+          A.persist = At.persist = persist;
+          for (int w = 0; w < 2; ++w) step(2, false);
+          PDFP_HIP(hipEventRecord(e0, stream));
+          for (int r = 0; r < 8; ++r) step(2, false);
+          PDFP_HIP(hipEventRecord(e1, stream));
+          PDFP_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            bA = na;
+            bAt = nt;
+            bP = persist;
+          }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A.schedule(bA, stream);
+    At.schedule(bAt, stream);
+    A.persist = At.persist = bP;
   }
   void step(int k_offset, bool major) {
     if (major) step_t<true>(k_offset);
@@ -451,11 +500,11 @@ struct Batch {
       EpiBatchPrimalY<false> ex{d_sp.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                                 rx.p, px.p, ds.p, py.p, ry.p};
       PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
-      launch_pass_npb<256, 0, 0>(At, GatherBatch{z.p}, ex, nullptr, stream);
+      launch_batch(At, GatherBatch{z.p}, ex);
       PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
       EpiBatchDualZ<false> ez{d_sp.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
       PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
-      launch_pass_npb<256, 0, 0>(A, GatherBatch{rx.p}, ez, nullptr, stream);
+      launch_batch(A, GatherBatch{rx.p}, ez);
       PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
     }
     sync();

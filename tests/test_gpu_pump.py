@@ -1,0 +1,42 @@
+"""The pump (src/pump.jl) with the device root LP and device projector.
+
+- test/runtests.jl:73-79 on the device: the pump finds a feasible point on
+  the four example instances, in both projector forms.
+- README.md:36-42 (p0201): the root LP ends OPTIMAL after 600 iterations,
+  the first rounding violates the rows by 9, and pump iteration 1 has ℓ1
+  distance 8.08613, violation 1 and 600 PDHG iterations — the reference's
+  log, reproduced by the FP64 form (the later iterations depend on Julia's
+  random stream and on the reference's CUDA summation order).
+"""
+import re
+
+import numpy as np
+import pytest
+
+from pdfp_amd.milp import is_feasible, read_mps
+from pdfp_amd.pump import feasibility_pump
+
+from .conftest import example_path
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["p0201", "gen-ip054", "pk1", "markshare2"])
+def test_pump_on_examples_device(name, precision):
+    print("pump", name, precision, flush=True)
+    m = read_mps(example_path(name))
+    r = feasibility_pump(m, time_limit=120.0, precision=precision)
+    assert r.found
+    assert is_feasible(m, r.x)
+    assert abs(r.objective - (float(m.c @ r.x) + m.c0)) <= 1e-9 * (1 + abs(r.objective))
+
+
+def test_p0201_log_matches_reference_readme():
+    m = read_mps(example_path("p0201"))
+    r = feasibility_pump(m, time_limit=60.0, precision="fp64")
+    log = "".join(r.log)
+    assert re.search(r"root LP\s+OPTIMAL after 600 iterations", log)
+    assert "first rounding violates by 9)" in log
+    it1 = [line for line in r.log if re.match(r"\s+1\s", line)][0].split()
+    assert it1[1] == "8.08613" and it1[2] == "1" and it1[3] == "600"
