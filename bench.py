@@ -270,8 +270,12 @@ def main():
         it, dt, st = run_batch(bt, pp, milp, sets, args.tol)
         ms_at, ms_a = bt.time_kernels(100)
         ms_iter = bt.time_iterations(200)
+        bi = bt.info(0)
         results["batch"] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
-                                ms_iter=ms_iter)
+                                ms_iter=ms_iter,
+                                npb={"A": bi.npb_a, "At": bi.npb_at,
+                                     "A_persistent_wg_per_cu": bi.persist_a,
+                                     "At_persistent_wg_per_cu": bi.persist_at})
         print(f"[bench] fp32 batch of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; batched iteration "
               f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)
@@ -319,6 +323,8 @@ def main():
                                   "row_pass": r32["ms_a"] * 1e3},
         "schedule_nnz_per_block": r32["npb"],
     }
+    if K > 1:
+        out["batch_schedule_nnz_per_block"] = head["npb"]
     if "fp64" in results:
         r64 = results["fp64"]
         v64 = r64["iters"] / r64["seconds"]

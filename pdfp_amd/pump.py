@@ -25,6 +25,9 @@ import numpy as np
 from .milp import MILP, is_integral, max_violation, num_constraints, num_continuous, num_variables
 from .rounding import BatchPolicy, Rounder, jround
 
+# Smallest relative KKT tolerance handed to the FP32 projector form.
+FP32_TOLERANCE_FLOOR = 1e-5
+
 
 @dataclass
 class PumpResult:
@@ -296,6 +299,34 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
     emit(f"{'iter':>6s} {'distance':>12s} {'violation':>10s} {'pdhg':>10s} {'gap tol':>9s} "
          f"{'action':>8s} {'time':>9s}\n")
     params_cls = _pdfp_mod.SolverParameters
+    proj = {"state": projector_state}
+
+    def solve_projection(params, target_scaled=None):
+        # FP32 iterates do not resolve relative KKT errors much below 1e-5:
+        # from the first projection asked for less, the FP64 form takes over,
+        # started at the last FP32 projection.
+        st = proj["state"]
+        if (getattr(st, "precision", None) == "fp32" and
+                min(params.eps_optimal_relative,
+                    params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR):
+            st64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                               precision="fp64", device=device)
+            if mixed:
+                P.original_primal_residual(st64, pp.D_r)
+            st64.set_x(st.pdhg_x)
+            P.set_target(st64, proj["target"] if target_scaled is None else target_scaled)
+            emit(f"projector   FP64 form from gap tolerance "
+                 f"{params.eps_optimal_relative:.2e} on\n")
+            proj["state"] = st = st64
+        elif target_scaled is not None:
+            P.set_target(st, target_scaled)
+        else:
+            P.warm_start(st)
+        if target_scaled is not None:
+            proj["target"] = target_scaled
+        P.pdfp_solve(st, params, primal_weight)
+        return D_c * st.pdhg_x, st.total_count
+
     while elapsed() < time_limit:
         violation = max_violation(milp, xt)
         if violation <= tol and is_integral(xt, int_idx, tol):
@@ -306,14 +337,11 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
 
         # Project x̃ onto the LP relaxation.
         t = time.perf_counter()
-        P.set_target(projector_state, xt / D_c)
         gap_used = gap
         params = params_cls(iteration_limit=projection_iteration_limit,
                             eps_optimal_relative=gap, eps_feasible_relative=gap * residual_ratio,
                             time_limit=remaining())
-        P.pdfp_solve(projector_state, params, primal_weight)
-        x = D_c * projector_state.pdhg_x
-        inner = projector_state.total_count
+        x, inner = solve_projection(params, xt / D_c)
         # Mixed models: re-solve an integral projection whose rounding is
         # infeasible only through its continuous part (src/pump.jl:340-356).
         while (mixed and tightening_budget > 1e-6 and elapsed() < time_limit
@@ -325,10 +353,8 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                                 eps_optimal_relative=gap,
                                 eps_feasible_relative=gap * residual_ratio,
                                 time_limit=remaining())
-            P.warm_start(projector_state)
-            P.pdfp_solve(projector_state, params, primal_weight)
-            x = D_c * projector_state.pdhg_x
-            inner += projector_state.total_count
+            x, count = solve_projection(params)
+            inner += count
         stats.pdhg_iterations += inner
         if gap > gap_floor:
             gap = max(gap * gap_decay, gap_floor)

@@ -109,3 +109,38 @@ def test_round_to_nearest_and_flip():
     xt = np.array([1.0, 0.0])
     flip_farthest(xt, np.array([1.9, 0.0]), m, 5)
     assert xt[0] == 2.0
+
+
+def test_tight_projections_leave_the_fp32_form():
+    """Projections asked for a relative KKT tolerance below what FP32
+    iterates resolve run in the FP64 form (pk1 is mixed: its integral
+    projections are tightened towards 1e-8, src/pump.jl:340-356)."""
+    from types import SimpleNamespace
+
+    from oracle import oracle as O
+    from pdfp_amd.pump import FP32_TOLERANCE_FLOOR
+
+    calls = []
+
+    def make_state(*args, precision="fp32", **kw):
+        s = O.OraclePDFPState(*args, **kw)
+        s.precision = precision
+        return s
+
+    def solve(s, p, pw):
+        calls.append((s.precision, min(p.eps_optimal_relative, p.eps_feasible_relative)))
+        return O.pdfp_solve(s, p, pw)
+
+    mod = SimpleNamespace(PDFPState=make_state, set_target=O.set_target,
+                          warm_start=O.warm_start, pdfp_solve=solve,
+                          original_primal_residual=O.original_primal_residual)
+    m = read_mps(example_path("pk1"))
+    r = feasibility_pump(m, projector=mod, root_lp=oracle_root_lp, time_limit=120.0,
+                         precision="fp32")
+    assert r.found and is_feasible(m, r.x)
+    assert calls and calls[0][0] == "fp32"
+    assert all(prec == "fp64" for prec, eps in calls if eps < FP32_TOLERANCE_FLOOR)
+    switched = [i for i, (prec, _) in enumerate(calls) if prec == "fp64"]
+    if switched:
+        assert all(prec == "fp64" for prec, _ in calls[switched[0]:])
+        assert any("FP64 form" in line for line in r.log)
