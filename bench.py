@@ -10,15 +10,24 @@ x0 (seed 1).  A step is one warm-started ℓ1 projection toward a new random
 projector (src/pump.jl:329-359).  The scaled matrix, its transpose and every
 iterate stay in HBM; the timed region covers set_target + pdfp_solve.
 
-value = Σ PDHG iterations / Σ projection wall time of the product form (FP32
-data and iterates, fused launches, FP64 reductions), the reference's own
-accounting (src/pump.jl:357-359).  The plain FP64 form (the reference's
-launch structure) runs the same projections and is reported beside it.
+value = Σ PDHG iterations / Σ projection wall time of the batched product
+form: 8 trajectories sharing the matrix in HBM, FP32 data and iterates,
+fused launches, FP64 reductions (``--batch 1``: a single trajectory), the
+reference's own accounting (src/pump.jl:357-359).  The single-trajectory
+FP32 form and the plain FP64 form (the reference's launch structure) run the
+same kind of projections and are reported beside it.
 
 Also reported: the roofline of the dominant kernel (algorithmic bytes per
-launch over its mean duration, timed with HIP events on the library's
-stream) and the CPU baseline (the FP64 C oracle on one host core, on a
-bounded number of iterations of the same workload).
+launch over its mean duration, timed with the kernel's own start/stop events
+on the library's stream; HBM traffic per launch from the committed rocprofv3
+counter summary of the same workload, when there is one), the CPU baseline
+(the FP64 C oracle on one host core, on a bounded number of iterations of the
+same workload) and the time to the first feasible point of two small MILPs.
+
+    python bench.py --copy-calibration
+
+copies 1 GiB device to device ten times (the plain copy kernel whose
+FETCH_SIZE / WRITE_SIZE counters calibrate the traffic figures).
 """
 from __future__ import annotations
 
@@ -197,6 +206,35 @@ def time_to_first_feasible(runs: int, cpu: bool):
     return out
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 10):
+    """A plain device-to-device copy of known size: the counters of its
+    dispatches give the scale of FETCH_SIZE and WRITE_SIZE."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}),
+          flush=True)
+
+
+def measured_traffic(workload: str, kernel_key: str):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (profiles/pmc_traffic_<workload>.md, its ```json block), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if "```json" not in text:
+        return None
+    block = text.split("```json", 1)[1].split("```", 1)[0]
+    rec = json.loads(block).get(kernel_key)
+    return None if rec is None else rec["traffic_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +248,12 @@ def main():
     ap.add_argument("--no-ttff", action="store_true")
     ap.add_argument("--batch", type=int, default=8,
                     help="trajectories per step of the headline measurement (1: single)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the projector has no multi-GPU path in this build "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
@@ -291,10 +334,13 @@ def main():
                       f"tolerance {args.tol:g}")
     r32 = results["fp32"]
     value = head["iters"] / head["seconds"]
-    dom_bytes, dom_ms, dom_name = max((bat, head["ms_at"], "transpose pass (A^T z + x/y update)"),
-                                      (ba, head["ms_a"], "row pass (A xbar + z update)"),
-                                      key=lambda t: t[1])
+    keys = (("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>") if K > 1 else
+            ("EpiPrimalY<float, false>", "EpiDualZ<float, false>"))
+    dom_bytes, dom_ms, dom_name, dom_key = max(
+        (bat, head["ms_at"], "transpose pass (A^T z + x/y update)", keys[0]),
+        (ba, head["ms_a"], "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(args.workload, dom_key)
     out = {
         "metric": METRIC,
         "value": value,
@@ -313,7 +359,7 @@ def main():
                    "iterations_timed": head["iters"]},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "traffic": traffic, "algorithmic_bytes": dom_bytes,
                      "kernel_ms": dom_ms},
         "iteration_us": head["ms_iter"] * 1e3,
         "kernel_us": {"transpose_pass": head["ms_at"] * 1e3, "row_pass": head["ms_a"] * 1e3},

@@ -209,10 +209,14 @@ struct Batch {
   // The batched passes stage 8 products per nonzero: 256 or 512 nonzeros
   // per block (48 KB of LDS at 512).
   template <class G, class E>
-  void launch_batch(const DevCsr<float>& M, G g, E e) {
-    if (M.nblocks == 0) return;
-    if (M.npb == 512) launch_pass_npb<512, 0, 0>(M, g, e, nullptr, stream);
-    else launch_pass_npb<256, 0, 0>(M, g, e, nullptr, stream);
+  void launch_batch(const DevCsr<float>& M, G g, E e, hipEvent_t t0 = nullptr,
+                    hipEvent_t t1 = nullptr) {
+    if (M.nblocks == 0) {
+      launch_pass<0>(M, g, e, nullptr, stream, t0, t1);  // records the events only
+      return;
+    }
+    if (M.npb == 512) launch_pass_npb<512, 0, 0>(M, g, e, nullptr, stream, t0, t1);
+    else launch_pass_npb<256, 0, 0>(M, g, e, nullptr, stream, t0, t1);
   }
 
   template <bool MAJOR>
@@ -499,13 +503,11 @@ struct Batch {
       const int k = 2 + (i % 198);
       EpiBatchPrimalY<false> ex{d_sp.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                                 rx.p, px.p, ds.p, py.p, ry.p};
-      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
-      launch_batch(At, GatherBatch{z.p}, ex);
-      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
+      launch_batch(At, GatherBatch{z.p}, ex,
+          ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
       EpiBatchDualZ<false> ez{d_sp.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
-      launch_batch(A, GatherBatch{rx.p}, ez);
-      PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
+      launch_batch(A, GatherBatch{rx.p}, ez,
+          ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
     }
     sync();
     double sa = 0.0, sb = 0.0;

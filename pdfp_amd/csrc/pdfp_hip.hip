@@ -526,26 +526,22 @@ struct Projector final : ProjectorBase {
       if constexpr (kFused) {
         EpiPrimalY<T, false> ex{d_sp.p, k, x.p, x0.p, y.p, y0.p, lb.p, ub.p, xt.p, w.p,
                                 rx.p, px.p, ds.p, py.p, ry.p};
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
-        launch_pass<0>(At, Gather1<T>{z.p}, ex, nullptr, stream);
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
+        launch_pass<0>(At, Gather1<T>{z.p}, ex, nullptr, stream,
+            ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
         EpiDualZ<T, false> ez{d_sp.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
-        launch_pass<0>(A, Gather1<T>{rx.p}, ez, nullptr, stream);
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
+        launch_pass<0>(A, Gather1<T>{rx.p}, ez, nullptr, stream,
+            ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
       } else {
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 0], stream));
-        launch_pass<0>(At, Gather1<double>{z.p}, EpiStore<double>{atz64.p}, nullptr, stream);
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 1], stream));
+        launch_pass<0>(At, Gather1<double>{z.p}, EpiStore<double>{atz64.p}, nullptr, stream,
+            ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
         if (n) {
           k_primal<<<grid_for(n), 256, 0, stream>>>(d_sp.p, k, (int)n, px.p, ds.p, rx.p, x.p,
                                                     x0.p, y.p, atz64.p, lb.p, ub.p);
           k_dual_y<<<grid_for(n), 256, 0, stream>>>(d_sp.p, k, (int)n, py.p, y.p, ry.p, y0.p,
                                                     xt.p, rx.p, w.p);
         }
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 2], stream));
-        launch_pass<0>(A, Gather1<double>{rx.p}, EpiStore<double>{ax64.p}, nullptr, stream);
-        PDFP_HIP(hipEventRecord(ev[(size_t)i * 4 + 3], stream));
+        launch_pass<0>(A, Gather1<double>{rx.p}, EpiStore<double>{ax64.p}, nullptr, stream,
+            ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
         if (m)
           k_dual_z<<<grid_for(m), 256, 0, stream>>>(d_sp.p, k, (int)m, pz.p, z.p, rz.p, z0.p,
                                                     ax64.p, b.p, is_eq.p);

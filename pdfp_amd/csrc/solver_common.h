@@ -4,6 +4,7 @@
 // iteration for ‖A‖₂ (src/preconditioner.jl:108-125).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +14,7 @@
 #include <vector>
 
 #include "../../include/pdfp_hip.h"
+
 #include "engine.h"
 #include "host_util.h"
 
@@ -163,27 +165,47 @@ struct DevCsr {
 
 // Launch one pass of a given NPB: one workgroup per row block, or (when the
 // operator's `persist` > 0) a persistent grid of persist × 256 workgroups
-// (256 CUs) that walks the blocks.
+// (256 CUs) that walks the blocks.  With events t0/t1 the launch records the
+// kernel's own start and end (hipExtLaunchKernel), which times the kernel
+// alone, without the gaps that events recorded around it on the stream add.
 template <int NPB, int NACC, int NMAX, typename TV, class G, class E>
-inline void launch_pass_npb(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
+inline void launch_pass_npb(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s,
+                            hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
   const int grid = A.persist * 256;
-  if (A.persist > 0 && grid < A.nblocks)
+  const bool persist = A.persist > 0 && grid < A.nblocks;
+  if (t0) {
+    if (persist)
+      hipExtLaunchKernelGGL(csr_pass_persistent<kBlock, NPB, TV, G, E, NACC, NMAX>, dim3(grid),
+                            dim3(kBlock), 0, s, t0, t1, 0, A.view(), g, e, partials);
+    else
+      hipExtLaunchKernelGGL(csr_pass<kBlock, NPB, TV, G, E, NACC, NMAX>, dim3(A.nblocks),
+                            dim3(kBlock), 0, s, t0, t1, 0, A.view(), g, e, partials);
+    PDFP_HIP(hipGetLastError());
+  } else if (persist) {
     csr_pass_persistent<kBlock, NPB, TV, G, E, NACC, NMAX>
         <<<grid, kBlock, 0, s>>>(A.view(), g, e, partials);
-  else
+  } else {
     csr_pass<kBlock, NPB, TV, G, E, NACC, NMAX>
         <<<A.nblocks, kBlock, 0, s>>>(A.view(), g, e, partials);
+  }
 }
 
 // Launch one CSR pass with the operator's schedule (no-op if it is empty).
 template <int NACC, int NMAX = 0, typename TV, class G, class E>
-inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s) {
-  if (A.nblocks == 0) return;
+inline void launch_pass(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s,
+                        hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+  if (A.nblocks == 0) {
+    if (t0) {
+      PDFP_HIP(hipEventRecord(t0, s));
+      PDFP_HIP(hipEventRecord(t1, s));
+    }
+    return;
+  }
   switch (A.npb) {
-    case 256: launch_pass_npb<256, NACC, NMAX>(A, g, e, partials, s); break;
-    case 512: launch_pass_npb<512, NACC, NMAX>(A, g, e, partials, s); break;
-    case 1024: launch_pass_npb<1024, NACC, NMAX>(A, g, e, partials, s); break;
-    default: launch_pass_npb<2048, NACC, NMAX>(A, g, e, partials, s); break;
+    case 256: launch_pass_npb<256, NACC, NMAX>(A, g, e, partials, s, t0, t1); break;
+    case 512: launch_pass_npb<512, NACC, NMAX>(A, g, e, partials, s, t0, t1); break;
+    case 1024: launch_pass_npb<1024, NACC, NMAX>(A, g, e, partials, s, t0, t1); break;
+    default: launch_pass_npb<2048, NACC, NMAX>(A, g, e, partials, s, t0, t1); break;
   }
 }
 

@@ -1,0 +1,133 @@
+"""Short text summaries of rocprofv3 output for the committed profiles.
+
+    python -m pdfp_amd.rocprof_summary stats KERNEL_STATS_CSV [--top N]
+    python -m pdfp_amd.rocprof_summary traffic --fetch CSV --write CSV \\
+        --copy-fetch CSV --copy-write CSV
+
+``stats`` turns a ``--kernel-trace --stats`` kernel_stats.csv into a
+markdown table.  ``traffic`` reads four ``--pmc`` counter_collection.csv
+files (FETCH_SIZE and WRITE_SIZE of the benchmark, and of the 1 GiB plain
+copy of ``bench.py --copy-calibration``), scales each counter by what it
+reads for the copy of known size, and prints a table plus a JSON block of
+HBM bytes per launch per CSR pass epilogue, which bench.py reads back.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import json
+
+
+def pass_key(name: str) -> str | None:
+    """The epilogue of a csr_pass / csr_pass_persistent instantiation (its
+    fifth template argument), e.g. 'EpiBatchPrimalY<false>'; None for other
+    kernels."""
+    for head in ("csr_pass_persistent<", "csr_pass<"):
+        i = name.find(head)
+        if i >= 0:
+            break
+    else:
+        return None
+    i += len(head)
+    depth, args, cur = 0, [], ""
+    for ch in name[i:]:
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            if depth == 0:
+                args.append(cur.strip())
+                break
+            depth -= 1
+        elif ch == "," and depth == 0:
+            args.append(cur.strip())
+            cur = ""
+            continue
+        cur += ch
+    return args[4].replace("pdfp::", "") if len(args) > 4 else None
+
+
+def short_name(name: str) -> str:
+    key = pass_key(name)
+    if key is None:
+        return name.split("(")[0].replace("pdfp::", "")[:70]
+    head = name.split("(")[0].replace("pdfp::", "").replace("void ", "")
+    return head[:110]
+
+
+def stats(path: str, top: int) -> str:
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = ["| kernel | calls | mean us | min us | max us | % time |",
+           "|---|---:|---:|---:|---:|---:|"]
+    for r in rows[:top]:
+        out.append(f"| `{short_name(r['Name'])}` | {r['Calls']} | "
+                   f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
+                   f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def _per_kernel(path: str) -> dict:
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return d
+
+
+def _copy_kib(path: str) -> float:
+    """Mean counter value (KiB) over the large copy dispatches."""
+    vals = [v for k, vs in _per_kernel(path).items() if "copyBuffer" in k or "copy" in k.lower()
+            for v in vs]
+    big = max(vals)
+    sel = [v for v in vals if v >= 0.5 * big]
+    return sum(sel) / len(sel)
+
+
+def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str,
+            copy_bytes: int = 1 << 30) -> str:
+    ff = copy_bytes / (_copy_kib(copy_fetch) * 1024.0)
+    fw = copy_bytes / (_copy_kib(copy_write) * 1024.0)
+    F, W = _per_kernel(fetch), _per_kernel(write)
+    # per epilogue: the instantiation with the most dispatches (the tuned one)
+    best = {}
+    for name, vals in F.items():
+        key = pass_key(name)
+        if key is None or name not in W:
+            continue
+        if key not in best or len(vals) > len(F[best[key]]):
+            best[key] = name
+    recs = {}
+    lines = [f"copy calibration: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f} "
+             f"(1 GiB copy / counter reading)", "",
+             "| epilogue | dispatches | FETCH KiB | WRITE KiB | HBM MB per launch |",
+             "|---|---:|---:|---:|---:|"]
+    for key, name in sorted(best.items(), key=lambda kv: -len(F[kv[1]])):
+        fk = sum(F[name]) / len(F[name])
+        wk = sum(W[name]) / len(W[name])
+        t = fk * 1024.0 * ff + wk * 1024.0 * fw
+        recs[key] = {"kernel": short_name(name), "dispatches": len(F[name]),
+                     "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                     "traffic_bytes": round(t)}
+        lines.append(f"| `{key}` | {len(F[name])} | {fk:.0f} | {wk:.0f} | {t / 1e6:.2f} |")
+    lines += ["", "```json", json.dumps(recs, indent=1), "```"]
+    return "\n".join(lines)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--top", type=int, default=15)
+    t = sub.add_parser("traffic")
+    for a in ("--fetch", "--write", "--copy-fetch", "--copy-write"):
+        t.add_argument(a, required=True)
+    a = ap.parse_args()
+    if a.cmd == "stats":
+        print(stats(a.csv, a.top))
+    else:
+        print(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write))
+
+
+if __name__ == "__main__":
+    main()
