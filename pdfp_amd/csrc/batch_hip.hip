@@ -212,7 +212,10 @@ struct Batch {
   void launch_batch(const DevCsr<float>& M, G g, E e, hipEvent_t t0 = nullptr,
                     hipEvent_t t1 = nullptr) {
     if (M.nblocks == 0) {
-      launch_pass<0>(M, g, e, nullptr, stream, t0, t1);  // records the events only
+      if (t0) {
+        PDFP_HIP(hipEventRecord(t0, stream));
+        PDFP_HIP(hipEventRecord(t1, stream));
+      }
       return;
     }
     if (M.npb == 512) launch_pass_npb<512, 0, 0>(M, g, e, nullptr, stream, t0, t1);

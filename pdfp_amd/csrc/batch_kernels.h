@@ -53,14 +53,22 @@ __device__ __forceinline__ void store8(float* p, const F8& f) {
   reinterpret_cast<float4*>(p)[1] = make_float4(f.v[4], f.v[5], f.v[6], f.v[7]);
 }
 
-// p_k = a · x[c·8 + k]  for the 8 trajectories (one 32-byte gather)
+// p_k = a · x[c·8 + k]  for the 8 trajectories (one 32-byte gather); the
+// pass stages the 8 FP32 values (raw) rather than 8 FP64 products.
 struct GatherBatch {
   static constexpr int NV = kBatch;
+  static constexpr bool kRawStage = true;
   const float* __restrict__ x;
   __device__ __forceinline__ void operator()(int c, double a, double* p) const {
     const F8 f = load8(x + (size_t)c * kBatch);
 #pragma unroll
     for (int q = 0; q < kBatch; ++q) p[q] = a * (double)f.v[q];
+  }
+  __device__ __forceinline__ void raw(int c, float* v) const {
+    const float4* src = reinterpret_cast<const float4*>(x + (size_t)c * kBatch);
+    float4* dst = reinterpret_cast<float4*>(v);
+    dst[0] = src[0];
+    dst[1] = src[1];
   }
 };
 
@@ -68,8 +76,11 @@ struct GatherBatch {
 // (src/pdfp.jl:119-143), s[k] = (Aᵀ z_k)_j.
 template <bool MAJOR>
 struct EpiBatchPrimalY {
+  // x and y load at the start of the pass; x⁰, y⁰ and x̃ load in the
+  // epilogue (prefetching them too costs registers, hence occupancy, and
+  // measured slower).
   struct Pre {
-    F8 x, y, x0, y0, xt;
+    F8 x, y;
     float lb, ub, w;
   };
   const BatchStep* sp;
@@ -89,12 +100,12 @@ struct EpiBatchPrimalY {
   float* __restrict__ ry;
   __device__ __forceinline__ Pre pre(int j) const {
     const size_t o = (size_t)j * kBatch;
-    return {load8(x + o), load8(y + o), load8(x0 + o), load8(y0 + o), load8(xt + o),
-            lb[j], ub[j], w[j]};
+    return {load8(x + o), load8(y + o), lb[j], ub[j], w[j]};
   }
   __device__ __forceinline__ void operator()(int j, const Pre& p, const double* s,
                                              double*) const {
     const size_t o = (size_t)j * kBatch;
+    const F8 px0 = load8(x0 + o), py0 = load8(y0 + o), pxt = load8(xt + o);
     F8 nx = p.x, ny = p.y, nr, npx, nds, npy, nry;
     const double lo = (double)p.lb, hi = (double)p.ub, wj = (double)p.w;
 #pragma unroll
@@ -105,13 +116,13 @@ struct EpiBatchPrimalY {
       const double temp = cx - c.tau * (s[q] - cy);
       const double pxq = jclamp(temp, lo, hi);
       const double rxq = 2.0 * pxq - cx;
-      const double pyq = wj > 0.0 ? jclamp(cy + c.sigma * ((double)p.xt.v[q] - rxq), -wj, wj) : 0.0;
+      const double pyq = wj > 0.0 ? jclamp(cy + c.sigma * ((double)pxt.v[q] - rxq), -wj, wj) : 0.0;
       const double ryq = 2.0 * pyq - cy;
       const bool on = st.active != 0.0;
       nr.v[q] = on ? (float)rxq : p.x.v[q];
       if (on) {
-        nx.v[q] = (float)(c.weight * (c.omega * rxq + c.omw * cx) + c.omwt * (double)p.x0.v[q]);
-        ny.v[q] = (float)(c.weight * (c.omega * ryq + c.omw * cy) + c.omwt * (double)p.y0.v[q]);
+        nx.v[q] = (float)(c.weight * (c.omega * rxq + c.omw * cx) + c.omwt * (double)px0.v[q]);
+        ny.v[q] = (float)(c.weight * (c.omega * ryq + c.omw * cy) + c.omwt * (double)py0.v[q]);
       }
       npx.v[q] = (float)pxq;
       nds.v[q] = (float)((pxq - temp) / c.tau);

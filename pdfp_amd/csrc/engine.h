@@ -13,7 +13,10 @@
 //
 //   stream mode  (tpr > 0): the block's contiguous slice of (col, val) is
 //     streamed once, coalesced; the products a_ij·x_j are formed in FP64 and
-//     staged in LDS together with the row offsets.  Groups of tpr lanes then
+//     staged in LDS together with the row offsets (a gather with many values
+//     per nonzero stages the FP32 values and a_ij instead, 4 bytes each, and
+//     the reduction forms the same FP64 products: less LDS per workgroup,
+//     more workgroups per CU).  Groups of tpr lanes then
 //     reduce the rows round-robin (xor-shuffles inside a group) into an LDS
 //     row-sum array, and finally lane t runs the epilogue of row t: the
 //     epilogue's operands were loaded into registers at the start of the
@@ -33,6 +36,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace pdfp {
 
@@ -159,6 +164,15 @@ struct GatherDelta {
   }
 };
 
+// A gather that sets `static constexpr bool kRawStage = true` also provides
+// raw(c, float* v), the NV gathered FP32 values of column c (v is 16-byte
+// aligned); the pass then stages them with the FP32 matrix value and forms
+// the products a·v[q] in FP64 during the row reduction.
+template <class G, class = void>
+struct raw_stage : std::false_type {};
+template <class G>
+struct raw_stage<G, decltype((void)G::kRawStage)> : std::integral_constant<bool, G::kRawStage> {};
+
 // ---- epilogue interface ------------------------------------------------------
 // struct Epi {
 //   struct Pre;                                        // per-row operands
@@ -178,7 +192,15 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   constexpr int NV = Gather::NV;
   constexpr int U = (NPB / BLOCK) < 4 ? (NPB / BLOCK > 0 ? NPB / BLOCK : 1) : 4;
   constexpr int NA = NACC > 0 ? NACC : 1;
-  __shared__ double stage[NV][NPB];
+  constexpr bool kRaw = raw_stage<Gather>::value;
+  static_assert(!kRaw || (std::is_same<TV, float>::value && NV % 4 == 0),
+                "raw staging: FP32 matrix values, a multiple of 4 gathered values");
+  // products: double[NV][NPB]; raw: float[NPB][NV] values then float[NPB] a_ij
+  constexpr int kStageBytes = kRaw ? NPB * (NV + 1) * 4 : NV * NPB * 8;
+  __shared__ __align__(16) unsigned char stage_mem[kStageBytes];
+  double(*stage)[NPB] = reinterpret_cast<double(*)[NPB]>(stage_mem);
+  float* const sraw = reinterpret_cast<float*>(stage_mem);
+  float* const sa = sraw + NPB * NV;
   __shared__ double rowsum[NV][BLOCK];
   __shared__ int off[BLOCK + 1];
   __shared__ double red[BLOCK / kWave];
@@ -215,10 +237,15 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
       for (int u = 0; u < U; ++u) {
         const int k = k0 + u * BLOCK + tid;
         if (k < cnt) {
-          double p[NV];
-          g(cc[u], vv[u], p);
+          if constexpr (kRaw) {
+            g.raw(cc[u], sraw + k * NV);
+            sa[k] = (float)vv[u];
+          } else {
+            double p[NV];
+            g(cc[u], vv[u], p);
 #pragma unroll
-          for (int q = 0; q < NV; ++q) stage[q][k] = p[q];
+            for (int q = 0; q < NV; ++q) stage[q][k] = p[q];
+          }
         }
       }
     }
@@ -233,9 +260,23 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
 #pragma unroll
       for (int q = 0; q < NV; ++q) s[q] = 0.0;
       const int k1 = off[lr + 1];
+#pragma unroll 1
       for (int k = off[lr] + lane; k < k1; k += tpr) {
+        if constexpr (kRaw) {
+          const double a = (double)sa[k];
+          const float4* v4 = reinterpret_cast<const float4*>(sraw + k * NV);
 #pragma unroll
-        for (int q = 0; q < NV; ++q) s[q] += stage[q][k];
+          for (int q4 = 0; q4 < NV / 4; ++q4) {
+            const float4 v = v4[q4];
+            s[4 * q4 + 0] += a * (double)v.x;
+            s[4 * q4 + 1] += a * (double)v.y;
+            s[4 * q4 + 2] += a * (double)v.z;
+            s[4 * q4 + 3] += a * (double)v.w;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < NV; ++q) s[q] += stage[q][k];
+        }
       }
       for (int o = tpr >> 1; o > 0; o >>= 1) {
 #pragma unroll
