@@ -1,13 +1,16 @@
 """Benchmark of the PDHG projector on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c5]
 
 Workload (default, "c2"): the mixed MILP with 50,000 rows × 100,000 columns
 (70 % binary, 30 % continuous in [0, 10]), 1,000,000 nonzeros, row lengths
 uniform in 10-30, values N(0,1), b = A x0 + |N(0,1)| for a random feasible
-x0 (seed 1).  A step is one warm-started ℓ1 projection toward a new random
-0/1 target at relative KKT tolerance 1e-4, the way the pump calls the
-projector (src/pump.jl:329-359).  The scaled matrix, its transpose and every
+x0 (seed 1).  ``--workload c3``: 500,000 × 1,000,000 with exactly 5,000,000
+nonzeros in power-law rows; ``c5``: a general-integer, equality-heavy MILP of
+200,000 × 400,000 with 4M nonzeros.  A step is one warm-started ℓ1
+projection toward a new random integer target (0/1 on binaries) at relative
+KKT tolerance 1e-4, the way the pump calls the projector
+(src/pump.jl:329-359).  The scaled matrix, its transpose and every
 iterate stay in HBM; the timed region covers set_target + pdfp_solve.
 
 value = Σ PDHG iterations / Σ projection wall time of the batched product
@@ -57,6 +60,10 @@ def workload(name: str):
         m = G.mixed(500_000, 1_000_000, 1, power_law=True, nnz=5_000_000, max_len=5000)
         desc = ("mixed MILP 500000 rows x 1000000 cols, 5M nnz, power-law row lengths "
                 "(Zipf 1.5, max 5000), N(0,1) values, seed 1")
+    elif name == "c5":
+        m = G.general_integer(200_000, 400_000, 1)
+        desc = ("general-integer MILP 200000 rows (40% equality) x 400000 integer cols in "
+                "[0, U_j], U_j in 1..20, 4M nnz of U{-9..9}, rows of 10-30, seed 1")
     else:
         raise SystemExit(f"unknown workload {name}")
     return m, desc
@@ -341,6 +348,14 @@ def main():
         (ba, head["ms_a"], "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, dom_key)
+    # The random-gather floor of each pass: nnz gathers of one record per
+    # column (Aᵀz gathers z: m records; A x̄ gathers x̄: n records).
+    from pdfp_amd._lib import gather_ceiling_us
+
+    rec = 32 if K > 1 else 4
+    ceil = {"transpose_pass": gather_ceiling_us(max(mm, 1) * rec, rec, nnz),
+            "row_pass": gather_ceiling_us(max(nn, 1) * rec, rec, nnz)}
+    dom_ceil = ceil["transpose_pass" if dom_key == keys[0] else "row_pass"]
     out = {
         "metric": METRIC,
         "value": value,
@@ -360,9 +375,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": dom_bytes,
-                     "kernel_ms": dom_ms},
+                     "kernel_ms": dom_ms, "gather_ceiling_us": dom_ceil,
+                     "frac_of_gather_ceiling": dom_ceil / (dom_ms * 1e3)},
         "iteration_us": head["ms_iter"] * 1e3,
         "kernel_us": {"transpose_pass": head["ms_at"] * 1e3, "row_pass": head["ms_a"] * 1e3},
+        "gather_ceiling_us": ceil,
         "fp32_single_value": r32["iters"] / r32["seconds"],
         "fp32_single_iteration_us": r32["ms_iter"] * 1e3,
         "fp32_single_kernel_us": {"transpose_pass": r32["ms_at"] * 1e3,

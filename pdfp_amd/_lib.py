@@ -131,6 +131,8 @@ SIGNATURES = {
     "pdfp_batch_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_batch_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_batch_destroy": (None, [_P]),
+    "pdfp_gather_ceiling": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, _D]),
+    "pdfp_probe_last_error": (C.c_char_p, []),
 }
 
 _lib = None
@@ -171,6 +173,19 @@ def f64(a) -> np.ndarray:
 
 class LibraryError(RuntimeError):
     pass
+
+
+def gather_ceiling_us(table_bytes: int, record_bytes: int, n_gathers: int,
+                      device: int = 0) -> float:
+    """Microseconds per pass of n_gathers random gathers of record_bytes-byte
+    records from a table_bytes-byte device table (pdfp_gather_ceiling)."""
+    lib = load()
+    us = C.c_double()
+    st = lib.pdfp_gather_ceiling(int(table_bytes), int(record_bytes), int(n_gathers), device,
+                                 C.byref(us))
+    if st != 0:
+        raise LibraryError(f"libpdfp_hip error {st}: {lib.pdfp_probe_last_error().decode()}")
+    return us.value
 
 
 def check(status: int, err_fn, ctx) -> None:
