@@ -108,6 +108,8 @@ typedef struct pdfp_info {
   int32_t npb_at;            /*   for the A pass and for the Aᵀ pass            */
   int32_t persist_a;         /* 0: one workgroup per row block; k: a persistent */
   int32_t persist_at;        /*   grid of k workgroups per CU (autotuned)       */
+  int32_t small_path;        /* 1: the last solve ran each block of iterations  */
+                             /*   in one workgroup (operators held in LDS)      */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */
@@ -285,6 +287,13 @@ int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at,
 
 /* Device memory held by the context, in bytes. */
 int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
+
+/* Kernel path of the FP32 form (no counterpart in the reference): 0 (the
+ * default) runs each block of iterations in one workgroup when both
+ * operators fit in LDS (4(m+n+2) + 16 nnz + 4(m+n) bytes ≤ 62 KiB), else the
+ * general passes; 1 always runs the general passes.  The FP64 form ignores
+ * it. */
+int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);
 
 /* ---- measurement: the random-gather ceiling ------------------------------
  * No counterpart in the reference.  Mean microseconds of one pass of

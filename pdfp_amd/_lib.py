@@ -62,6 +62,7 @@ class PdfpInfo(C.Structure):
         ("npb_at", C.c_int32),
         ("persist_a", C.c_int32),
         ("persist_at", C.c_int32),
+        ("small_path", C.c_int32),
     ]
 
 
@@ -115,6 +116,7 @@ SIGNATURES = {
     "pdfp_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_device_bytes": (C.c_int64, [_P]),
+    "pdfp_set_kernel_path": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_last_error": (C.c_char_p, [_P]),
     "pdfp_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int64, C.c_int64,
                                     _I64, _I64, _D, C.c_int32, _D, _U8, _U8, _D, _D, _D,

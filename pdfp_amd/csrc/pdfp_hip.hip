@@ -22,6 +22,7 @@
 #include "engine.h"
 #include "host_util.h"
 #include "pdfp_kernels.h"
+#include "small_kernels.h"
 #include "solver_common.h"
 
 namespace pdfp {
@@ -43,6 +44,7 @@ struct ProjectorBase {
   virtual double time_iterations(int iters) = 0;
   virtual void time_kernels(int iters, double* ms_at, double* ms_a) = 0;
   virtual int64_t device_bytes() const = 0;
+  virtual void set_path(int path) = 0;
 };
 
 template <typename T>
@@ -75,6 +77,11 @@ struct Projector final : ProjectorBase {
   double relative_l1_gap = INFINITY, dual_den = 1.0, l1_den = 1.0;
   double l1_distance = NAN;
   int termination = PDFP_UNSPECIFIED;
+  // FP32 form: whole blocks in one workgroup when the operators fit in LDS
+  // (small_kernels.h); path 1 forces the general passes.
+  bool small_ok = false;
+  int path = 0;
+  bool small_used = false;
   int n_restarts = 0, n_evals = 0;
   double solve_seconds = 0.0;
   bool x64_valid = false;
@@ -91,9 +98,17 @@ struct Projector final : ProjectorBase {
     nnz = (int64_t)ha.col.size();
     A.upload(ha, stream);
     At.upload(ht, stream);
-    // pick the nonzeros-per-block of each operator's schedule by timing
-    autotune_schedule(A, n, stream);
-    autotune_schedule(At, m, stream);
+    small_ok = kFused && m > 0 && n > 0 && nnz > 0 &&
+               small_lds_bytes(m, n, nnz) <= kSmallLdsMax - 2048;
+    if (small_ok) {
+      // the general passes only run when forced: untuned schedules
+      A.schedule(512, stream);
+      At.schedule(512, stream);
+    } else {
+      // pick the nonzeros-per-block of each operator's schedule by timing
+      autotune_schedule(A, n, stream);
+      autotune_schedule(At, m, stream);
+    }
 
     h_w.assign((size_t)n, 0.0);
     for (int64_t j = 0; j < n; ++j) h_w[(size_t)j] = is_int_[j] ? w_[j] : 0.0;
@@ -139,8 +154,6 @@ struct Projector final : ProjectorBase {
       vec64(xt64, zeros.data());
       vec64(x64, zeros.data());
     }
-    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
-    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
     part_v.alloc((size_t)std::max<int64_t>(grid_for(std::max(m, n)), 1) * 2 + 8);
     sums.alloc(64);
     d_sp.alloc(1);
@@ -149,6 +162,61 @@ struct Projector final : ProjectorBase {
     // terms of an empty operator are never written: start every slot at 0
     std::memset(h_sums.p, 0, sizeof(double) * 64);
     sums.upload(h_sums.p, 64, stream);
+    if constexpr (kFused)
+      if (!small_ok) tune_fused();
+    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
+    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
+    PDFP_HIP(hipStreamSynchronize(stream));
+  }
+
+  // FP32 form: refine each operator's schedule (nonzeros per block, grid
+  // form) by timing the fused iteration itself, whose epilogue moves the
+  // optimum away from the plain pass's; coordinate-wise, Aᵀ first.  The
+  // iterates are zeroed afterwards.
+  void tune_fused() {
+    if (nnz == 0) return;
+    h_sp.p[0] = StepParams{1e-3, 1.0, 1.0, 0.0};
+    PDFP_HIP(hipMemcpyAsync(d_sp.p, h_sp.p, sizeof(StepParams), hipMemcpyHostToDevice, stream));
+    hipEvent_t e0, e1;
+    PDFP_HIP(hipEventCreate(&e0));
+    PDFP_HIP(hipEventCreate(&e1));
+    auto time_steps = [&] {
+      float best = 1e30f;
+      for (int rep = 0; rep < 3; ++rep) {
+        step(2, false);
+        PDFP_HIP(hipEventRecord(e0, stream));
+        for (int i = 0; i < 6; ++i) step(2 + i, false);
+        PDFP_HIP(hipEventRecord(e1, stream));
+        PDFP_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+      }
+      return best;
+    };
+    constexpr int kPersist[] = {0, 2, 4, 8};
+    for (DevCsr<T>* M : {&At, &A}) {
+      int bn = M->npb, bp = M->persist;
+      float best = time_steps();
+      for (int npb : kNpbChoices) {
+        M->schedule(npb, stream);
+        for (int persist : kPersist) {
+          M->persist = persist;
+          const float t = time_steps();
+          if (t < best) {
+            best = t;
+            bn = npb;
+            bp = persist;
+          }
+        }
+      }
+      M->schedule(bn, stream);
+      M->persist = bp;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (DevBuf<T>* v : {&x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry, &z, &z0, &pz, &rz})
+      PDFP_HIP(hipMemsetAsync(v->p, 0, v->bytes(), stream));
     PDFP_HIP(hipStreamSynchronize(stream));
   }
 
@@ -269,6 +337,39 @@ struct Projector final : ProjectorBase {
     }
   }
 
+  // One block of `freq` iterations and its evaluation(s) in one workgroup.
+  void small_block(int freq, bool first_major) {
+    if constexpr (kFused) {
+      SmallArgs a;
+      a.A = SmallCsr{A.rowptr.p, A.col.p, A.val.p};
+      a.At = SmallCsr{At.rowptr.p, At.col.p, At.val.p};
+      a.m = (int)m;
+      a.n = (int)n;
+      a.nnz = (int)nnz;
+      a.freq = freq;
+      a.first_major = first_major ? 1 : 0;
+      a.ex = EpiPrimalY<float, false>{d_sp.p, 0, x.p, x0.p, y.p, y0.p, lb.p, ub.p, xt.p, w.p,
+                                      rx.p, px.p, ds.p, py.p, ry.p};
+      a.ez = EpiDualZ<float, false>{d_sp.p, 0, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+      a.ec = EpiEvalCol<float>{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p};
+      a.er = EpiEvalRow<float>{pz.p, rz.p, b.p, z0.p, is_eq.p,
+                               orig_primal ? row_unscale.p : nullptr};
+      a.z = z.p;
+      a.rx = rx.p;
+      a.pz = pz.p;
+      a.rz = rz.p;
+      a.px = px.p;
+      a.sums = sums.p;
+      k_small_block<kSmallBlock><<<1, kSmallBlock, (size_t)small_lds_bytes(m, n, nnz), stream>>>(a);
+      PDFP_HIP(hipGetLastError());
+    }
+  }
+
+  void set_path(int p) override {
+    if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
+    path = p;
+  }
+
   // Evaluation: FP64 sums into sums[slot*16 ...]; see EpiEvalCol/EpiEvalRow.
   void evaluate(int slot) {
     EpiEvalCol<T> ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p};
@@ -359,13 +460,18 @@ struct Projector final : ProjectorBase {
     n_restarts = 0;
     n_evals = 0;
     x64_valid = false;
+    small_used = small_ok && path == 0;
     while (r.total_count < p.iteration_limit) {
       upload_step_params(omega);
       const bool restart_eval = do_restart;
-      step(1, restart_eval || freq == 1);
-      if (restart_eval) evaluate(1);
-      for (int i = 2; i <= freq; ++i) step(i, i == freq);
-      evaluate(0);
+      if (small_used) {
+        small_block(freq, restart_eval);
+      } else {
+        step(1, restart_eval || freq == 1);
+        if (restart_eval) evaluate(1);
+        for (int i = 2; i <= freq; ++i) step(i, i == freq);
+        evaluate(0);
+      }
       PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 32, hipMemcpyDeviceToHost,
                               stream));
       sync();
@@ -471,6 +577,7 @@ struct Projector final : ProjectorBase {
     o->npb_at = At.npb;
     o->persist_a = A.persist;
     o->persist_at = At.persist;
+    o->small_path = small_used ? 1 : 0;
   }
 
   void spmv(int transpose, const double* xin, double* yout) override {
@@ -761,6 +868,10 @@ int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at, double* 
 
 int64_t pdfp_device_bytes(const pdfp_ctx* ctx) {
   return ctx && ctx->impl ? ctx->impl->device_bytes() : 0;
+}
+
+int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path) {
+  return guarded(ctx, [&] { ctx->impl->set_path(path); });
 }
 
 }  // extern "C"

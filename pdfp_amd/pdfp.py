@@ -186,6 +186,15 @@ class PDFPState:
     def device_bytes(self) -> int:
         return int(self._lib.pdfp_device_bytes(self._ctx))
 
+    def set_kernel_path(self, path: str) -> None:
+        """FP32 form: "auto" runs each block of iterations in one workgroup
+        when the operators fit in LDS (the default); "general" always runs
+        the general passes."""
+        codes = {"auto": 0, "general": 1}
+        if path not in codes:
+            raise ValueError("kernel path must be 'auto' or 'general'")
+        self._check(self._lib.pdfp_set_kernel_path(self._ctx, codes[path]))
+
 
 def original_primal_residual(s: PDFPState, D_r) -> PDFPState:
     """src/pdfp.jl:85-90."""

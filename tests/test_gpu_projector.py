@@ -139,15 +139,26 @@ def test_fp64_l1_matches_highs(case):
     assert abs(l1_distance(milp, x, xt) - ref) <= 1e-6 * max(1.0, ref)
 
 
+@pytest.mark.parametrize("path", ["auto", "general"])
 @pytest.mark.parametrize("case", l1_fixtures(), ids=lambda c: c["name"])
-def test_fp32_l1_matches_highs_and_kkt(case):
-    say("fp32 highs", case["name"])
+def test_fp32_l1_matches_highs_and_kkt(case, path):
+    """Both kernel paths of the FP32 form: "auto" runs whole blocks in one
+    workgroup wherever the operators fit in LDS."""
+    say("fp32 highs", case["name"], path)
     milp = build_instance(case["spec"])
     xt = np.array(case["target"])
     tol = 1e-5
-    pp, s, status = project_with(P, P.PDFPState, milp, xt, tol=tol, precision="fp32",
-                                 iteration_limit=200_000)
+    pp = projector_problem(milp)
+    s = hip_state(pp, "fp32")
+    s.set_kernel_path(path)
+    P.set_target(s, xt / pp.D_c)
+    status = P.pdfp_solve(s, SolverParameters(eps_optimal_relative=tol,
+                                              eps_feasible_relative=tol,
+                                              iteration_limit=200_000), 1.0)
     assert status == "OPTIMAL"
+    m_, n_ = pp.A.shape
+    fits = 4 * (n_ + 1) + 4 * (m_ + 1) + 16 * pp.A.nnz + 4 * (m_ + n_) <= 62 * 1024
+    assert s.info().small_path == (1 if path == "auto" and fits and pp.A.nnz > 0 else 0)
     x = pp.D_c * s.pdhg_x
     ref = case["l1_optimum"]
     assert abs(l1_distance(milp, x, xt) - ref) <= 1e-4 * max(1.0, ref)
