@@ -289,8 +289,8 @@ int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at,
 int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
 
 /* Kernel path of the FP32 form (no counterpart in the reference): 0 (the
- * default) runs each block of iterations in one workgroup when both
- * operators fit in LDS (4(m+n+2) + 16 nnz + 4(m+n) bytes ≤ 62 KiB), else the
+ * default) runs each block of iterations in one workgroup when the instance
+ * fits in LDS (4(m+n+2) + 16 nnz + 36 n + 13 m bytes ≤ 62 KiB), else the
  * general passes; 1 always runs the general passes.  The FP64 form ignores
  * it. */
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);

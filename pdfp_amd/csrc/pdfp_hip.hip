@@ -80,6 +80,7 @@ struct Projector final : ProjectorBase {
   // FP32 form: whole blocks in one workgroup when the operators fit in LDS
   // (small_kernels.h); path 1 forces the general passes.
   bool small_ok = false;
+  int small_lanes_a = 1, small_lanes_t = 1;
   int path = 0;
   bool small_used = false;
   int n_restarts = 0, n_evals = 0;
@@ -99,8 +100,10 @@ struct Projector final : ProjectorBase {
     A.upload(ha, stream);
     At.upload(ht, stream);
     small_ok = kFused && m > 0 && n > 0 && nnz > 0 &&
-               small_lds_bytes(m, n, nnz) <= kSmallLdsMax - 2048;
+               small_lds_bytes(m, n, nnz) <= kSmallLdsMax;
     if (small_ok) {
+      small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
+      small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);
       // the general passes only run when forced: untuned schedules
       A.schedule(512, stream);
       At.schedule(512, stream);
@@ -346,19 +349,33 @@ struct Projector final : ProjectorBase {
       a.m = (int)m;
       a.n = (int)n;
       a.nnz = (int)nnz;
+      a.lanes_a = small_lanes_a;
+      a.lanes_t = small_lanes_t;
       a.freq = freq;
       a.first_major = first_major ? 1 : 0;
-      a.ex = EpiPrimalY<float, false>{d_sp.p, 0, x.p, x0.p, y.p, y0.p, lb.p, ub.p, xt.p, w.p,
-                                      rx.p, px.p, ds.p, py.p, ry.p};
-      a.ez = EpiDualZ<float, false>{d_sp.p, 0, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+      a.sp = d_sp.p;
+      a.x = x.p;
+      a.y = y.p;
+      a.z = z.p;
+      a.x0 = x0.p;
+      a.y0 = y0.p;
+      a.z0 = z0.p;
+      a.lb = lb.p;
+      a.ub = ub.p;
+      a.xt = xt.p;
+      a.w = w.p;
+      a.b = b.p;
+      a.is_eq = is_eq.p;
+      a.rx = rx.p;
+      a.px = px.p;
+      a.ds = ds.p;
+      a.py = py.p;
+      a.ry = ry.p;
+      a.pz = pz.p;
+      a.rz = rz.p;
       a.ec = EpiEvalCol<float>{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p};
       a.er = EpiEvalRow<float>{pz.p, rz.p, b.p, z0.p, is_eq.p,
                                orig_primal ? row_unscale.p : nullptr};
-      a.z = z.p;
-      a.rx = rx.p;
-      a.pz = pz.p;
-      a.rz = rz.p;
-      a.px = px.p;
       a.sums = sums.p;
       k_small_block<kSmallBlock><<<1, kSmallBlock, (size_t)small_lds_bytes(m, n, nnz), stream>>>(a);
       PDFP_HIP(hipGetLastError());

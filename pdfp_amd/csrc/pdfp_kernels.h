@@ -52,6 +52,46 @@ __device__ __forceinline__ double jclamp(double x, double lo, double hi) {
   return x > hi ? hi : (x < lo ? lo : x);
 }
 
+// ---- the per-element math of one PDHG iteration (FP64) ---------------------
+// Shared by the fused epilogues below and by the one-workgroup block kernel
+// (small_kernels.h), so that both run the same operations.
+
+struct PrimalYOut {
+  double x, y, rx, px, ds, py, ry;
+};
+
+// Column j: K1 (primal step, pdfp_kernels.jl:4-20) and K2 (dual-y step,
+// pdfp_kernels.jl:33-48) given s = (Aᵀz)_j.
+__device__ __forceinline__ PrimalYOut primal_y_math(const StepScalars& c, double s, double cx,
+                                                   double cy, double x0, double y0, double lb,
+                                                   double ub, double xt, double wj) {
+  PrimalYOut o;
+  const double temp = cx - c.tau * (s - cy);
+  o.px = jclamp(temp, lb, ub);
+  o.rx = 2.0 * o.px - cx;
+  o.x = c.weight * (c.omega * o.rx + c.omw * cx) + c.omwt * x0;
+  o.py = wj > 0.0 ? jclamp(cy + c.sigma * (xt - o.rx), -wj, wj) : 0.0;
+  o.ry = 2.0 * o.py - cy;
+  o.y = c.weight * (c.omega * o.ry + c.omw * cy) + c.omwt * y0;
+  o.ds = (o.px - temp) / c.tau;
+  return o;
+}
+
+struct DualZOut {
+  double z, pz, rz;
+};
+
+// Row i: K3 (dual-z step, pdfp_kernels.jl:60-75) given s = (A x̄)_i.
+__device__ __forceinline__ DualZOut dual_z_math(const StepScalars& c, double s, double cz,
+                                               double z0, double b, bool eq) {
+  DualZOut o;
+  const double raw = cz + c.sigma * (s - b);
+  o.pz = eq ? raw : (raw > 0.0 ? raw : 0.0);
+  o.rz = 2.0 * o.pz - cz;
+  o.z = c.weight * (c.omega * o.rz + c.omw * cz) + c.omwt * z0;
+  return o;
+}
+
 // ---- fused epilogues (precision-generic iterates T, FP64 arithmetic) ------
 // Each epilogue loads its per-row operands in pre() (issued by the engine at
 // the start of the pass) and does the math and the stores in operator().
@@ -86,22 +126,17 @@ struct EpiPrimalY {
   __device__ __forceinline__ void operator()(int j, const Pre& p, const double* s,
                                              double*) const {
     const StepScalars c = step_scalars(sp, k_offset);
-    const double cx = (double)p.x;
-    const double cy = (double)p.y;
-    const double temp = cx - c.tau * (s[0] - cy);
-    const double pxj = jclamp(temp, (double)p.lb, (double)p.ub);
-    const double rxj = 2.0 * pxj - cx;
-    x[j] = (T)(c.weight * (c.omega * rxj + c.omw * cx) + c.omwt * (double)p.x0);
-    rx[j] = (T)rxj;
-    const double wj = (double)p.w;
-    const double pyj = wj > 0.0 ? jclamp(cy + c.sigma * ((double)p.xt - rxj), -wj, wj) : 0.0;
-    const double ryj = 2.0 * pyj - cy;
-    y[j] = (T)(c.weight * (c.omega * ryj + c.omw * cy) + c.omwt * (double)p.y0);
+    const PrimalYOut o = primal_y_math(c, s[0], (double)p.x, (double)p.y, (double)p.x0,
+                                       (double)p.y0, (double)p.lb, (double)p.ub, (double)p.xt,
+                                       (double)p.w);
+    x[j] = (T)o.x;
+    rx[j] = (T)o.rx;
+    y[j] = (T)o.y;
     if (MAJOR) {
-      px[j] = (T)pxj;
-      ds[j] = (T)((pxj - temp) / c.tau);
-      py[j] = (T)pyj;
-      ry[j] = (T)ryj;
+      px[j] = (T)o.px;
+      ds[j] = (T)o.ds;
+      py[j] = (T)o.py;
+      ry[j] = (T)o.ry;
     }
   }
 };
@@ -125,14 +160,11 @@ struct EpiDualZ {
   __device__ __forceinline__ void operator()(int i, const Pre& p, const double* s,
                                              double*) const {
     const StepScalars c = step_scalars(sp, k_offset);
-    const double cz = (double)p.z;
-    const double raw = cz + c.sigma * (s[0] - (double)p.b);
-    const double pzi = p.eq ? raw : (raw > 0.0 ? raw : 0.0);
-    const double rzi = 2.0 * pzi - cz;
-    z[i] = (T)(c.weight * (c.omega * rzi + c.omw * cz) + c.omwt * (double)p.z0);
+    const DualZOut o = dual_z_math(c, s[0], (double)p.z, (double)p.z0, (double)p.b, p.eq != 0);
+    z[i] = (T)o.z;
     if (MAJOR) {
-      pz[i] = (T)pzi;
-      rz[i] = (T)rzi;
+      pz[i] = (T)o.pz;
+      rz[i] = (T)o.rz;
     }
   }
 };

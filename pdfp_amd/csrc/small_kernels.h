@@ -4,29 +4,34 @@
 // nonzeros) make a PDHG iteration launch-bound: the reference issues five
 // launches per iteration (src/pdfp.jl:158-163), the general form here two,
 // each a few microseconds of launch and drain for well under a microsecond
-// of work.  When both CSR operators and the two gathered vectors fit in LDS,
-// one launch of one workgroup runs a whole block of `freq` iterations and
-// its evaluation (src/pdfp.jl:252-264):
+// of work.  When the instance fits in LDS, one launch of one workgroup runs
+// a whole block of `freq` iterations and its evaluation
+// (src/pdfp.jl:252-264):
 //
-//   LDS: CSR(Aᵀ) and CSR(A) (int32 pointers and columns, FP32 values), the
-//        current z and the reflected x̄ (FP32), i.e. 4(n+1) + 4(m+1) +
-//        16·nnz + 4m + 4n bytes.
-//   per iteration: thread-per-column pass s_j = Σ a·z (FP64, ascending
-//        column order) with the x/y epilogue, which writes x̄ to LDS; a
-//        barrier; thread-per-row pass s_i = Σ a·x̄ with the z epilogue,
-//        which writes z to LDS; a barrier.
+//   LDS: CSR(Aᵀ) and CSR(A) (int32 pointers and columns, FP32 values) and
+//        every iterate and problem vector the iteration touches (FP32): z,
+//        z⁰, b, is_eq per row; x, y, x⁰, y⁰, ℓ, u, x̃, w, x̄ per column.
+//        See small_lds_bytes.
+//   per iteration: a pass over the columns, s_j = Σ a·z (FP64), and the x /
+//        y steps; a barrier; a pass over the rows, s_i = Σ a·x̄, and the z
+//        step; a barrier.  Each pass gives every row (of Aᵀ or A) a group
+//        of L lanes (host-chosen per operator from its row count and its
+//        longest row: the dependent LDS loads of one long row, not the
+//        bandwidth, bound an iteration here); the lanes stride the row and
+//        combine by xor-shuffles, and the group's first lane runs the
+//        per-element math, which is the general epilogues'
+//        (primal_y_math, dual_z_math).
 //   major iterations (the first after a restart and the last) also write
-//        pdhg x, y, z, reflected y, z, the dual slack and x̄ to HBM, which
-//        the evaluation and the restart read.
-//   evaluation: the same FP64 terms as the general passes' evaluation
-//        epilogues, reduced in a fixed order by the workgroup into the
-//        16-double slot.
-//
-// The other iterates (x, y and the anchors) stay in HBM: each thread reads
-// and writes only its own columns and rows, so the L1 serves them.  The
-// epilogues are the general passes' (pdfp_kernels.h), only their x̄ and z
-// pointers point into LDS.
+//        pdhg x, y, z, reflected x, y, z and the dual slack to HBM, which the
+//        evaluation and the restart read.
+//   evaluation: the general evaluation epilogues' FP64 terms, reduced in a
+//        fixed order by the workgroup into the 16-double slot.
+//   end: x, y and z go back to HBM.
 #pragma once
+
+#include <algorithm>
+#include <climits>
+#include <vector>
 
 #include "engine.h"
 #include "pdfp_kernels.h"
@@ -34,10 +39,31 @@
 namespace pdfp {
 
 constexpr int kSmallBlock = 1024;
-constexpr int64_t kSmallLdsMax = 64 * 1024;
+constexpr int64_t kSmallLdsMax = 62 * 1024;
 
 inline int64_t small_lds_bytes(int64_t m, int64_t n, int64_t nnz) {
-  return 4 * (n + 1) + 4 * (m + 1) + 16 * nnz + 4 * m + 4 * n;
+  return 4 * (n + 1) + 4 * (m + 1) + 16 * nnz + 36 * n + 13 * m;
+}
+
+// Lanes per row L ∈ {1, 2, ..., 64} of one operator in the one-workgroup
+// kernel: the fewest dependent steps, ⌈rows / (block / L)⌉ rounds of
+// ⌈longest / L⌉ loads and log₂ L shuffles each.
+inline int small_lanes(const std::vector<int32_t>& rowptr, int block) {
+  const int64_t rows = (int64_t)rowptr.size() - 1;
+  int64_t longest = 0;
+  for (int64_t r = 0; r < rows; ++r)
+    longest = std::max<int64_t>(longest, rowptr[(size_t)r + 1] - rowptr[(size_t)r]);
+  int best = 1;
+  int64_t best_cost = INT64_MAX;
+  for (int L = 1, lg = 0; L <= 64; L *= 2, ++lg) {
+    const int64_t groups = block / L;
+    const int64_t cost = ((rows + groups - 1) / groups) * ((longest + L - 1) / L + lg);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = L;
+    }
+  }
+  return best;
 }
 
 struct SmallCsr {
@@ -49,18 +75,18 @@ struct SmallCsr {
 struct SmallArgs {
   SmallCsr A, At;
   int m, n, nnz;
-  int freq;        // iterations in this launch
-  int first_major; // iteration 1 is a major step (and is evaluated into slot 1)
-  EpiPrimalY<float, false> ex;   // k_offset and the x̄ pointer are set in the kernel
-  EpiDualZ<float, false> ez;     // k_offset and the z pointer are set in the kernel
+  int lanes_a, lanes_t;  // lanes per row of A and of Aᵀ (powers of two ≤ 64)
+  int freq;         // iterations in this launch
+  int first_major;  // iteration 1 is a major step, evaluated into slot 1
+  const StepParams* sp;
+  // HBM vectors (FP32)
+  float *x, *y, *z;
+  const float *x0, *y0, *z0, *lb, *ub, *xt, *w, *b;
+  const uint8_t* is_eq;
+  float *rx, *px, *ds, *py, *ry, *pz, *rz;
   EpiEvalCol<float> ec;
   EpiEvalRow<float> er;
-  float* __restrict__ z;         // current z in HBM (read at start, written at end)
-  float* __restrict__ rx;        // reflected x in HBM (written on major steps)
-  const float* __restrict__ pz;  // pdhg z, reflected z, pdhg x (evaluation gathers)
-  const float* __restrict__ rz;
-  const float* __restrict__ px;
-  double* __restrict__ sums;     // slot 0 at sums[0..16), slot 1 at sums[16..32)
+  double* sums;  // slot 0 at sums[0..16), slot 1 at sums[16..32)
 };
 
 template <int BLOCK, int NACC>
@@ -83,6 +109,7 @@ __device__ __forceinline__ void small_reduce(double (&acc)[NACC], double* __rest
   __syncthreads();
 }
 
+// The evaluation of the last major step from the HBM copies it wrote.
 template <int BLOCK>
 __device__ __forceinline__ void small_evaluate(const SmallArgs& a, double* __restrict__ out,
                                                double* red) {
@@ -131,7 +158,18 @@ __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
   float* t_val = reinterpret_cast<float*>(a_col + nnz);
   float* a_val = t_val + nnz;
   float* s_z = a_val + nnz;
-  float* s_rx = s_z + m;
+  float* s_z0 = s_z + m;
+  float* s_b = s_z0 + m;
+  float* s_rx = s_b + m;
+  float* s_x = s_rx + n;
+  float* s_y = s_x + n;
+  float* s_x0 = s_y + n;
+  float* s_y0 = s_x0 + n;
+  float* s_lb = s_y0 + n;
+  float* s_ub = s_lb + n;
+  float* s_xt = s_ub + n;
+  float* s_w = s_xt + n;
+  uint8_t* s_eq = reinterpret_cast<uint8_t*>(s_w + n);
 
   for (int k = tid; k <= n; k += BLOCK) t_ptr[k] = a.At.ptr[k];
   for (int k = tid; k <= m; k += BLOCK) a_ptr[k] = a.A.ptr[k];
@@ -141,41 +179,72 @@ __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
     a_col[k] = a.A.col[k];
     a_val[k] = a.A.val[k];
   }
-  for (int i = tid; i < m; i += BLOCK) s_z[i] = a.z[i];
+  for (int i = tid; i < m; i += BLOCK) {
+    s_z[i] = a.z[i];
+    s_z0[i] = a.z0[i];
+    s_b[i] = a.b[i];
+    s_eq[i] = a.is_eq[i];
+  }
+  for (int j = tid; j < n; j += BLOCK) {
+    s_x[j] = a.x[j];
+    s_y[j] = a.y[j];
+    s_x0[j] = a.x0[j];
+    s_y0[j] = a.y0[j];
+    s_lb[j] = a.lb[j];
+    s_ub[j] = a.ub[j];
+    s_xt[j] = a.xt[j];
+    s_w[j] = a.w[j];
+  }
   __syncthreads();
 
   for (int it = 1; it <= a.freq; ++it) {
     const bool major = it == a.freq || (it == 1 && a.first_major);
-    // columns: s_j = (Aᵀ z)_j, then the x / y steps (x̄ into LDS)
-    for (int j = tid; j < n; j += BLOCK) {
+    const StepScalars c = step_scalars(a.sp, it);
+    // columns: s_j = (Aᵀ z)_j, then the x / y steps
+    const int Lt = a.lanes_t, gt = BLOCK / Lt, lt = tid & (Lt - 1);
+    for (int j0 = 0; j0 < n; j0 += gt) {
+      const int j = j0 + tid / Lt;
       double s = 0.0;
-      for (int k = t_ptr[j]; k < t_ptr[j + 1]; ++k) s += (double)t_val[k] * (double)s_z[t_col[k]];
+      if (j < n) {
+        const int k1 = t_ptr[j + 1];
+#pragma unroll 4
+        for (int k = t_ptr[j] + lt; k < k1; k += Lt) s += (double)t_val[k] * (double)s_z[t_col[k]];
+      }
+      for (int o = Lt >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, Lt);
+      if (j >= n || lt != 0) continue;
+      const PrimalYOut o = primal_y_math(c, s, (double)s_x[j], (double)s_y[j], (double)s_x0[j],
+                                         (double)s_y0[j], (double)s_lb[j], (double)s_ub[j],
+                                         (double)s_xt[j], (double)s_w[j]);
+      s_x[j] = (float)o.x;
+      s_y[j] = (float)o.y;
+      s_rx[j] = (float)o.rx;
       if (major) {
-        EpiPrimalY<float, true> e{a.ex.sp, it, a.ex.x, a.ex.x0, a.ex.y, a.ex.y0, a.ex.lb,
-                                  a.ex.ub, a.ex.xt, a.ex.w, s_rx, a.ex.px, a.ex.ds, a.ex.py,
-                                  a.ex.ry};
-        e(j, e.pre(j), &s, nullptr);
-        a.rx[j] = s_rx[j];
-      } else {
-        EpiPrimalY<float, false> e = a.ex;
-        e.k_offset = it;
-        e.rx = s_rx;
-        e(j, e.pre(j), &s, nullptr);
+        a.rx[j] = (float)o.rx;
+        a.px[j] = (float)o.px;
+        a.ds[j] = (float)o.ds;
+        a.py[j] = (float)o.py;
+        a.ry[j] = (float)o.ry;
       }
     }
     __syncthreads();
-    // rows: s_i = (A x̄)_i, then the z step (z into LDS)
-    for (int i = tid; i < m; i += BLOCK) {
+    // rows: s_i = (A x̄)_i, then the z step
+    const int La = a.lanes_a, ga = BLOCK / La, la = tid & (La - 1);
+    for (int i0 = 0; i0 < m; i0 += ga) {
+      const int i = i0 + tid / La;
       double s = 0.0;
-      for (int k = a_ptr[i]; k < a_ptr[i + 1]; ++k) s += (double)a_val[k] * (double)s_rx[a_col[k]];
+      if (i < m) {
+        const int k1 = a_ptr[i + 1];
+#pragma unroll 4
+        for (int k = a_ptr[i] + la; k < k1; k += La) s += (double)a_val[k] * (double)s_rx[a_col[k]];
+      }
+      for (int o = La >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, La);
+      if (i >= m || la != 0) continue;
+      const DualZOut o = dual_z_math(c, s, (double)s_z[i], (double)s_z0[i], (double)s_b[i],
+                                     s_eq[i] != 0);
+      s_z[i] = (float)o.z;
       if (major) {
-        EpiDualZ<float, true> e{a.ez.sp, it, s_z, a.ez.z0, a.ez.b, a.ez.is_eq, a.ez.pz, a.ez.rz};
-        e(i, e.pre(i), &s, nullptr);
-      } else {
-        EpiDualZ<float, false> e = a.ez;
-        e.k_offset = it;
-        e.z = s_z;
-        e(i, e.pre(i), &s, nullptr);
+        a.pz[i] = (float)o.pz;
+        a.rz[i] = (float)o.rz;
       }
     }
     __syncthreads();
@@ -187,6 +256,10 @@ __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
     if (tid < 16) a.sums[16 + tid] = a.sums[tid];
   }
   for (int i = tid; i < m; i += BLOCK) a.z[i] = s_z[i];
+  for (int j = tid; j < n; j += BLOCK) {
+    a.x[j] = s_x[j];
+    a.y[j] = s_y[j];
+  }
 }
 
 }  // namespace pdfp
