@@ -205,7 +205,8 @@ def time_to_first_feasible(runs: int, cpu: bool):
                          "rounding_s": r0.rounding_time}
         if cpu:
             proj, root_lp = _oracle_pump_backends()
-            r = feasibility_pump(milp, time_limit=120.0, projector=proj, root_lp=root_lp)
+            r = feasibility_pump(milp, time_limit=120.0, projector=proj, root_lp=root_lp,
+                                 scaling="host")
             rec["cpu_oracle"] = {"seconds": r.time, "found": bool(r.found), "cores": 1,
                                  "kind": "port"}
         out[name] = rec
@@ -271,11 +272,22 @@ def main():
 
     t0 = time.perf_counter()
     milp, desc = workload(args.workload)
-    pp = P.projector_problem(milp)
+    P.projector_problem(milp)                      # warm-up: first use of the device
+    t1 = time.perf_counter()
+    pp = P.projector_problem(milp)                 # the matrix equilibrated on the device
+    t2 = time.perf_counter()
+    pph = P.projector_problem(milp, scaling="host")
+    t3 = time.perf_counter()
+    if not (np.array_equal(pp.A.data, pph.A.data) and np.array_equal(pp.D_r, pph.D_r)
+            and np.array_equal(pp.D_c, pph.D_c)):
+        raise SystemExit("device and host scaling differ")
+    scaling_s = {"device": t2 - t1, "host_numpy": t3 - t2}
+    del pph
     mm, nn = milp.A.shape
     nnz = milp.A.nnz
     print(f"[bench] workload {args.workload}: {mm}x{nn}, nnz {nnz}, built in "
-          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+          f"{time.perf_counter() - t0:.1f}s; projector scaling {scaling_s['device']:.3f}s on the "
+          f"device, {scaling_s['host_numpy']:.3f}s on the host", file=sys.stderr, flush=True)
     targets_w = [G.binary_targets(milp, 1000 + k) for k in range(args.warmup)]
     targets = [G.binary_targets(milp, 2000 + k) for k in range(args.steps)]
 
@@ -385,6 +397,7 @@ def main():
         "fp32_single_kernel_us": {"transpose_pass": r32["ms_at"] * 1e3,
                                   "row_pass": r32["ms_a"] * 1e3},
         "schedule_nnz_per_block": r32["npb"],
+        "scaling_seconds": scaling_s,
     }
     if K > 1:
         out["batch_schedule_nnz_per_block"] = head["npb"]

@@ -295,6 +295,21 @@ int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
  * it. */
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);
 
+/* ---- preconditioning ------------------------------------------------------
+ * scale_problem (src/preconditioner.jl:26-63), the matrix part, on the
+ * device: `ruiz_iterations` Ruiz rounds and one Pock–Chambolle pass
+ * (pock_chambolle_alpha must be 1, the reference's value).  Input: the CSC of
+ * the m×n matrix (index_base 0 or 1).  Output: the scaled values in the same
+ * CSC order (scaled_nzval[nnz]) and the scalings D_r[m], D_c[n], bit-identical
+ * to the reference's CPU loop.  The caller forms c·D_c, ℓ/D_c, u/D_c and
+ * D_r·(row bounds) (src/preconditioner.jl:65-99).  Error text:
+ * pdfp_scale_last_error(). */
+int pdfp_scale_matrix(int64_t m, int64_t n, int64_t nnz, const int64_t* colptr,
+                      const int64_t* rowval, const double* nzval, int32_t index_base,
+                      int32_t ruiz_iterations, double pock_chambolle_alpha,
+                      double* scaled_nzval, double* D_r, double* D_c, int32_t device);
+const char* pdfp_scale_last_error(void);
+
 /* ---- measurement: the random-gather ceiling ------------------------------
  * No counterpart in the reference.  Mean microseconds of one pass of
  * n_gathers gathers of record_bytes-byte records (4, 8, 16, 32 or 64) at

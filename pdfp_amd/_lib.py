@@ -134,6 +134,9 @@ SIGNATURES = {
     "pdfp_batch_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_batch_destroy": (None, [_P]),
     "pdfp_gather_ceiling": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, _D]),
+    "pdfp_scale_matrix": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32,
+                                    C.c_int32, C.c_double, _D, _D, _D, C.c_int32]),
+    "pdfp_scale_last_error": (C.c_char_p, []),
     "pdfp_probe_last_error": (C.c_char_p, []),
 }
 

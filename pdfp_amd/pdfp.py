@@ -245,9 +245,10 @@ class ProjectorProblem:
     D_c: np.ndarray
 
 
-def projector_problem(milp) -> ProjectorProblem:
+def projector_problem(milp, *, scaling: str = "device", device: int = 0) -> ProjectorProblem:
     """Scale ``milp`` for the projector: no bound/objective rescaling, the
-    ℓ1 weights are D_c on I and 1 elsewhere (src/pump.jl:295-301)."""
+    ℓ1 weights are D_c on I and 1 elsewhere (src/pump.jl:295-301).
+    ``scaling``: where the matrix is equilibrated, "device" or "host"."""
     from .preconditioner import scale_problem
 
     n = len(milp.c)
@@ -257,7 +258,7 @@ def projector_problem(milp) -> ProjectorProblem:
     is_int[milp.int_idx] = True
     con_lb = np.where(is_eq, milp.b, -np.inf) if m else np.zeros(0)
     spr = scale_problem(milp.A, np.zeros(n), milp.l.copy(), milp.u.copy(), con_lb, milp.b.copy(),
-                        bound_objective_rescaling=False)
+                        bound_objective_rescaling=False, backend=scaling, device=device)
     w = np.ones(n)
     w[milp.int_idx] = spr.D_c[milp.int_idx]
     return ProjectorProblem(A=spr.A, b=spr.con_ub, is_eq=is_eq, is_int=is_int, var_lb=spr.var_lb,

@@ -95,12 +95,12 @@ def pdlp_solve(s: PDLPState, p: SolverParameters) -> str:
 
 
 def solve_lp_relaxation(milp: MILP, *, tolerance: float, time_limit: float,
-                        cupdlpx_termination: bool, device: int = 0):
+                        cupdlpx_termination: bool, device: int = 0, scaling: str = "device"):
     """src/pdlp.jl:381-393: the LP relaxation by PDLP.  Returns
     (x in original units, dual_slack in scaled units, status, iterations)."""
     con_lb = np.where(milp.row_sense == "le", -np.inf, milp.b)
     sp = scale_problem(milp.A, milp.c.copy(), milp.l.copy(), milp.u.copy(), con_lb,
-                       milp.b.copy())
+                       milp.b.copy(), backend=scaling, device=device)
     s = PDLPState(sp, device=device)
     if cupdlpx_termination:
         _cupdlpx_termination(s, sp.D_r, sp.D_c)

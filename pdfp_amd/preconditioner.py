@@ -9,8 +9,14 @@ rescaling.  With D_r, D_c the row and column scalings the scaled problem is
 
 and x = D_c .* x̂.  The arithmetic is max, sqrt, divide and multiply, in the
 reference's order, so the result is bit-identical to the reference's CPU
-code; the Pock–Chambolle sums accumulate in the CSC's column-major order
-(np.bincount adds sequentially in input order).
+code; the Pock–Chambolle sums accumulate in the CSC's column-major order.
+
+The matrix part (the Ruiz rounds and the Pock–Chambolle pass, ≈22 passes over
+the nonzeros) runs on the device by default (``pdfp_scale_matrix``, one
+launch per pass); ``backend="host"`` runs the same arithmetic in numpy
+(np.bincount adds sequentially in input order).  Both are bit-identical to
+the oracle.  The vector part (src/preconditioner.jl:65-99) is O(m + n) and
+runs on the host.
 """
 from __future__ import annotations
 
@@ -58,24 +64,15 @@ def _segment_max(vals: np.ndarray, indptr: np.ndarray, n: int) -> np.ndarray:
     return np.maximum(out, 0.0)
 
 
-def scale_problem(A: sp.csc_matrix, c, var_lb, var_ub, con_lb, con_ub, *,
-                  ruiz_iterations: int = 10, pock_chambolle_alpha: float = 1.0,
-                  bound_objective_rescaling: bool = True) -> ScaledProblem:
-    """src/preconditioner.jl:26-99."""
-    A = sp.csc_matrix(A, dtype=np.float64, copy=True)
-    A.sort_indices()
+def _scale_matrix_host(A: sp.csc_matrix, ruiz_iterations: int, alpha: float):
+    """src/preconditioner.jl:33-63 in numpy: returns (scaled values in CSC
+    order, D_r, D_c)."""
     m, n = A.shape
     rows = A.indices.astype(np.int64)
     cols = _col_of_entry(A)
-    vals = A.data
+    vals = A.data.copy()
     D_r = np.ones(m)
     D_c = np.ones(n)
-    c = np.asarray(c, dtype=np.float64).copy()
-    var_lb = np.asarray(var_lb, dtype=np.float64)
-    var_ub = np.asarray(var_ub, dtype=np.float64)
-    con_lb = np.asarray(con_lb, dtype=np.float64)
-    con_ub = np.asarray(con_ub, dtype=np.float64)
-
     # entries in row-major order, for the row maxima (max is order-free)
     perm = np.argsort(rows, kind="stable")
     row_ptr = np.zeros(m + 1, dtype=np.int64)
@@ -90,7 +87,6 @@ def scale_problem(A: sp.csc_matrix, c, var_lb, var_ub, con_lb, con_ub, *,
         D_r *= rscale
         D_c *= cscale
 
-    alpha = pock_chambolle_alpha
     av = np.abs(vals)
     # v^α with α == 1 is v itself; np.power(v, 1.0) returns v exactly.
     row_sum = np.bincount(rows, weights=np.power(av, alpha), minlength=m)
@@ -100,6 +96,50 @@ def scale_problem(A: sp.csc_matrix, c, var_lb, var_ub, con_lb, con_ub, *,
     vals *= rscale[rows] * cscale[cols]
     D_r *= rscale
     D_c *= cscale
+    return vals, D_r, D_c
+
+
+def _scale_matrix_device(A: sp.csc_matrix, ruiz_iterations: int, alpha: float, device: int):
+    """The matrix part on the device (pdfp_scale_matrix)."""
+    from . import _lib
+
+    lib = _lib.load()
+    m, n = A.shape
+    colptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    rowval = np.ascontiguousarray(A.indices, dtype=np.int64)
+    nzval = np.ascontiguousarray(A.data, dtype=np.float64)
+    vals = np.empty(len(nzval))
+    D_r = np.empty(m)
+    D_c = np.empty(n)
+    st = lib.pdfp_scale_matrix(m, n, len(nzval), _lib.i64ptr(colptr), _lib.i64ptr(rowval),
+                               _lib.dptr(nzval), 0, int(ruiz_iterations), float(alpha),
+                               _lib.dptr(vals), _lib.dptr(D_r), _lib.dptr(D_c), device)
+    if st != 0:
+        raise _lib.LibraryError(f"libpdfp_hip error {st}: "
+                                f"{lib.pdfp_scale_last_error().decode()}")
+    return vals, D_r, D_c
+
+
+def scale_problem(A: sp.csc_matrix, c, var_lb, var_ub, con_lb, con_ub, *,
+                  ruiz_iterations: int = 10, pock_chambolle_alpha: float = 1.0,
+                  bound_objective_rescaling: bool = True, backend: str = "device",
+                  device: int = 0) -> ScaledProblem:
+    """src/preconditioner.jl:26-99.  ``backend``: "device" (the matrix part
+    on the GPU) or "host"."""
+    A = sp.csc_matrix(A, dtype=np.float64, copy=True)
+    A.sort_indices()
+    m, n = A.shape
+    c = np.asarray(c, dtype=np.float64).copy()
+    var_lb = np.asarray(var_lb, dtype=np.float64)
+    var_ub = np.asarray(var_ub, dtype=np.float64)
+    con_lb = np.asarray(con_lb, dtype=np.float64)
+    con_ub = np.asarray(con_ub, dtype=np.float64)
+    if backend == "device":
+        vals, D_r, D_c = _scale_matrix_device(A, ruiz_iterations, pock_chambolle_alpha, device)
+    elif backend == "host":
+        vals, D_r, D_c = _scale_matrix_host(A, ruiz_iterations, pock_chambolle_alpha)
+    else:
+        raise ValueError("backend must be 'device' or 'host'")
 
     c_s = c * D_c
     with np.errstate(invalid="ignore"):

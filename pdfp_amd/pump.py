@@ -198,12 +198,14 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                      projection_iteration_limit: int = 500_000, batch_policy: str = "cost",
                      push_threshold: float = 0.01, violation_threshold: float = 5000.0,
                      propagation_budget: int = 100_000, feasibility_tolerance: float = 1e-6,
-                     precision: str = "fp32", device: int = 0,
+                     precision: str = "fp32", device: int = 0, scaling: str = "device",
                      projector=None, root_lp=None) -> PumpResult:
     """Search for a feasible solution of ``milp`` with the feasibility pump
     (src/pump.jl:231-410).  The keyword arguments and defaults are the
     reference's (README.md:89-108); ``precision`` picks the projector's form
-    ("fp32": FP32 data and iterates with FP64 reductions and polish; "fp64").
+    ("fp32": FP32 data and iterates with FP64 reductions and polish; "fp64");
+    ``scaling`` where the projector's matrix is equilibrated ("device" or
+    "host").
 
     ``projector`` / ``root_lp`` replace the device solvers: a module with
     ``PDFPState``, ``set_target``, ``warm_start``, ``pdfp_solve``,
@@ -217,7 +219,11 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
     from . import pdlp as _pdlp_mod
 
     P = projector if projector is not None else _pdfp_mod
-    solve_lp = root_lp if root_lp is not None else _pdlp_mod.solve_lp_relaxation
+    if root_lp is not None:
+        solve_lp = root_lp
+    else:
+        def solve_lp(milp_, **kw):
+            return _pdlp_mod.solve_lp_relaxation(milp_, scaling=scaling, device=device, **kw)
     t0 = time.perf_counter()
 
     def elapsed():
@@ -277,7 +283,7 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
 
     # The projector, built once: scaled data on the device, warm-started at
     # the root LP solution (src/pump.jl:293-309).
-    pp = _pdfp_mod.projector_problem(milp)
+    pp = _pdfp_mod.projector_problem(milp, scaling=scaling, device=device)
     D_c = pp.D_c
     projector_state = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
                                   precision=precision, device=device)

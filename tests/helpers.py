@@ -46,7 +46,9 @@ def project_with(module, state_cls, milp, x_tilde, tol=1e-8, iteration_limit=1_0
                  **kw):
     """The reference's test helper (test/runtests.jl:12-27) with a chosen
     state implementation: module provides set_target / pdfp_solve."""
-    pp = projector_problem(milp)
+    # the oracle's runs stay on the CPU: its scaling is the host restatement
+    on_host = getattr(module, "__name__", "").startswith("oracle")
+    pp = projector_problem(milp, scaling="host" if on_host else "device")
     s = state_cls(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w, **kw)
     module.set_target(s, np.asarray(x_tilde, dtype=np.float64) / pp.D_c)
     p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,

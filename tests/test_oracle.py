@@ -142,7 +142,8 @@ def test_iteration_and_time_limits():
 def test_scaling_numpy_equals_oracle_bitwise(maker, rescale):
     m = maker()
     con_lb = np.where(m.row_sense == "eq", m.b, -np.inf)
-    a = scale_problem(m.A, m.c, m.l, m.u, con_lb, m.b, bound_objective_rescaling=rescale)
+    a = scale_problem(m.A, m.c, m.l, m.u, con_lb, m.b, bound_objective_rescaling=rescale,
+                      backend="host")
     b = O.scale_problem(m.A, m.c, m.l, m.u, con_lb, m.b, bound_objective_rescaling=rescale)
     for k in ("D_r", "D_c", "c", "var_lb", "var_ub", "con_lb", "con_ub"):
         assert np.array_equal(getattr(a, k), b[k]), k
@@ -161,7 +162,7 @@ def test_scaling_permutation_closed_form():
     perm = np.array([2, 0, 3, 1])
     A = sp.csc_matrix((vals, (np.arange(4), perm)), shape=(4, 4))
     a = scale_problem(A, np.zeros(4), np.zeros(4), np.ones(4), np.full(4, -np.inf), np.ones(4),
-                      bound_objective_rescaling=False)
+                      bound_objective_rescaling=False, backend="host")
     np.testing.assert_allclose(np.abs(a.A.data), 1.0, rtol=1e-15)
     np.testing.assert_allclose(a.D_r * a.D_c[perm], 1.0 / np.abs(vals), rtol=1e-15)
 

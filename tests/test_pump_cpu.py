@@ -20,7 +20,7 @@ from .helpers import oracle_projector_module, oracle_root_lp
 
 def pump(milp, **kw):
     return feasibility_pump(milp, projector=oracle_projector_module(), root_lp=oracle_root_lp,
-                            **kw)
+                            scaling="host", **kw)
 
 
 @pytest.mark.parametrize("name", ["p0201", "gen-ip054", "pk1", "markshare2"])
@@ -136,7 +136,7 @@ def test_tight_projections_leave_the_fp32_form():
                           original_primal_residual=O.original_primal_residual)
     m = read_mps(example_path("pk1"))
     r = feasibility_pump(m, projector=mod, root_lp=oracle_root_lp, time_limit=120.0,
-                         precision="fp32")
+                         precision="fp32", scaling="host")
     assert r.found and is_feasible(m, r.x)
     assert calls and calls[0][0] == "fp32"
     assert all(prec == "fp64" for prec, eps in calls if eps < FP32_TOLERANCE_FLOOR)
