@@ -420,6 +420,11 @@ struct Projector final : ProjectorBase {
 
   // estimate_max_singular_value  src/preconditioner.jl:108-125
   double max_singular_value(int max_iterations, double tolerance) override {
+    // FP32 form on a small instance: the whole iteration in one workgroup
+    if (kFused && small_ok && path == 0) {
+      const double s = small_sigma(A, At, m, n, max_iterations, tolerance, stream);
+      if (!std::isnan(s)) return s;
+    }
     return power_iteration(A, At, m, n, max_iterations, tolerance, stream);
   }
 

@@ -17,6 +17,7 @@
 #include "engine.h"
 #include "host_util.h"
 #include "pdlp_kernels.h"
+#include "small_kernels.h"
 #include "solver_common.h"
 
 namespace pdfp {
@@ -163,7 +164,10 @@ struct LpSolver final : LpBase {
     if (nnz == 0) {
       step_size = 1.0;
     } else {
-      const double sv = power_iteration(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
+      // a small instance runs the whole iteration in one workgroup
+      double sv = small_sigma(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
+      if (std::isnan(sv))
+        sv = power_iteration(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
       step_size = (std::isfinite(sv) && sv > 0) ? 0.998 / sv : 0.998;
     }
     double lo_sq = 0.0, hi_sq = 0.0;

@@ -31,9 +31,11 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <vector>
 
 #include "engine.h"
+#include "host_util.h"
 #include "pdfp_kernels.h"
 
 namespace pdfp {
@@ -260,6 +262,165 @@ __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
     a.x[j] = s_x[j];
     a.y[j] = s_y[j];
   }
+}
+
+// ---- the power iteration in one workgroup ----------------------------------
+// estimate_max_singular_value (src/preconditioner.jl:108-125) for an instance
+// that fits in LDS: the iteration runs to its stopping rule inside one launch
+// (each iteration is two SpMVs and three reductions, a few microseconds),
+// instead of seven launches per iteration.  The sums are FP64, in a fixed
+// order.  LDS: small_power_lds_bytes.
+
+template <typename TV>
+struct SmallCsrT {
+  const int32_t* __restrict__ ptr;
+  const int32_t* __restrict__ col;
+  const TV* __restrict__ val;
+};
+
+template <typename TV>
+inline int64_t small_power_lds_bytes(int64_t m, int64_t n, int64_t nnz) {
+  return 4 * (n + 1) + 4 * (m + 1) + 8 * nnz + 2 * (int64_t)sizeof(TV) * nnz + 16 * m + 8 * n;
+}
+
+// Sum over the workgroup, returned to every thread.
+template <int BLOCK>
+__device__ __forceinline__ double block_sum_all(double v, double* red /* BLOCK/64 + 1 */) {
+  constexpr int kW = BLOCK / kWave;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kW; ++w) s += red[w];
+    red[kW] = s;
+  }
+  __syncthreads();
+  const double s = red[kW];
+  __syncthreads();  // red is reused by the next call
+  return s;
+}
+
+// y[r] = Σ_k val[k]·x[col[k]] over the rows r < nrows of a CSR in LDS, one
+// group of L lanes per row.
+template <int BLOCK, typename TV>
+__device__ __forceinline__ void small_spmv(int nrows, int L, const int32_t* ptr,
+                                           const int32_t* col, const TV* val, const double* x,
+                                           double* y) {
+  const int groups = BLOCK / L, lane = threadIdx.x & (L - 1);
+  for (int r0 = 0; r0 < nrows; r0 += groups) {
+    const int r = r0 + (int)threadIdx.x / L;
+    double s = 0.0;
+    if (r < nrows) {
+      const int k1 = ptr[r + 1];
+#pragma unroll 4
+      for (int k = ptr[r] + lane; k < k1; k += L) s += (double)val[k] * x[col[k]];
+    }
+    for (int o = L >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o, L);
+    if (r < nrows && lane == 0) y[r] = s;
+  }
+}
+
+template <int BLOCK, typename TV>
+__global__ __launch_bounds__(BLOCK) void k_small_power(SmallCsrT<TV> A, SmallCsrT<TV> At, int m,
+                                                       int n, int nnz, int lanes_a, int lanes_t,
+                                                       const double* __restrict__ start,
+                                                       int max_iterations, double tolerance,
+                                                       double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[BLOCK / kWave + 1];
+  const int tid = threadIdx.x;
+  double* eig = reinterpret_cast<double*>(smem);
+  double* nxt = eig + m;
+  double* dual = nxt + m;
+  TV* t_val = reinterpret_cast<TV*>(dual + n);
+  TV* a_val = t_val + nnz;
+  int32_t* t_ptr = reinterpret_cast<int32_t*>(a_val + nnz);
+  int32_t* a_ptr = t_ptr + (n + 1);
+  int32_t* t_col = a_ptr + (m + 1);
+  int32_t* a_col = t_col + nnz;
+  for (int k = tid; k <= n; k += BLOCK) t_ptr[k] = At.ptr[k];
+  for (int k = tid; k <= m; k += BLOCK) a_ptr[k] = A.ptr[k];
+  for (int k = tid; k < nnz; k += BLOCK) {
+    t_col[k] = At.col[k];
+    t_val[k] = At.val[k];
+    a_col[k] = A.col[k];
+    a_val[k] = A.val[k];
+  }
+  for (int i = tid; i < m; i += BLOCK) eig[i] = start[i];
+  __syncthreads();
+  double sigma_sq = 1.0;
+  int it = 0;
+  for (; it < max_iterations; ++it) {
+    double acc = 0.0;
+    for (int i = tid; i < m; i += BLOCK) acc += eig[i] * eig[i];
+    const double nrm = sqrt(block_sum_all<BLOCK>(acc, red));
+    if (nrm == 0.0) break;
+    for (int i = tid; i < m; i += BLOCK) nxt[i] = eig[i] / nrm;
+    __syncthreads();
+    small_spmv<BLOCK>(n, lanes_t, t_ptr, t_col, t_val, nxt, dual);
+    __syncthreads();
+    small_spmv<BLOCK>(m, lanes_a, a_ptr, a_col, a_val, dual, eig);
+    __syncthreads();
+    acc = 0.0;
+    for (int i = tid; i < m; i += BLOCK) acc += nxt[i] * eig[i];
+    sigma_sq = block_sum_all<BLOCK>(acc, red);
+    acc = 0.0;
+    for (int i = tid; i < m; i += BLOCK) {
+      const double v = eig[i] - sigma_sq * nxt[i];
+      nxt[i] = v;
+      acc += v * v;
+    }
+    const double res = block_sum_all<BLOCK>(acc, red);
+    if (sqrt(res) < tolerance) {
+      ++it;
+      break;
+    }
+  }
+  if (tid == 0) {
+    out[0] = sigma_sq;
+    out[1] = (double)it;
+  }
+}
+
+// Host side: σ_max by the one-workgroup power iteration from `start` (m
+// values), or NAN when the instance does not fit in LDS.
+template <typename TV>
+double small_power_iteration(SmallCsrT<TV> A, SmallCsrT<TV> At, const std::vector<int32_t>& a_ptr,
+                             const std::vector<int32_t>& t_ptr, int64_t m, int64_t n, int64_t nnz,
+                             const double* start, int max_iterations, double tolerance,
+                             hipStream_t stream) {
+  const int64_t lds = small_power_lds_bytes<TV>(m, n, nnz);
+  if (m <= 0 || n <= 0 || nnz <= 0 || lds > kSmallLdsMax) return NAN;
+  DevBuf<double> d_start, d_out;
+  PinnedBuf<double> h_out;
+  d_start.alloc((size_t)m);
+  d_out.alloc(2);
+  h_out.alloc(2);
+  d_start.upload(start, (size_t)m, stream);
+  k_small_power<kSmallBlock, TV><<<1, kSmallBlock, (size_t)lds, stream>>>(
+      A, At, (int)m, (int)n, (int)nnz, small_lanes(a_ptr, kSmallBlock),
+      small_lanes(t_ptr, kSmallBlock), d_start.p, max_iterations, tolerance, d_out.p);
+  PDFP_HIP(hipGetLastError());
+  PDFP_HIP(hipMemcpyAsync(h_out.p, d_out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+  PDFP_HIP(hipStreamSynchronize(stream));
+  return std::sqrt(std::max(h_out.p[0], 0.0));
+}
+
+// The same for an operator pair on the device, from the reference's start
+// vector; NAN when it does not fit.
+template <typename TV>
+double small_sigma(const DevCsr<TV>& A, const DevCsr<TV>& At, int64_t m, int64_t n,
+                   int max_iterations, double tolerance, hipStream_t stream) {
+  if (m <= 0 || small_power_lds_bytes<TV>(m, n, A.nnz) > kSmallLdsMax) return NAN;
+  std::vector<double> start((size_t)m);
+  start_vector(m, kSvSeed, start.data());
+  return small_power_iteration<TV>(SmallCsrT<TV>{A.rowptr.p, A.col.p, A.val.p},
+                                   SmallCsrT<TV>{At.rowptr.p, At.col.p, At.val.p}, A.h_rowptr,
+                                   At.h_rowptr, m, n, A.nnz, start.data(), max_iterations,
+                                   tolerance, stream);
 }
 
 }  // namespace pdfp

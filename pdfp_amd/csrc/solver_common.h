@@ -218,6 +218,24 @@ static __global__ __launch_bounds__(256) void k_scale_div(int m, double* __restr
   if (i < m) out[i] = in[i] / nrm;
 }
 
+// out = in / √(*sumsq): the norm is read from device memory, so that power
+// iterations run back to back without a host round trip.
+static __global__ __launch_bounds__(256) void k_scale_div_dev(int m, double* __restrict__ out,
+                                                              const double* __restrict__ in,
+                                                              const double* __restrict__ sumsq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = in[i] / sqrt(sumsq[0]);
+}
+
+// hist[0..3) = {σ², ‖nxt‖², ‖eig‖²} of one power iteration
+static __global__ void k_pi_record(const double* __restrict__ sums, double* __restrict__ hist) {
+  if (threadIdx.x == 0) {
+    hist[0] = sums[4];
+    hist[1] = sums[0];
+    hist[2] = sums[1];
+  }
+}
+
 // nxt = eig - σ² nxt;  acc0 = ‖nxt‖², acc1 = ‖eig‖²  (σ² read from device)
 static __global__ __launch_bounds__(256) void k_pi_resid(int m, double* __restrict__ nxt,
                                                          const double* __restrict__ eig,
@@ -315,8 +333,7 @@ int autotune_schedule(DevCsr<TV>& M, int64_t ncols, hipStream_t s) {
 }
 
 // estimate_max_singular_value(A, A_t, m; max_iterations, tolerance)
-// src/preconditioner.jl:108-125, on the device: one host synchronisation per
-// iteration (the norms of the residual and of the next vector).
+// src/preconditioner.jl:108-125, on the device.
 template <typename TV>
 double power_iteration(const DevCsr<TV>& A, const DevCsr<TV>& At, int64_t m, int64_t n,
                        int max_iterations, double tolerance, hipStream_t stream) {
@@ -333,25 +350,49 @@ double power_iteration(const DevCsr<TV>& A, const DevCsr<TV>& At, int64_t m, int
   std::vector<double> start((size_t)m);
   start_vector(m, kSvSeed, start.data());
   eig.upload(start.data(), (size_t)m, stream);
+  // Iterations run in chunks with no host synchronisation inside a chunk:
+  // each records {σ², ‖residual‖², ‖eig‖²} on the device, and the host then
+  // replays the reference's stopping rule over the records, so the result is
+  // the one the iteration-by-iteration loop returns (iterations past the
+  // stopping point are computed and discarded).
+  constexpr int kMaxChunk = 64;
+  DevBuf<double> hist;
+  PinnedBuf<double> h_hist;
+  hist.alloc(3 * kMaxChunk);
+  h_hist.alloc(3 * kMaxChunk);
   k_sumsq<<<gm, 256, 0, stream>>>((int)m, eig.p, part.p);
-  finalize_partials<256><<<1, 256, 0, stream>>>(part.p, gm, 1, 0, sums.p);
+  finalize_partials<256><<<1, 256, 0, stream>>>(part.p, gm, 1, 0, sums.p + 1);
   PDFP_HIP(hipMemcpyAsync(h.p, sums.p, sizeof(double) * 8, hipMemcpyDeviceToHost, stream));
   PDFP_HIP(hipStreamSynchronize(stream));
-  double nrm = std::sqrt(h.p[0]);
+  double nrm = std::sqrt(h.p[1]);
   double sigma_sq = 1.0;
-  for (int it = 0; it < max_iterations; ++it) {
-    if (nrm == 0.0) break;
-    k_scale_div<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, nrm);
-    launch_pass<0>(At, Gather1<double>{nxt.p}, EpiStore<double>{dual.p}, nullptr, stream);
-    launch_pass<1>(A, Gather1<double>{dual.p}, EpiStoreDot{eig.p, nxt.p}, part.p, stream);
-    finalize_partials<256><<<1, 256, 0, stream>>>(part.p, A.nblocks, 1, 0, sums.p + 4);
-    k_pi_resid<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, sums.p + 4, part.p);
-    finalize_partials<256><<<1, 256, 0, stream>>>(part.p, gm, 2, 0, sums.p);
-    PDFP_HIP(hipMemcpyAsync(h.p, sums.p, sizeof(double) * 8, hipMemcpyDeviceToHost, stream));
+  int done = 0, chunk = 4;
+  bool stop = false;
+  while (!stop && done < max_iterations) {
+    const int c = std::min(chunk, max_iterations - done);
+    for (int t = 0; t < c; ++t) {
+      k_scale_div_dev<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, sums.p + 1);
+      launch_pass<0>(At, Gather1<double>{nxt.p}, EpiStore<double>{dual.p}, nullptr, stream);
+      launch_pass<1>(A, Gather1<double>{dual.p}, EpiStoreDot{eig.p, nxt.p}, part.p, stream);
+      finalize_partials<256><<<1, 256, 0, stream>>>(part.p, A.nblocks, 1, 0, sums.p + 4);
+      k_pi_resid<<<gm, 256, 0, stream>>>((int)m, nxt.p, eig.p, sums.p + 4, part.p);
+      finalize_partials<256><<<1, 256, 0, stream>>>(part.p, gm, 2, 0, sums.p);
+      k_pi_record<<<1, 64, 0, stream>>>(sums.p, hist.p + 3 * t);
+    }
+    PDFP_HIP(hipMemcpyAsync(h_hist.p, hist.p, sizeof(double) * 3 * (size_t)c,
+                            hipMemcpyDeviceToHost, stream));
     PDFP_HIP(hipStreamSynchronize(stream));
-    sigma_sq = h.p[4];
-    if (std::sqrt(h.p[0]) < tolerance) break;
-    nrm = std::sqrt(h.p[1]);
+    for (int t = 0; t < c && !stop; ++t) {
+      if (nrm == 0.0) {
+        stop = true;
+        break;
+      }
+      sigma_sq = h_hist.p[3 * t];
+      if (std::sqrt(h_hist.p[3 * t + 1]) < tolerance) stop = true;
+      nrm = std::sqrt(h_hist.p[3 * t + 2]);
+    }
+    done += c;
+    chunk = std::min(2 * chunk, kMaxChunk);
   }
   return std::sqrt(jmax(sigma_sq, 0.0));
 }
