@@ -295,6 +295,36 @@ int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
  * it. */
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);
 
+/* ---- the pump's rounding (host) -------------------------------------------
+ * Constraint-propagation rounding (FP 2.0's CPR) and the guarded two-phase
+ * CPR (src/rounding.jl:100-256) over the bound propagation of
+ * src/propagation.jl, in C++ on the host with the reference's arithmetic.
+ * The caller ranks the integer variables (src/rounding.jl:105-115, the only
+ * random draws) and keeps the batch policy's timing (src/rounding.jl:22-63).
+ * A rounder holds PropagationData for one model (src/propagation.jl:19-84). */
+typedef struct pdfp_rounder pdfp_rounder;
+
+/* Rounder(milp) (src/rounding.jl:78-98): the model's CSC (m rows), right-hand
+ * sides, is_le (1: ≤ row, 0: = row), variable bounds (±Inf allowed), the
+ * integer mask and the propagation budget per call (max_pops). */
+int pdfp_rounder_create(pdfp_rounder** out, int64_t m, int64_t n, const int64_t* colptr,
+                        const int64_t* rowval, const double* nzval, int32_t index_base,
+                        const double* b, const uint8_t* is_le, const double* var_lb,
+                        const double* var_ub, const uint8_t* is_int, int64_t max_pops);
+/* rounder.direction: n flags of the violation-guided columns, or NULL (off). */
+int pdfp_rounder_set_direction(pdfp_rounder* r, const uint8_t* direction);
+/* round_sequential (src/rounding.jl:191-197): CPR of x_star[n] over the k
+ * ranked columns; writes xt[n]. */
+int pdfp_round_sequential(pdfp_rounder* r, const double* x_star, const int64_t* ranked,
+                          int64_t k, double* xt);
+/* round_guarded's batch and rounding (src/rounding.jl:205-256) with schedule
+ * regroup (1) or all (0).  *batch_infeasible = 1 when the guard discards the
+ * batch: the caller then rounds sequentially (the reference's fallback). */
+int pdfp_round_guarded(pdfp_rounder* r, const double* x_star, const int64_t* ranked, int64_t k,
+                       int32_t regroup, double* xt, int32_t* batch_infeasible);
+const char* pdfp_rounder_last_error(const pdfp_rounder* r);
+void pdfp_rounder_destroy(pdfp_rounder* r);
+
 /* ---- preconditioning ------------------------------------------------------
  * scale_problem (src/preconditioner.jl:26-63), the matrix part, on the
  * device: `ruiz_iterations` Ruiz rounds and one Pock–Chambolle pass

@@ -137,6 +137,14 @@ SIGNATURES = {
     "pdfp_scale_matrix": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32,
                                     C.c_int32, C.c_double, _D, _D, _D, C.c_int32]),
     "pdfp_scale_last_error": (C.c_char_p, []),
+    "pdfp_rounder_create": (C.c_int, [C.POINTER(_P), C.c_int64, C.c_int64, _I64, _I64, _D,
+                                      C.c_int32, _D, _U8, _D, _D, _U8, C.c_int64]),
+    "pdfp_rounder_set_direction": (C.c_int, [_P, _U8]),
+    "pdfp_round_sequential": (C.c_int, [_P, _D, _I64, C.c_int64, _D]),
+    "pdfp_round_guarded": (C.c_int, [_P, _D, _I64, C.c_int64, C.c_int32, _D,
+                                     C.POINTER(C.c_int32)]),
+    "pdfp_rounder_last_error": (C.c_char_p, [_P]),
+    "pdfp_rounder_destroy": (None, [_P]),
     "pdfp_probe_last_error": (C.c_char_p, []),
 }
 

@@ -13,6 +13,11 @@ caller's ``numpy.random.Generator`` is used at the same points
 (``rand(rng, 1:len)`` ↔ ``rng.integers(1, len + 1)``).  Every rounding that
 draws nothing (iteration < swap_iteration, no flips or perturbations) is
 therefore identical to the reference's.
+
+``Rounder(backend="native")`` (the default) runs the propagation and the
+rounding passes in C++ (``pdfp_rounder_*`` in libpdfp_hip); the ranking, its
+random draws and the batch policy's timing stay here.  ``backend="python"``
+runs everything in this module (PropagationState); the two are bit-identical.
 """
 from __future__ import annotations
 
@@ -81,38 +86,53 @@ class Rounder:
     (src/rounding.jl:78-98)."""
 
     def __init__(self, milp: MILP, *, schedule: str = "all", max_pops: int = 2**63 - 1,
-                 swap_iteration: int = 10, policy: BatchPolicy | None | str = "default"):
+                 swap_iteration: int = 10, policy: BatchPolicy | None | str = "default",
+                 backend: str = "native"):
+        if backend not in ("native", "python"):
+            raise ValueError("backend must be 'native' or 'python'")
         self.milp = milp
         self.data: PropagationData | None = None
         self.schedule = schedule                # "all" or "regroup"
-        self.direction: list | None = None      # violation-guided columns, or None
+        self._direction: list | None = None     # violation-guided columns, or None
         self.max_pops = max_pops
         self.swap_iteration = swap_iteration
         self.policy = BatchPolicy() if policy == "default" else policy
+        self.backend = backend
         self.n_calls = 0
         self.n_batched = 0
         self.n_fallbacks = 0
         self._int_idx = milp.int_idx.tolist()
+        self._int_arr = np.asarray(milp.int_idx, dtype=np.int64)
         self._binary = [(milp.l[j] == 0.0 and milp.u[j] == 1.0) for j in range(len(milp.c))]
+        self._bin_penalty = np.where((milp.l == 0.0) & (milp.u == 1.0), 10.0, 0.0)[self._int_arr]
         self._b = milp.b.tolist()
         self._is_le = [s == "le" for s in milp.row_sense.tolist()]
+        self._native = None
+        if backend == "native":
+            self._native = _NativeRounder(milp, max_pops)
+
+    @property
+    def direction(self) -> list | None:
+        return self._direction
+
+    @direction.setter
+    def direction(self, value) -> None:
+        self._direction = value
+        if self._native is not None:
+            self._native.set_direction(value)
 
     def _new_state(self) -> PropagationState:
         if self.data is None:
             self.data = PropagationData(self.milp)
         return PropagationState(self.data, self.max_pops)
 
-    def _rank_key(self, j: int, x) -> float:
-        v = x[j]
-        return abs(v - jround(v)) - (10.0 if self._binary[j] else 0.0)
-
-    def ranking(self, x_star, rng, iteration: int) -> list:
-        """Ascending fractionality, binaries first; from swap_iteration on,
-        |I|/10 random transpositions (src/rounding.jl:105-115)."""
-        idx = self._int_idx
-        keys = [self._rank_key(j, x_star) for j in idx]
-        order = sorted(range(len(idx)), key=keys.__getitem__)      # stable, as sortperm
-        ranked = [idx[k] for k in order]
+    def ranking(self, x_star, rng, iteration: int) -> np.ndarray:
+        """Ascending fractionality, binaries first (a stable sort, as
+        sortperm); from swap_iteration on, |I|/10 random transpositions
+        (src/rounding.jl:105-115)."""
+        v = np.asarray(x_star, dtype=np.float64)[self._int_arr]
+        keys = np.abs(v - np.rint(v)) - self._bin_penalty
+        ranked = self._int_arr[np.argsort(keys, kind="stable")]
         n = len(ranked)
         if iteration >= self.swap_iteration and n >= 10:
             for _ in range(max(1, n // 10)):
@@ -183,11 +203,13 @@ class Rounder:
 
     def round_sequential(self, x_star, rng, iteration: int) -> np.ndarray:
         """FP 2.0's CPR of x_star (src/rounding.jl:191-197)."""
+        ranked = self.ranking(x_star, rng, iteration)
+        if self._native is not None:
+            return self._native.round_sequential(x_star, ranked)
         s = self._new_state()
         xs = [float(v) for v in x_star]
         xt = list(xs)
-        ranked = self.ranking(xs, rng, iteration)
-        self._round_sequentially(s, xt, ranked, xs, self._activity())
+        self._round_sequentially(s, xt, ranked.tolist(), xs, self._activity())
         return np.array(self._read_integers(s, xt, xs))
 
     def round_guarded(self, x_star, rng, iteration: int) -> np.ndarray:
@@ -199,11 +221,26 @@ class Rounder:
             xt = self.round_sequential(x_star, rng, iteration)
             P.observe_skip((time.perf_counter_ns() - t0) / 1e9)
             return xt
+        ranked_arr = self.ranking(x_star, rng, iteration)
+        if self._native is not None:
+            xt_native, infeasible = self._native.round_guarded(x_star, ranked_arr,
+                                                               self.schedule == "regroup")
+            if infeasible:               # the guard: discard the batch
+                self.n_fallbacks += 1
+                wasted = 0.0 if P is None else (time.perf_counter_ns() - t0) / 1e9
+                out = self.round_sequential(x_star, rng, iteration)
+                if P is not None:
+                    P.observe_batch(True, (time.perf_counter_ns() - t0) / 1e9, wasted)
+                return out
+            self.n_batched += 1
+            if P is not None:
+                P.observe_batch(False, (time.perf_counter_ns() - t0) / 1e9, 0.0)
+            return xt_native
         s = self._new_state()
         xs = [float(v) for v in x_star]
         xt = list(xs)
         activity = self._activity()
-        ranked = self.ranking(xs, rng, iteration)
+        ranked = ranked_arr.tolist()
 
         def integral(j):
             return abs(xs[j] - jround(xs[j])) == 0.0
@@ -246,3 +283,71 @@ class Rounder:
         if P is not None:
             P.observe_batch(False, (time.perf_counter_ns() - t0) / 1e9, 0.0)
         return np.array(xt)
+
+
+class _NativeRounder:
+    """The C++ rounder of libpdfp_hip (pdfp_rounder_*)."""
+
+    def __init__(self, milp: MILP, max_pops: int):
+        import ctypes as C
+
+        from . import _lib
+
+        self._C = C
+        self._L = _lib
+        lib = _lib.load()
+        self._lib = lib
+        A = milp.A.tocsc()
+        self.n = A.shape[1]
+        colptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        rowval = np.ascontiguousarray(A.indices, dtype=np.int64)
+        nzval = np.ascontiguousarray(A.data, dtype=np.float64)
+        b = np.ascontiguousarray(milp.b, dtype=np.float64)
+        is_le = np.ascontiguousarray(milp.row_sense == "le", dtype=np.uint8)
+        lb = np.ascontiguousarray(milp.l, dtype=np.float64)
+        ub = np.ascontiguousarray(milp.u, dtype=np.float64)
+        is_int = np.zeros(self.n, dtype=np.uint8)
+        is_int[milp.int_idx] = 1
+        ctx = C.c_void_p()
+        st = lib.pdfp_rounder_create(C.byref(ctx), A.shape[0], self.n, _lib.i64ptr(colptr),
+                                     _lib.i64ptr(rowval), _lib.dptr(nzval), 0, _lib.dptr(b),
+                                     _lib.u8ptr(is_le), _lib.dptr(lb), _lib.dptr(ub),
+                                     _lib.u8ptr(is_int), int(min(max_pops, 2**63 - 1)))
+        _lib.check(st, lib.pdfp_rounder_last_error, None)
+        self._ctx = ctx
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            self._lib.pdfp_rounder_destroy(ctx)
+            self._ctx = None
+
+    def _check(self, st):
+        self._L.check(st, self._lib.pdfp_rounder_last_error, self._ctx)
+
+    def set_direction(self, direction) -> None:
+        if direction is None:
+            self._check(self._lib.pdfp_rounder_set_direction(self._ctx, None))
+        else:
+            d = np.ascontiguousarray(direction, dtype=np.uint8)
+            self._check(self._lib.pdfp_rounder_set_direction(self._ctx, self._L.u8ptr(d)))
+
+    def round_sequential(self, x_star, ranked) -> np.ndarray:
+        xs = np.ascontiguousarray(x_star, dtype=np.float64)
+        rk = np.ascontiguousarray(ranked, dtype=np.int64)
+        xt = np.empty(self.n)
+        self._check(self._lib.pdfp_round_sequential(self._ctx, self._L.dptr(xs),
+                                                    self._L.i64ptr(rk), len(rk),
+                                                    self._L.dptr(xt)))
+        return xt
+
+    def round_guarded(self, x_star, ranked, regroup: bool):
+        C = self._C
+        xs = np.ascontiguousarray(x_star, dtype=np.float64)
+        rk = np.ascontiguousarray(ranked, dtype=np.int64)
+        xt = np.empty(self.n)
+        bad = C.c_int32(0)
+        self._check(self._lib.pdfp_round_guarded(self._ctx, self._L.dptr(xs),
+                                                 self._L.i64ptr(rk), len(rk), int(regroup),
+                                                 self._L.dptr(xt), C.byref(bad)))
+        return xt, bool(bad.value)
