@@ -180,29 +180,39 @@ def _oracle_pump_backends():
 
 def time_to_first_feasible(runs: int, cpu: bool):
     """The full pump (root LP, projections, rounding) on the 133 × 201 binary
-    set-covering instance (seed 1) and on p0201: median PumpResult.time over
-    `runs` runs after one warm-up run, as the reference times it
-    (scripts/solve.jl:88-97)."""
+    set-covering instance (seed 1), on p0201 and on a 2000 × 4000 mixed MILP
+    (seed 1): median PumpResult.time over `runs` runs after one warm-up run,
+    as the reference times it (scripts/solve.jl:88-97); FP32 and FP64 single
+    trajectories and 8 concurrent trajectories on the batched projector
+    (first feasible wins, as scripts/solve.jl:121-148)."""
     from pdfp_amd import generators as G
     from pdfp_amd.milp import is_feasible, read_mps
-    from pdfp_amd.pump import feasibility_pump
+    from pdfp_amd.pump import feasibility_pump, feasibility_pump_batch
 
     out = {}
     cases = [("set_cover_133x201_seed1", G.set_cover(133, 201, 1)),
              ("p0201", read_mps(os.path.join(ROOT, "tests", "golden", "examples",
-                                             "p0201.mps.gz")))]
+                                             "p0201.mps.gz"))),
+             ("mixed_2000x4000_seed1", G.mixed(2000, 4000, 1))]
     feasibility_pump(G.set_cover(133, 201, 7), time_limit=60.0)        # warm-up
+    feasibility_pump_batch(G.set_cover(133, 201, 7), time_limit=60.0)
+    forms = {"fp32": lambda m: feasibility_pump(m, time_limit=60.0, precision="fp32"),
+             "fp64": lambda m: feasibility_pump(m, time_limit=60.0, precision="fp64"),
+             "fp32_batch8": lambda m: feasibility_pump_batch(m, trajectories=8,
+                                                             time_limit=60.0)}
     for name, milp in cases:
         rec = {}
-        for prec in ("fp32", "fp64"):
-            rs = [feasibility_pump(milp, time_limit=60.0, precision=prec) for _ in range(runs)]
+        for form, run in forms.items():
+            rs = [run(milp) for _ in range(runs)]
             ok = all(r.found and is_feasible(milp, r.x) for r in rs)
             ts = sorted(r.time for r in rs)
             r0 = rs[0]
-            rec[prec] = {"seconds": ts[len(ts) // 2], "found": ok, "iterations": r0.iterations,
+            rec[form] = {"seconds": ts[len(ts) // 2], "found": ok, "iterations": r0.iterations,
                          "pdhg_iterations": r0.pdhg_iterations,
                          "root_lp_s": r0.root_lp_time, "projection_s": r0.projection_time,
                          "rounding_s": r0.rounding_time}
+            if form == "fp32_batch8":
+                rec[form]["winning_trajectory"] = r0.trajectory
         if cpu:
             proj, root_lp = _oracle_pump_backends()
             r = feasibility_pump(milp, time_limit=120.0, projector=proj, root_lp=root_lp,

@@ -47,6 +47,8 @@ class PumpResult:
     rounding_time: float
     time: float
     log: list = field(default_factory=list, repr=False)
+    trajectory: int = 0     # feasibility_pump_batch: the trajectory that found x
+    seed: int = 0
 
 
 @dataclass
@@ -232,82 +234,38 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
     def remaining():
         return max(time_limit - elapsed(), 1.0)
 
-    n = num_variables(milp)
-    m = num_constraints(milp)
-    int_idx = milp.int_idx
-    int_list = int_idx.tolist()
-    tol = feasibility_tolerance
+    cfg = _Config(time_limit, flip_parameter, restart_period, swap_iteration, gap_start,
+                  gap_decay, gap_floor, residual_ratio, projection_iteration_limit,
+                  batch_policy, push_threshold, violation_threshold, propagation_budget,
+                  feasibility_tolerance, _pdfp_mod.SolverParameters, t0)
     mixed = num_continuous(milp) > 0
-    rng = np.random.Generator(np.random.PCG64(seed))
-    rounder = Rounder(milp, max_pops=(2**63 - 1) if mixed else propagation_budget,
-                      swap_iteration=swap_iteration,
-                      policy=BatchPolicy() if batch_policy == "cost" else None)
     stats = _Counters()
     log: list = []
+    emit = _emitter(log, verbose)
+    x, dual_slack, done = _root(milp, cfg, solve_lp, root_lp_tolerance, stats, emit)
+    if done is not None:
+        return _finish(milp, True, done, stats, elapsed(), emit, log)
 
-    def emit(s: str):
-        log.append(s)
-        if verbose:
-            sys.stdout.write(s)
-            sys.stdout.flush()
+    # The projector, built at the first projection: scaled data on the
+    # device, warm-started at the root LP solution (src/pump.jl:293-309).
+    proj: dict = {}
+    m = num_constraints(milp)
+    primal_weight = math.sqrt(1.0 + (len(milp.int_idx) / m if m > 0 else 0.0))
 
-    emit(f"PDFP feasibility pump: {milp.name}\n")
-    emit(f"model       {m} rows, {n} columns ({len(int_list)} integer, {num_continuous(milp)} "
-         f"continuous), {milp.A.nnz} nonzeros\n")
+    def build():
+        pp = _pdfp_mod.projector_problem(milp, scaling=scaling, device=device)
+        st = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                         precision=precision, device=device)
+        if mixed:
+            P.original_primal_residual(st, pp.D_r)
+        st.set_x(x / pp.D_c)
+        proj.update(pp=pp, state=st)
 
-    # Root LP relaxation.
-    x, dual_slack, stats.root_lp_status, root_iterations = solve_lp(
-        milp, tolerance=root_lp_tolerance, time_limit=remaining(), cupdlpx_termination=mixed)
-    stats.root_lp_time = elapsed()
-    emit(f"root LP     {stats.root_lp_status} after {root_iterations} iterations, "
-         f"{stats.root_lp_time:.2f} s\n")
-    if is_integral(x, int_idx, tol):
-        xt = round_to_nearest(x, milp)
-        if max_violation(milp, xt) <= tol:
-            return _finish(milp, True, xt, stats, elapsed(), emit, log)
-
-    # First rounding, then the decision on the violation-guided rule.
-    rounder.schedule = batch_schedule(milp, x, dual_slack, push_threshold)
-    t = time.perf_counter()
-    xt = rounder.round_guarded(x, rng, 0)
-    stats.rounding_time += time.perf_counter() - t
-    total_violation = total_integer_row_violation(milp, xt)
-    if total_violation >= violation_threshold:
-        direction = [False] * n
-        for j in int_list:
-            direction[j] = True
-        rounder.direction = direction
-    emit(f"rounder     {rounder.schedule} schedule, violation-guided rounding "
-         f"{'off' if rounder.direction is None else 'on'} (first rounding violates by "
-         f"{total_violation:.4g})\n")
-
-    # The projector, built once: scaled data on the device, warm-started at
-    # the root LP solution (src/pump.jl:293-309).
-    pp = _pdfp_mod.projector_problem(milp, scaling=scaling, device=device)
-    D_c = pp.D_c
-    projector_state = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
-                                  precision=precision, device=device)
-    if mixed:
-        P.original_primal_residual(projector_state, pp.D_r)
-    projector_state.set_x(x / D_c)
-    primal_weight = math.sqrt(1.0 + (len(int_list) / m if m > 0 else 0.0))
-    gap = gap_start
-    tightening_budget = gap_start
-
-    general = [j for j in int_list if not (milp.l[j] == 0.0 and milp.u[j] == 1.0)]
-    is_general = np.zeros(n, dtype=bool)
-    is_general[general] = True
-    ladder = RestartLadder(0.0, 0, flip_parameter, max(len(general) // 10, 10))
-    seen: set = set()
-    h_current = integer_key(xt, int_idx)
-    since_restart = 0
-
-    emit(f"{'iter':>6s} {'distance':>12s} {'violation':>10s} {'pdhg':>10s} {'gap tol':>9s} "
-         f"{'action':>8s} {'time':>9s}\n")
-    params_cls = _pdfp_mod.SolverParameters
-    proj = {"state": projector_state}
-
-    def solve_projection(params, target_scaled=None):
+    def solve_projection(kind, xt, params):
+        if not proj:
+            build()
+        pp = proj["pp"]
+        target_scaled = None if kind == "tighten" else xt / pp.D_c
         # FP32 iterates do not resolve relative KKT errors much below 1e-5:
         # from the first projection asked for less, the FP64 form takes over,
         # started at the last FP32 projection.
@@ -331,12 +289,128 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         if target_scaled is not None:
             proj["target"] = target_scaled
         P.pdfp_solve(st, params, primal_weight)
-        return D_c * st.pdhg_x, st.total_count
+        return pp.D_c * st.pdhg_x, st.total_count
 
-    while elapsed() < time_limit:
+    traj = _trajectory(milp, cfg, seed, x, dual_slack, stats, emit)
+    found, xt = _drive(traj, solve_projection)
+    return _finish(milp, found, xt, stats, elapsed(), emit, log)
+
+
+@dataclass
+class _Config:
+    time_limit: float
+    flip_parameter: int
+    restart_period: int
+    swap_iteration: int
+    gap_start: float
+    gap_decay: float
+    gap_floor: float
+    residual_ratio: float
+    projection_iteration_limit: int
+    batch_policy: str
+    push_threshold: float
+    violation_threshold: float
+    propagation_budget: int
+    feasibility_tolerance: float
+    params_cls: type
+    t0: float
+
+    def elapsed(self) -> float:
+        return time.perf_counter() - self.t0
+
+    def remaining(self) -> float:
+        return max(self.time_limit - self.elapsed(), 1.0)
+
+
+def _emitter(log: list, verbose: bool):
+    def emit(s: str):
+        log.append(s)
+        if verbose:
+            sys.stdout.write(s)
+            sys.stdout.flush()
+    return emit
+
+
+def _root(milp, cfg: _Config, solve_lp, root_lp_tolerance, stats, emit):
+    """The header and the root LP relaxation (src/pump.jl:245-279); returns
+    (x, dual_slack, the rounded point when it is already feasible)."""
+    n = num_variables(milp)
+    m = num_constraints(milp)
+    mixed = num_continuous(milp) > 0
+    emit(f"PDFP feasibility pump: {milp.name}\n")
+    emit(f"model       {m} rows, {n} columns ({len(milp.int_idx)} integer, "
+         f"{num_continuous(milp)} continuous), {milp.A.nnz} nonzeros\n")
+    x, dual_slack, stats.root_lp_status, root_iterations = solve_lp(
+        milp, tolerance=root_lp_tolerance, time_limit=cfg.remaining(),
+        cupdlpx_termination=mixed)
+    stats.root_lp_time = cfg.elapsed()
+    emit(f"root LP     {stats.root_lp_status} after {root_iterations} iterations, "
+         f"{stats.root_lp_time:.2f} s\n")
+    if is_integral(x, milp.int_idx, cfg.feasibility_tolerance):
+        xt = round_to_nearest(x, milp)
+        if max_violation(milp, xt) <= cfg.feasibility_tolerance:
+            return x, dual_slack, xt
+    return x, dual_slack, None
+
+
+def _drive(traj, solve):
+    """Run one trajectory generator to its end with a projection callable."""
+    try:
+        req = traj.send(None)
+        while True:
+            req = traj.send(solve(*req))
+    except StopIteration as e:
+        return e.value
+
+
+def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
+    """One pump trajectory from the first rounding on (src/pump.jl:281-410),
+    as a generator: it yields each projection it needs, ("target", x̃,
+    params) for x̃ in original units or ("tighten", None, params) for a
+    warm-started re-solve of the same target, receives (x in original
+    units, PDHG iterations), and returns (found, x̃)."""
+    n = num_variables(milp)
+    int_idx = milp.int_idx
+    int_list = int_idx.tolist()
+    tol = cfg.feasibility_tolerance
+    mixed = num_continuous(milp) > 0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rounder = Rounder(milp, max_pops=(2**63 - 1) if mixed else cfg.propagation_budget,
+                      swap_iteration=cfg.swap_iteration,
+                      policy=BatchPolicy() if cfg.batch_policy == "cost" else None)
+    params_cls = cfg.params_cls
+
+    # First rounding, then the decision on the violation-guided rule.
+    rounder.schedule = batch_schedule(milp, x, dual_slack, cfg.push_threshold)
+    t = time.perf_counter()
+    xt = rounder.round_guarded(x, rng, 0)
+    stats.rounding_time += time.perf_counter() - t
+    total_violation = total_integer_row_violation(milp, xt)
+    if total_violation >= cfg.violation_threshold:
+        direction = [False] * n
+        for j in int_list:
+            direction[j] = True
+        rounder.direction = direction
+    emit(f"rounder     {rounder.schedule} schedule, violation-guided rounding "
+         f"{'off' if rounder.direction is None else 'on'} (first rounding violates by "
+         f"{total_violation:.4g})\n")
+
+    gap = cfg.gap_start
+    tightening_budget = cfg.gap_start
+    general = [j for j in int_list if not (milp.l[j] == 0.0 and milp.u[j] == 1.0)]
+    is_general = np.zeros(n, dtype=bool)
+    is_general[general] = True
+    ladder = RestartLadder(0.0, 0, cfg.flip_parameter, max(len(general) // 10, 10))
+    seen: set = set()
+    h_current = integer_key(xt, int_idx)
+    since_restart = 0
+
+    emit(f"{'iter':>6s} {'distance':>12s} {'violation':>10s} {'pdhg':>10s} {'gap tol':>9s} "
+         f"{'action':>8s} {'time':>9s}\n")
+    while cfg.elapsed() < cfg.time_limit:
         violation = max_violation(milp, xt)
         if violation <= tol and is_integral(xt, int_idx, tol):
-            return _finish(milp, True, xt.copy(), stats, elapsed(), emit, log)
+            return True, xt.copy()
         stats.iterations += 1
         iteration = stats.iterations
         since_restart += 1
@@ -344,26 +418,27 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         # Project x̃ onto the LP relaxation.
         t = time.perf_counter()
         gap_used = gap
-        params = params_cls(iteration_limit=projection_iteration_limit,
-                            eps_optimal_relative=gap, eps_feasible_relative=gap * residual_ratio,
-                            time_limit=remaining())
-        x, inner = solve_projection(params, xt / D_c)
+        params = params_cls(iteration_limit=cfg.projection_iteration_limit,
+                            eps_optimal_relative=gap,
+                            eps_feasible_relative=gap * cfg.residual_ratio,
+                            time_limit=cfg.remaining())
+        x, inner = yield ("target", xt, params)
         # Mixed models: re-solve an integral projection whose rounding is
         # infeasible only through its continuous part (src/pump.jl:340-356).
-        while (mixed and tightening_budget > 1e-6 and elapsed() < time_limit
+        while (mixed and tightening_budget > 1e-6 and cfg.elapsed() < cfg.time_limit
                and is_integral(x, int_idx, tol)
                and max_violation(milp, round_to_nearest(x, milp)) > tol):
             tightening_budget *= 0.1
             gap = max(gap * 0.1, 1e-8)
-            params = params_cls(iteration_limit=projection_iteration_limit,
+            params = params_cls(iteration_limit=cfg.projection_iteration_limit,
                                 eps_optimal_relative=gap,
-                                eps_feasible_relative=gap * residual_ratio,
-                                time_limit=remaining())
-            x, count = solve_projection(params)
+                                eps_feasible_relative=gap * cfg.residual_ratio,
+                                time_limit=cfg.remaining())
+            x, count = yield ("tighten", None, params)
             inner += count
         stats.pdhg_iterations += inner
-        if gap > gap_floor:
-            gap = max(gap * gap_decay, gap_floor)
+        if gap > cfg.gap_floor:
+            gap = max(gap * cfg.gap_decay, cfg.gap_floor)
         stats.projection_time += time.perf_counter() - t
         distance = 0.0
         for j in int_list:
@@ -373,8 +448,8 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
             candidate = round_to_nearest(x, milp)
             if max_violation(milp, candidate) <= tol:
                 _print_iteration(emit, iteration, distance, violation, inner, gap_used,
-                                 "integral", elapsed())
-                return _finish(milp, True, candidate, stats, elapsed(), emit, log)
+                                 "integral", cfg.elapsed())
+                return True, candidate
 
         # Round x; if the rounding did not change, flip (FP's stalling move).
         t = time.perf_counter()
@@ -385,14 +460,14 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         if _differs(x_round, xt, int_list):
             candidate[:] = x_round
         else:
-            flips = int(rng.integers(flip_parameter // 2, 3 * flip_parameter // 2 + 1))
+            flips = int(rng.integers(cfg.flip_parameter // 2, 3 * cfg.flip_parameter // 2 + 1))
             flip_farthest(candidate, x, milp, flips)
             action = "flip"
 
         # Restart on a cycle (a point seen before) and periodically.
         h_candidate = integer_key(candidate, int_idx)
         cycled = h_candidate in seen
-        periodic = since_restart >= restart_period
+        periodic = since_restart >= cfg.restart_period
         if cycled or periodic:
             perturb(candidate, x, milp, rng, general, is_general, ladder, iteration)
             stats.perturbations += 1
@@ -408,9 +483,163 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         xt = candidate
         if iteration <= 10 or iteration % 10 == 0:
             _print_iteration(emit, iteration, distance, violation, inner, gap_used, action,
-                             elapsed())
+                             cfg.elapsed())
     found = max_violation(milp, xt) <= tol and is_integral(xt, int_idx, tol)
-    return _finish(milp, found, xt, stats, elapsed(), emit, log)
+    return found, xt
+
+
+def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
+                           time_limit: float = 1800.0, verbose: bool = False,
+                           flip_parameter: int = 20, restart_period: int = 100,
+                           swap_iteration: int = 10, gap_start: float = 1e-2,
+                           gap_decay: float = 0.98, gap_floor: float = 1e-4,
+                           residual_ratio: float = 0.1, root_lp_tolerance: float = 1e-2,
+                           projection_iteration_limit: int = 500_000,
+                           batch_policy: str = "cost", push_threshold: float = 0.01,
+                           violation_threshold: float = 5000.0,
+                           propagation_budget: int = 100_000,
+                           feasibility_tolerance: float = 1e-6, device: int = 0,
+                           scaling: str = "device") -> PumpResult:
+    """K concurrent pump trajectories on one GPU; the first feasible point
+    wins.  Replaces solve_concurrent (scripts/solve.jl:121-148), which forks K
+    processes with seeds 100k + seed, each holding its own copy of the
+    instance on the GPU: here the trajectories (seeds 100k + seed, k = 1..K)
+    share one root LP and one batched FP32 projector (PDFPBatch, the matrix
+    in HBM once), and each round projects every trajectory that asks for a
+    projection in the same launches, then rounds each on the host.  A
+    trajectory asked for a tolerance tighter than the FP32 form resolves moves
+    to an FP64 projector of its own, as in feasibility_pump.  The result is
+    the winning trajectory's, with ``trajectory`` and ``seed`` set; its
+    ``time`` is the time to the first feasible point of the whole run."""
+    if not 1 <= trajectories <= 8:
+        raise ValueError("trajectories must be in 1..8")
+    if batch_policy not in ("cost", "always"):
+        raise ValueError("batch_policy must be 'cost' or 'always'")
+    from . import pdfp as _pdfp_mod
+    from . import pdlp as _pdlp_mod
+    from .batch import PDFPBatch
+
+    K = trajectories
+    t0 = time.perf_counter()
+    cfg = _Config(time_limit, flip_parameter, restart_period, swap_iteration, gap_start,
+                  gap_decay, gap_floor, residual_ratio, projection_iteration_limit,
+                  batch_policy, push_threshold, violation_threshold, propagation_budget,
+                  feasibility_tolerance, _pdfp_mod.SolverParameters, t0)
+    mixed = num_continuous(milp) > 0
+    root_stats = _Counters()
+    root_log: list = []
+
+    def solve_lp(milp_, **kw):
+        return _pdlp_mod.solve_lp_relaxation(milp_, scaling=scaling, device=device, **kw)
+
+    x, dual_slack, done = _root(milp, cfg, solve_lp, root_lp_tolerance, root_stats,
+                                _emitter(root_log, verbose))
+    seeds = [100 * (k + 1) + seed for k in range(K)]
+    if done is not None:
+        r = _finish(milp, True, done, root_stats, cfg.elapsed(), _emitter(root_log, verbose),
+                    root_log)
+        r.seed = seeds[0]
+        return r
+
+    stats = []
+    logs = []
+    gens = []
+    for k in range(K):
+        st = _Counters(root_lp_status=root_stats.root_lp_status,
+                       root_lp_time=root_stats.root_lp_time)
+        lg = list(root_log)
+        lg.append(f"trajectory  {k + 1} of {K}, seed {seeds[k]}\n")
+        stats.append(st)
+        logs.append(lg)
+        gens.append(_trajectory(milp, cfg, seeds[k], x, dual_slack, st,
+                                _emitter(lg, verbose and k == 0)))
+
+    pp = _pdfp_mod.projector_problem(milp, scaling=scaling, device=device)
+    D_c = pp.D_c
+    batch = PDFPBatch(K, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                      device=device)
+    if mixed:
+        batch.original_primal_residual(pp.D_r)
+    for k in range(K):
+        batch.set_x(k, x / D_c)
+    m = num_constraints(milp)
+    primal_weight = math.sqrt(1.0 + (len(milp.int_idx) / m if m > 0 else 0.0))
+    fp64: list = [None] * K        # a trajectory's own FP64 projector, once it needs one
+    target: list = [None] * K      # its last target (scaled)
+
+    def finish(k, found, xt):
+        r = _finish(milp, found, xt, stats[k], cfg.elapsed(), _emitter(logs[k], False),
+                    logs[k])
+        r.trajectory = k
+        r.seed = seeds[k]
+        return r
+
+    pending = {}
+    for k in range(K):
+        try:
+            pending[k] = gens[k].send(None)
+        except StopIteration as e:
+            found, xt = e.value
+            if found:
+                return finish(k, True, xt)
+    last = None
+    while pending:
+        results = {}
+        groups: dict = {}
+        for k, (kind, xt, params) in pending.items():
+            if kind == "target":
+                target[k] = xt / D_c
+            tight = min(params.eps_optimal_relative,
+                        params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR
+            if fp64[k] is None and tight:
+                st64 = _pdfp_mod.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb,
+                                           pp.var_ub, pp.w, precision="fp64", device=device)
+                if mixed:
+                    _pdfp_mod.original_primal_residual(st64, pp.D_r)
+                st64.set_x(batch.pdhg_x(k))
+                _pdfp_mod.set_target(st64, target[k])
+                fp64[k] = st64
+                kind = "started"
+            if fp64[k] is not None:
+                st64 = fp64[k]
+                if kind == "target":
+                    _pdfp_mod.set_target(st64, target[k])
+                elif kind == "tighten":
+                    _pdfp_mod.warm_start(st64)
+                _pdfp_mod.pdfp_solve(st64, params, primal_weight)
+                results[k] = (D_c * st64.pdhg_x, st64.total_count)
+            else:
+                key = (params.eps_optimal_relative, params.eps_feasible_relative)
+                groups.setdefault(key, []).append(k)
+        for ks in groups.values():
+            active = np.zeros(K, dtype=np.uint8)
+            for k in ks:
+                kind, _, _ = pending[k]
+                if kind == "target":
+                    batch.set_target(k, target[k])
+                else:
+                    batch.warm_start(k)
+                active[k] = 1
+            p0 = pending[ks[0]][2]
+            params = cfg.params_cls(iteration_limit=p0.iteration_limit,
+                                    eps_optimal_relative=p0.eps_optimal_relative,
+                                    eps_feasible_relative=p0.eps_feasible_relative,
+                                    time_limit=min(pending[k][2].time_limit for k in ks))
+            batch.solve(params, primal_weight, active)
+            for k in ks:
+                results[k] = (D_c * batch.pdhg_x(k), batch.total_count(k))
+        new_pending = {}
+        for k in sorted(results):
+            try:
+                new_pending[k] = gens[k].send(results[k])
+            except StopIteration as e:
+                found, xt = e.value
+                if found:
+                    return finish(k, True, xt)
+                last = (k, xt)
+        pending = new_pending
+    k, xt = last if last is not None else (0, x)
+    return finish(k, False, xt)
 
 
 def _print_iteration(emit, iteration, distance, violation, inner, gap, action, t):

@@ -40,3 +40,29 @@ def test_p0201_log_matches_reference_readme():
     assert "first rounding violates by 9)" in log
     it1 = [line for line in r.log if re.match(r"\s+1\s", line)][0].split()
     assert it1[1] == "8.08613" and it1[2] == "1" and it1[3] == "600"
+
+
+@pytest.mark.parametrize("name", ["p0201", "gen-ip054", "pk1", "markshare2"])
+def test_batched_pump_finds_feasible(name):
+    """K = 8 trajectories (seeds 100k + 1) on one batched projector, the
+    first feasible point wins (replaces scripts/solve.jl:121-148)."""
+    from pdfp_amd.pump import feasibility_pump_batch
+
+    print("batched pump", name, flush=True)
+    m = read_mps(example_path(name))
+    r = feasibility_pump_batch(m, trajectories=8, time_limit=120.0)
+    assert r.found and is_feasible(m, r.x)
+    assert 0 <= r.trajectory < 8 and r.seed == 100 * (r.trajectory + 1) + 1
+    assert abs(r.objective - (float(m.c @ r.x) + m.c0)) <= 1e-9 * (1 + abs(r.objective))
+
+
+def test_batched_pump_mixed_model_takes_fp64_projections():
+    """A mixed model whose integral projections are tightened below the FP32
+    form's reach moves those trajectories to FP64 projectors."""
+    from pdfp_amd import generators as G
+    from pdfp_amd.pump import feasibility_pump_batch
+
+    print("batched pump mixed", flush=True)
+    m = G.mixed(2000, 4000, 1)
+    r = feasibility_pump_batch(m, trajectories=4, time_limit=120.0)
+    assert r.found and is_feasible(m, r.x)
