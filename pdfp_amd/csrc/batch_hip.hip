@@ -74,8 +74,8 @@ struct Batch {
     At.upload(ht, stream);
     // eight gathered values per nonzero: the staged products need 64 bytes
     // each, so blocks hold 256 or 512 nonzeros (autotune() picks)
-    A.schedule(256, stream);
-    At.schedule(256, stream);
+    A.schedule(256, stream, kBlock / kBatch);
+    At.schedule(256, stream, kBlock / kBatch);
     tr.resize((size_t)K);
     h_w.assign((size_t)n, 0.0);
     for (int64_t j = 0; j < n; ++j) {
@@ -218,6 +218,9 @@ struct Batch {
       }
       return;
     }
+    // the lane epilogues serve a block's (row, trajectory) pairs in one pass
+    if (M.max_rows > kBlock / kBatch)
+      throw Error(PDFP_ERR_STATE, "batch schedule holds too many rows per block");
     if (M.npb == 512) launch_pass_npb<512, 0, 0>(M, g, e, nullptr, stream, t0, t1);
     else launch_pass_npb<256, 0, 0>(M, g, e, nullptr, stream, t0, t1);
   }
@@ -246,8 +249,8 @@ struct Batch {
     for (int na : {256, 512})
       for (int nt : {256, 512})
         for (int persist : {0, 4}) {
-          A.schedule(na, stream);
-          At.schedule(nt, stream);
+          A.schedule(na, stream, kBlock / kBatch);
+          At.schedule(nt, stream, kBlock / kBatch);
           A.persist = At.persist = persist;
           for (int w = 0; w < 2; ++w) step(2, false);
           PDFP_HIP(hipEventRecord(e0, stream));
@@ -265,8 +268,8 @@ struct Batch {
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    A.schedule(bA, stream);
-    At.schedule(bAt, stream);
+    A.schedule(bA, stream, kBlock / kBatch);
+    At.schedule(bAt, stream, kBlock / kBatch);
     A.persist = At.persist = bP;
   }
   void step(int k_offset, bool major) {

@@ -72,16 +72,15 @@ struct GatherBatch {
   }
 };
 
-// Column j: the fused primal and dual-y step of every active trajectory
-// (src/pdfp.jl:119-143), s[k] = (Aᵀ z_k)_j.
+// Column j, trajectory q (lane q of the row's 8): the fused primal and dual-y
+// step (src/pdfp.jl:119-143) with s = (Aᵀ z_q)_j.  An inactive trajectory
+// is left untouched.  MAJOR steps also store pdhg x, the dual slack, pdhg y
+// and reflected y.
 template <bool MAJOR>
 struct EpiBatchPrimalY {
-  // x and y load at the start of the pass; x⁰, y⁰ and x̃ load in the
-  // epilogue (prefetching them too costs registers, hence occupancy, and
-  // measured slower).
+  static constexpr int kLanes = kBatch;
   struct Pre {
-    F8 x, y;
-    float lb, ub, w;
+    float x, y, lb, ub, w;
   };
   const BatchStep* sp;
   int k_offset;
@@ -98,68 +97,38 @@ struct EpiBatchPrimalY {
   float* __restrict__ ds;
   float* __restrict__ py;
   float* __restrict__ ry;
-  __device__ __forceinline__ Pre pre(int j) const {
-    const size_t o = (size_t)j * kBatch;
-    return {load8(x + o), load8(y + o), lb[j], ub[j], w[j]};
+  __device__ __forceinline__ Pre pre(int j, int q) const {
+    const size_t o = (size_t)j * kBatch + q;
+    return {x[o], y[o], lb[j], ub[j], w[j]};
   }
-  __device__ __forceinline__ void operator()(int j, const Pre& p, const double* s,
+  __device__ __forceinline__ void operator()(int j, int q, const Pre& p, double s,
                                              double*) const {
-    const size_t o = (size_t)j * kBatch;
-    const F8 px0 = load8(x0 + o), py0 = load8(y0 + o), pxt = load8(xt + o);
-    F8 nx = p.x, ny = p.y, nr, npx, nds, npy, nry;
-    const double lo = (double)p.lb, hi = (double)p.ub, wj = (double)p.w;
-#pragma unroll
-    for (int q = 0; q < kBatch; ++q) {
-      const BatchStep& st = sp[q];
-      const StepScalars c = batch_scalars(st, k_offset);
-      const double cx = (double)p.x.v[q], cy = (double)p.y.v[q];
-      const double temp = cx - c.tau * (s[q] - cy);
-      const double pxq = jclamp(temp, lo, hi);
-      const double rxq = 2.0 * pxq - cx;
-      const double pyq = wj > 0.0 ? jclamp(cy + c.sigma * ((double)pxt.v[q] - rxq), -wj, wj) : 0.0;
-      const double ryq = 2.0 * pyq - cy;
-      const bool on = st.active != 0.0;
-      nr.v[q] = on ? (float)rxq : p.x.v[q];
-      if (on) {
-        nx.v[q] = (float)(c.weight * (c.omega * rxq + c.omw * cx) + c.omwt * (double)px0.v[q]);
-        ny.v[q] = (float)(c.weight * (c.omega * ryq + c.omw * cy) + c.omwt * (double)py0.v[q]);
-      }
-      npx.v[q] = (float)pxq;
-      nds.v[q] = (float)((pxq - temp) / c.tau);
-      npy.v[q] = (float)pyq;
-      nry.v[q] = (float)ryq;
-    }
-    store8(x + o, nx);
-    store8(y + o, ny);
-    store8(rx + o, nr);
+    const BatchStep& st = sp[q];
+    if (st.active == 0.0) return;
+    const size_t o = (size_t)j * kBatch + q;
+    const StepScalars c = batch_scalars(st, k_offset);
+    const PrimalYOut r = primal_y_math(c, s, (double)p.x, (double)p.y, (double)x0[o],
+                                       (double)y0[o], (double)p.lb, (double)p.ub,
+                                       (double)xt[o], (double)p.w);
+    x[o] = (float)r.x;
+    y[o] = (float)r.y;
+    rx[o] = (float)r.rx;
     if (MAJOR) {
-      // an inactive trajectory keeps its pdhg values: merge per lane
-      F8 opx = load8(px + o), ods = load8(ds + o), opy = load8(py + o), ory = load8(ry + o);
-#pragma unroll
-      for (int q = 0; q < kBatch; ++q) {
-        if (sp[q].active != 0.0) {
-          opx.v[q] = npx.v[q];
-          ods.v[q] = nds.v[q];
-          opy.v[q] = npy.v[q];
-          ory.v[q] = nry.v[q];
-        }
-      }
-      store8(px + o, opx);
-      store8(ds + o, ods);
-      store8(py + o, opy);
-      store8(ry + o, ory);
+      px[o] = (float)r.px;
+      ds[o] = (float)r.ds;
+      py[o] = (float)r.py;
+      ry[o] = (float)r.ry;
     }
   }
 };
 
-// Row i: the dual-z step of every active trajectory (src/pdfp.jl:146-156),
-// s[k] = (A x̄_k)_i.
+// Row i, trajectory q: the dual-z step (src/pdfp.jl:146-156) with
+// s = (A x̄_q)_i.
 template <bool MAJOR>
 struct EpiBatchDualZ {
+  static constexpr int kLanes = kBatch;
   struct Pre {
-    F8 z, z0;
-    float b;
-    uint8_t eq;
+    float z;
   };
   const BatchStep* sp;
   int k_offset;
@@ -169,40 +138,18 @@ struct EpiBatchDualZ {
   const uint8_t* __restrict__ is_eq;
   float* __restrict__ pz;
   float* __restrict__ rz;
-  __device__ __forceinline__ Pre pre(int i) const {
-    const size_t o = (size_t)i * kBatch;
-    return {load8(z + o), load8(z0 + o), b[i], is_eq[i]};
-  }
-  __device__ __forceinline__ void operator()(int i, const Pre& p, const double* s,
+  __device__ __forceinline__ Pre pre(int i, int q) const { return {z[(size_t)i * kBatch + q]}; }
+  __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
                                              double*) const {
-    const size_t o = (size_t)i * kBatch;
-    F8 nz = p.z, npz, nrz;
-    const double bi = (double)p.b;
-#pragma unroll
-    for (int q = 0; q < kBatch; ++q) {
-      const BatchStep& st = sp[q];
-      const StepScalars c = batch_scalars(st, k_offset);
-      const double cz = (double)p.z.v[q];
-      const double raw = cz + c.sigma * (s[q] - bi);
-      const double pzq = p.eq ? raw : (raw > 0.0 ? raw : 0.0);
-      const double rzq = 2.0 * pzq - cz;
-      if (st.active != 0.0)
-        nz.v[q] = (float)(c.weight * (c.omega * rzq + c.omw * cz) + c.omwt * (double)p.z0.v[q]);
-      npz.v[q] = (float)pzq;
-      nrz.v[q] = (float)rzq;
-    }
-    store8(z + o, nz);
+    const BatchStep& st = sp[q];
+    if (st.active == 0.0) return;
+    const size_t o = (size_t)i * kBatch + q;
+    const StepScalars c = batch_scalars(st, k_offset);
+    const DualZOut r = dual_z_math(c, s, (double)p.z, (double)z0[o], (double)b[i], is_eq[i] != 0);
+    z[o] = (float)r.z;
     if (MAJOR) {
-      F8 opz = load8(pz + o), orz = load8(rz + o);
-#pragma unroll
-      for (int q = 0; q < kBatch; ++q) {
-        if (sp[q].active != 0.0) {
-          opz.v[q] = npz.v[q];
-          orz.v[q] = nrz.v[q];
-        }
-      }
-      store8(pz + o, opz);
-      store8(rz + o, orz);
+      pz[o] = (float)r.pz;
+      rz[o] = (float)r.rz;
     }
   }
 };

@@ -183,6 +183,15 @@ struct raw_stage<G, decltype((void)G::kRawStage)> : std::integral_constant<bool,
 
 struct NoPre {};
 
+// An epilogue may instead serve each row with kLanes threads, one per
+// gathered value (kLanes == Gather::NV): Pre pre(int row, int q) and
+// operator()(int row, int q, const Pre&, double sum_q, double* acc).  The
+// batch of trajectories uses it: lane q updates trajectory q of the row.
+template <class E, class = void>
+struct epi_lanes : std::integral_constant<int, 0> {};
+template <class E>
+struct epi_lanes<E, decltype((void)E::kLanes)> : std::integral_constant<int, E::kLanes> {};
+
 // ---- the pass ------------------------------------------------------------------
 
 // One row block b of the schedule, processed by the whole workgroup.
@@ -201,8 +210,11 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   double(*stage)[NPB] = reinterpret_cast<double(*)[NPB]>(stage_mem);
   float* const sraw = reinterpret_cast<float*>(stage_mem);
   float* const sa = sraw + NPB * NV;
-  __shared__ double rowsum[NV][BLOCK];
-  __shared__ int off[BLOCK + 1];
+  // A lane epilogue serves every (row, value) pair of a block in one pass of
+  // the workgroup: its schedule holds at most BLOCK / kLanes rows.
+  constexpr int MAXROWS = epi_lanes<Epi>::value > 0 ? BLOCK / epi_lanes<Epi>::value : BLOCK;
+  __shared__ double rowsum[NV][MAXROWS];
+  __shared__ int off[MAXROWS + 1];
   __shared__ double red[BLOCK / kWave];
 
   const int tid = threadIdx.x;
@@ -215,8 +227,14 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   const int nrows = r1 - r0;
 
   // The epilogue's operands: in flight while the row slice streams in.
+  constexpr int EL = epi_lanes<Epi>::value;
+  static_assert(EL == 0 || EL == NV, "a lane epilogue takes one lane per gathered value");
   typename Epi::Pre pre{};
-  if (tid < nrows) pre = epi.pre(r0 + tid);
+  if constexpr (EL > 0) {
+    if (tid < nrows * EL) pre = epi.pre(r0 + tid / EL, tid % EL);
+  } else {
+    if (tid < nrows) pre = epi.pre(r0 + tid);
+  }
 
   double acc[NA];
 #pragma unroll
@@ -288,7 +306,13 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
       }
     }
     __syncthreads();
-    if (tid < nrows) {
+    if constexpr (EL > 0) {
+      for (int e = tid; e < nrows * EL; e += BLOCK) {
+        const int lr = e / EL, q = e % EL;
+        const typename Epi::Pre p = e == tid ? pre : epi.pre(r0 + lr, q);
+        epi(r0 + lr, q, p, rowsum[q][lr], acc);
+      }
+    } else if (tid < nrows) {
       double s[NV];
 #pragma unroll
       for (int q = 0; q < NV; ++q) s[q] = rowsum[q][tid];
@@ -321,7 +345,16 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
     double t[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) t[q] = block_sum_to0<BLOCK>(s[q], red);
-    if (tid == 0) epi(r0, pre, t, acc);
+    if constexpr (EL > 0) {
+      if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) rowsum[q][0] = t[q];
+      }
+      __syncthreads();
+      if (tid < EL) epi(r0, tid, pre, rowsum[tid][0], acc);
+    } else if (tid == 0) {
+      epi(r0, pre, t, acc);
+    }
   }
   if constexpr (NACC > 0) {
     block_store_partials<BLOCK, NACC, NMAX>(acc, partials + (size_t)b * NACC);

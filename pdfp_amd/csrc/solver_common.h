@@ -121,6 +121,7 @@ struct DevCsr {
   int32_t nblocks = 0, nrows = 0;
   int32_t npb = 512;  // nonzeros per stream-mode block of the current schedule
   int32_t persist = 0;  // 0: one workgroup per block; k: k workgroups per CU, looping
+  int32_t max_rows = kBlock;  // rows per block of the current schedule, at most
   int64_t nnz = 0;
   std::vector<int32_t> h_rowptr;  // kept for re-scheduling
 
@@ -142,9 +143,10 @@ struct DevCsr {
 
   // Row blocks of at most kBlock rows and `max_nnz` nonzeros, with their
   // lanes per row.
-  void schedule(int max_nnz, hipStream_t s) {
+  void schedule(int max_nnz, hipStream_t s, int rows = kBlock) {
     npb = max_nnz;
-    std::vector<int32_t> blk = row_blocks(h_rowptr, kBlock, max_nnz);
+    max_rows = rows;
+    std::vector<int32_t> blk = row_blocks(h_rowptr, rows, max_nnz);
     std::vector<uint8_t> lanes = block_lanes(h_rowptr, blk, kBlock, max_nnz);
     nblocks = (int32_t)blk.size() - 1;
     std::vector<int4> rec(blk.size());
