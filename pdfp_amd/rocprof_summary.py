@@ -88,27 +88,35 @@ def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str,
     ff = copy_bytes / (_copy_kib(copy_fetch) * 1024.0)
     fw = copy_bytes / (_copy_kib(copy_write) * 1024.0)
     F, W = _per_kernel(fetch), _per_kernel(write)
-    # per epilogue: the instantiation with the most dispatches (the tuned one)
-    best = {}
-    for name, vals in F.items():
-        key = pass_key(name)
-        if key is None or name not in W:
-            continue
-        if key not in best or len(vals) > len(F[best[key]]):
-            best[key] = name
+
+    def tuned(counts: dict) -> dict:
+        # per epilogue, the instantiation with the most dispatches (the tuned
+        # schedule; the two counter runs may tune to different block sizes)
+        best: dict = {}
+        for name, vals in counts.items():
+            key = pass_key(name)
+            if key is not None and (key not in best or len(vals) > len(counts[best[key]])):
+                best[key] = name
+        return best
+
+    bf, bw = tuned(F), tuned(W)
     recs = {}
     lines = [f"copy calibration: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f} "
              f"(1 GiB copy / counter reading)", "",
-             "| epilogue | dispatches | FETCH KiB | WRITE KiB | HBM MB per launch |",
+             "| epilogue | dispatches (fetch / write run) | FETCH KiB | WRITE KiB | "
+             "HBM MB per launch |",
              "|---|---:|---:|---:|---:|"]
-    for key, name in sorted(best.items(), key=lambda kv: -len(F[kv[1]])):
-        fk = sum(F[name]) / len(F[name])
-        wk = sum(W[name]) / len(W[name])
+    for key in sorted(set(bf) & set(bw), key=lambda k: -len(F[bf[k]])):
+        nf, nw = bf[key], bw[key]
+        fk = sum(F[nf]) / len(F[nf])
+        wk = sum(W[nw]) / len(W[nw])
         t = fk * 1024.0 * ff + wk * 1024.0 * fw
-        recs[key] = {"kernel": short_name(name), "dispatches": len(F[name]),
+        recs[key] = {"kernel_fetch_run": short_name(nf), "kernel_write_run": short_name(nw),
+                     "dispatches": [len(F[nf]), len(W[nw])],
                      "fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
                      "traffic_bytes": round(t)}
-        lines.append(f"| `{key}` | {len(F[name])} | {fk:.0f} | {wk:.0f} | {t / 1e6:.2f} |")
+        lines.append(f"| `{key}` | {len(F[nf])} / {len(W[nw])} | {fk:.0f} | {wk:.0f} | "
+                     f"{t / 1e6:.2f} |")
     lines += ["", "```json", json.dumps(recs, indent=1), "```"]
     return "\n".join(lines)
 
