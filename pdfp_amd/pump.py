@@ -16,6 +16,7 @@ Differences that cannot be avoided without Julia:
 from __future__ import annotations
 
 import math
+import os
 import sys
 import time
 from dataclasses import dataclass, field
@@ -583,6 +584,9 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             if found:
                 return finish(k, True, xt)
     last = None
+    from concurrent.futures import ThreadPoolExecutor
+
+    pool = ThreadPoolExecutor(max_workers=max(1, min(K, os.cpu_count() or 1)))
     while pending:
         results = {}
         groups: dict = {}
@@ -628,16 +632,29 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             batch.solve(params, primal_weight, active)
             for k in ks:
                 results[k] = (D_c * batch.pdhg_x(k), batch.total_count(k))
-        new_pending = {}
-        for k in sorted(results):
+        # The trajectories round concurrently: the C++ rounder releases the
+        # interpreter while it works.  Among those that find a feasible point
+        # in the same round, the lowest index wins.
+        def advance(k):
             try:
-                new_pending[k] = gens[k].send(results[k])
+                return k, gens[k].send(results[k]), None
             except StopIteration as e:
-                found, xt = e.value
-                if found:
-                    return finish(k, True, xt)
-                last = (k, xt)
+                return k, None, e.value
+
+        new_pending = {}
+        winner = None
+        for k, req, end in pool.map(advance, sorted(results)):
+            if end is None:
+                new_pending[k] = req
+            elif end[0]:
+                winner = winner if winner is not None else (k, end[1])
+            else:
+                last = (k, end[1])
+        if winner is not None:
+            pool.shutdown()
+            return finish(winner[0], True, winner[1])
         pending = new_pending
+    pool.shutdown()
     k, xt = last if last is not None else (0, x)
     return finish(k, False, xt)
 
