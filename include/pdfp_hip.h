@@ -325,6 +325,19 @@ int pdfp_round_guarded(pdfp_rounder* r, const double* x_star, const int64_t* ran
 const char* pdfp_rounder_last_error(const pdfp_rounder* r);
 void pdfp_rounder_destroy(pdfp_rounder* r);
 
+/* ---- the pump's feasibility check on the device ---------------------------
+ * max_violation(milp, x) (src/milp.jl:102-114) on the original A kept
+ * resident in HBM (FP64 CSR): each call uploads x[n] and returns the largest
+ * violation of A x ≤ b (is_le = 1) / A x = b (is_le = 0), bit-identical to
+ * the reference's (each row summed in the CSC product's order). */
+typedef struct pdfp_checker pdfp_checker;
+int pdfp_checker_create(pdfp_checker** out, int64_t m, int64_t n, const int64_t* colptr,
+                        const int64_t* rowval, const double* nzval, int32_t index_base,
+                        const double* b, const uint8_t* is_le, int32_t device);
+int pdfp_checker_max_violation(pdfp_checker* c, const double* x, double* max_violation);
+const char* pdfp_checker_last_error(const pdfp_checker* c);
+void pdfp_checker_destroy(pdfp_checker* c);
+
 /* ---- preconditioning ------------------------------------------------------
  * scale_problem (src/preconditioner.jl:26-63), the matrix part, on the
  * device: `ruiz_iterations` Ruiz rounds and one Pock–Chambolle pass

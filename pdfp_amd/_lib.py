@@ -145,6 +145,11 @@ SIGNATURES = {
                                      C.POINTER(C.c_int32)]),
     "pdfp_rounder_last_error": (C.c_char_p, [_P]),
     "pdfp_rounder_destroy": (None, [_P]),
+    "pdfp_checker_create": (C.c_int, [C.POINTER(_P), C.c_int64, C.c_int64, _I64, _I64, _D,
+                                      C.c_int32, _D, _U8, C.c_int32]),
+    "pdfp_checker_max_violation": (C.c_int, [_P, _D, _D]),
+    "pdfp_checker_last_error": (C.c_char_p, [_P]),
+    "pdfp_checker_destroy": (None, [_P]),
     "pdfp_probe_last_error": (C.c_char_p, []),
 }
 

@@ -105,6 +105,51 @@ def max_violation(milp: MILP, x) -> float:
     return float(v)
 
 
+class DeviceViolationCheck:
+    """max_violation(milp, x) on the device (pdfp_checker_*): the original A
+    stays in HBM; each call uploads x and reads back one scalar, bit-identical
+    to the reference's max_violation (src/milp.jl:102-114; a NaN activity
+    never raises the maximum, as there)."""
+
+    def __init__(self, milp: MILP, device: int = 0):
+        import ctypes as C
+
+        from . import _lib
+
+        self._L = _lib
+        self._lib = lib = _lib.load()
+        A = sp.csc_matrix(milp.A, dtype=np.float64)
+        self.n = A.shape[1]
+        colptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        rowval = np.ascontiguousarray(A.indices, dtype=np.int64)
+        nzval = np.ascontiguousarray(A.data, dtype=np.float64)
+        b = np.ascontiguousarray(milp.b, dtype=np.float64)
+        is_le = np.ascontiguousarray(milp.row_sense == "le", dtype=np.uint8)
+        ctx = C.c_void_p()
+        st = lib.pdfp_checker_create(C.byref(ctx), A.shape[0], self.n, _lib.i64ptr(colptr),
+                                     _lib.i64ptr(rowval), _lib.dptr(nzval), 0, _lib.dptr(b),
+                                     _lib.u8ptr(is_le), device)
+        _lib.check(st, lib.pdfp_checker_last_error, None)
+        self._ctx = ctx
+        self._out = C.c_double()
+        self._C = C
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            self._lib.pdfp_checker_destroy(ctx)
+            self._ctx = None
+
+    def __call__(self, x) -> float:
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        if len(xs) != self.n:
+            raise ValueError("x has the wrong length")
+        st = self._lib.pdfp_checker_max_violation(self._ctx, self._L.dptr(xs),
+                                                  self._C.byref(self._out))
+        self._L.check(st, self._lib.pdfp_checker_last_error, self._ctx)
+        return float(self._out.value)
+
+
 def _csc_matvec(A: sp.csc_matrix, x: np.ndarray) -> np.ndarray:
     """y = A x in the column order of SparseArrays' CSC product: for each
     column j, y[row] += a * x[j], accumulated in that order (np.bincount

@@ -18,12 +18,14 @@ from __future__ import annotations
 import math
 import os
 import sys
+import threading
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from .milp import MILP, is_integral, max_violation, num_constraints, num_continuous, num_variables
+from .milp import (MILP, DeviceViolationCheck, is_integral, max_violation, num_constraints,
+                   num_continuous, num_variables)
 from .rounding import BatchPolicy, Rounder, jround
 
 # Smallest relative KKT tolerance handed to the FP32 projector form.
@@ -239,6 +241,10 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                   gap_decay, gap_floor, residual_ratio, projection_iteration_limit,
                   batch_policy, push_threshold, violation_threshold, propagation_budget,
                   feasibility_tolerance, _pdfp_mod.SolverParameters, t0)
+    # the device pump checks its points on the device (the original A stays
+    # in HBM); with substituted solvers the check runs on the host
+    cfg.violation = (DeviceViolationCheck(milp, device) if projector is None
+                     else (lambda v: max_violation(milp, v)))
     mixed = num_continuous(milp) > 0
     stats = _Counters()
     log: list = []
@@ -315,6 +321,7 @@ class _Config:
     feasibility_tolerance: float
     params_cls: type
     t0: float
+    violation: object = None   # x -> max_violation(milp, x)
 
     def elapsed(self) -> float:
         return time.perf_counter() - self.t0
@@ -349,7 +356,7 @@ def _root(milp, cfg: _Config, solve_lp, root_lp_tolerance, stats, emit):
          f"{stats.root_lp_time:.2f} s\n")
     if is_integral(x, milp.int_idx, cfg.feasibility_tolerance):
         xt = round_to_nearest(x, milp)
-        if max_violation(milp, xt) <= cfg.feasibility_tolerance:
+        if cfg.violation(xt) <= cfg.feasibility_tolerance:
             return x, dual_slack, xt
     return x, dual_slack, None
 
@@ -409,7 +416,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
     emit(f"{'iter':>6s} {'distance':>12s} {'violation':>10s} {'pdhg':>10s} {'gap tol':>9s} "
          f"{'action':>8s} {'time':>9s}\n")
     while cfg.elapsed() < cfg.time_limit:
-        violation = max_violation(milp, xt)
+        violation = cfg.violation(xt)
         if violation <= tol and is_integral(xt, int_idx, tol):
             return True, xt.copy()
         stats.iterations += 1
@@ -428,7 +435,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
         # infeasible only through its continuous part (src/pump.jl:340-356).
         while (mixed and tightening_budget > 1e-6 and cfg.elapsed() < cfg.time_limit
                and is_integral(x, int_idx, tol)
-               and max_violation(milp, round_to_nearest(x, milp)) > tol):
+               and cfg.violation(round_to_nearest(x, milp)) > tol):
             tightening_budget *= 0.1
             gap = max(gap * 0.1, 1e-8)
             params = params_cls(iteration_limit=cfg.projection_iteration_limit,
@@ -447,7 +454,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
 
         if is_integral(x, int_idx, tol):
             candidate = round_to_nearest(x, milp)
-            if max_violation(milp, candidate) <= tol:
+            if cfg.violation(candidate) <= tol:
                 _print_iteration(emit, iteration, distance, violation, inner, gap_used,
                                  "integral", cfg.elapsed())
                 return True, candidate
@@ -485,7 +492,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
         if iteration <= 10 or iteration % 10 == 0:
             _print_iteration(emit, iteration, distance, violation, inner, gap_used, action,
                              cfg.elapsed())
-    found = max_violation(milp, xt) <= tol and is_integral(xt, int_idx, tol)
+    found = cfg.violation(xt) <= tol and is_integral(xt, int_idx, tol)
     return found, xt
 
 
@@ -526,6 +533,14 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                   gap_decay, gap_floor, residual_ratio, projection_iteration_limit,
                   batch_policy, push_threshold, violation_threshold, propagation_budget,
                   feasibility_tolerance, _pdfp_mod.SolverParameters, t0)
+    checker = DeviceViolationCheck(milp, device)
+    lock = threading.Lock()
+
+    def violation(v):
+        with lock:              # one checker, trajectories on several threads
+            return checker(v)
+
+    cfg.violation = violation
     mixed = num_continuous(milp) > 0
     root_stats = _Counters()
     root_log: list = []
