@@ -72,3 +72,46 @@ def test_batch_inactive_trajectory_is_untouched_and_warm_starts():
     # the same targets in all slots give the same projection
     l1 = [bt.info(k).l1_distance for k in (0, 2, 3)]
     assert max(l1) - min(l1) <= 1e-6 * max(1.0, max(l1))
+
+
+def _with_dense_column(m, value=0.1):
+    """m plus one continuous column in every row: a row of Aᵀ longer than
+    any block, so both batched passes take their long-row (vector) path."""
+    import scipy.sparse as sp
+
+    from pdfp_amd.milp import MILP
+
+    rows = m.A.shape[0]
+    col = sp.csc_matrix(np.full((rows, 1), value))
+    A = sp.hstack([m.A, col], format="csc")
+    return MILP(c=np.append(m.c, 0.0), c0=m.c0, A=A, b=m.b, row_sense=m.row_sense,
+                l=np.append(m.l, 0.0), u=np.append(m.u, 1.0), int_idx=m.int_idx,
+                name=m.name + "_dense_col")
+
+
+def test_batch_long_rows_and_columns():
+    """Power-law rows (up to 3000 nonzeros) and a dense column: the
+    long-row path of both passes, with 5 trajectories."""
+    print("batch long rows", flush=True)
+    m = _with_dense_column(G.mixed(2000, 5000, 3, power_law=True, nnz=60_000, max_len=3000))
+    pp = projector_problem(m)
+    assert np.diff(pp.A.tocsr().indptr).max() > 2048 and np.diff(pp.A.indptr).max() >= 2000
+    K = 5
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    bt = _batch(pp, K)
+    targets = [G.binary_targets(m, 700 + k) / pp.D_c for k in range(K)]
+    for k in range(K):
+        bt.set_target(k, targets[k])
+    assert bt.solve(p, 1.0) == ["OPTIMAL"] * K
+    for k in range(K):
+        r = _oracle_kkt(pp, targets[k], bt.pdhg_x(k), bt.vector(k, "pdhg_y"),
+                        bt.vector(k, "pdhg_z"), bt.vector(k, "dual_slack"))
+        assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (k, r)
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, targets[k])
+        assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
+        l1b, l1s = bt.info(k).l1_distance, s.info().l1_distance
+        assert abs(l1b - l1s) <= 1e-4 * max(1.0, l1s), (k, l1b, l1s)
