@@ -179,6 +179,23 @@ typedef struct pdfp_batch pdfp_batch;
 
 const char* pdfp_batch_last_error(const pdfp_batch* ctx);
 
+/* The continuous form of pdfp_batch_solve (no counterpart in the reference;
+ * the batched pump uses it): pdfp_batch_begin starts trajectory k's solve
+ * with its own parameters and primal weight (after pdfp_batch_set_target or
+ * pdfp_batch_warm_start), as pdfp_solve! does on entry
+ * (src/pdfp.jl:220-241).  pdfp_batch_run runs every begun, unfinished
+ * trajectory in lockstep blocks of termination_evaluation_frequency
+ * iterations (which, with the reflection coefficient, the running
+ * trajectories must share), each to its own termination, and returns after
+ * the first block in which one ends (until_any = 1) or when none runs
+ * (until_any = 0): termination[K] (0 for those still running),
+ * total_count[K], and the number still running.  A trajectory that has
+ * ended can be given a new target and begun again while the others run on. */
+int pdfp_batch_begin(pdfp_batch* ctx, int32_t k, const pdfp_params* params,
+                     double primal_weight);
+int pdfp_batch_run(pdfp_batch* ctx, int32_t until_any, int32_t* termination,
+                   int64_t* total_count, int32_t* n_running);
+
 /* As pdfp_create, for K ∈ 1..8 trajectories (FP32 form). */
 int pdfp_batch_create(pdfp_batch** out, int32_t K, int64_t m, int64_t n,
                       int64_t nnz, const int64_t* colptr, const int64_t* rowval,

@@ -127,6 +127,9 @@ SIGNATURES = {
     "pdfp_batch_warm_start": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_solve": (C.c_int, [_P, C.POINTER(PdfpParams), _D, _U8, C.POINTER(C.c_int32),
                                    _I64]),
+    "pdfp_batch_begin": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpParams), C.c_double]),
+    "pdfp_batch_run": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _I64,
+                                 C.POINTER(C.c_int32)]),
     "pdfp_batch_get_x": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_batch_get_vector": (C.c_int, [_P, C.c_int32, C.c_int32, _D]),
     "pdfp_batch_get_info": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpInfo)]),

@@ -34,6 +34,9 @@ struct Trajectory {
   double solve_seconds = 0.0;
   bool x64_valid = false;
   bool do_restart = false;
+  bool running = false;  // begun and not yet terminated
+  pdfp_params p{};       // the parameters of its current solve
+  double t_start = 0.0;
   std::vector<double> h_xt;
 };
 
@@ -325,47 +328,68 @@ struct Batch {
                             stream));
   }
 
-  // pdfp_solve! (src/pdfp.jl:252-288) for every trajectory with active[k],
-  // in lockstep blocks of `freq` iterations.
-  void solve(const pdfp_params& p, const double* pw, const uint8_t* act) {
-    const double t_start = wall_seconds();
+  // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for
+  // trajectory k: it starts a solve with its own parameters.
+  void begin(int k, const pdfp_params& p, double pw) {
+    check_k(k);
     if (p.termination_evaluation_frequency < 1)
       throw Error(PDFP_ERR_ARGUMENT, "termination_evaluation_frequency must be >= 1");
-    if (nnz > 0 && std::isnan(sv)) sv = power_iteration(A, At, m, n, p.sv_max_iterations,
-                                                       p.sv_tolerance, stream);
+    if (!(pw > 0.0)) throw Error(PDFP_ERR_ARGUMENT, "primal_weight must be > 0");
+    if (nnz > 0 && std::isnan(sv))
+      sv = power_iteration(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
     step_size = nnz == 0 ? 1.0 : 0.998 / std::sqrt(1.0 + sv * sv);
-    std::vector<bool> active((size_t)K, false);
-    for (int k = 0; k < K; ++k) {
-      if (act && !act[k]) continue;
-      if (!(pw[k] > 0.0)) throw Error(PDFP_ERR_ARGUMENT, "primal_weight must be > 0");
-      active[(size_t)k] = true;
-      Trajectory& t = tr[(size_t)k];
-      // initialize_step_size_and_primal_weight!  src/pdfp.jl:220-241
-      t.r.primal_weight = pw[k];
-      t.r.best_primal_weight = pw[k];
-      t.r.primal_weight_error_sum = 0.0;
-      t.r.primal_weight_last_error = 0.0;
-      double acc = 0.0;
-      for (int64_t j = 0; j < n; ++j)
-        acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(t.h_xt[(size_t)j]) : 0.0;
-      t.l1_den = 1.0 + acc;
-      t.do_restart = false;
-      t.n_restarts = 0;
-      t.n_evals = 0;
-      t.x64_valid = false;
-      if (t.r.total_count >= p.iteration_limit) {
-        t.termination = PDFP_ITERATION_LIMIT;
-        active[(size_t)k] = false;
+    Trajectory& t = tr[(size_t)k];
+    t.p = p;
+    t.t_start = wall_seconds();
+    t.r.primal_weight = pw;
+    t.r.best_primal_weight = pw;
+    t.r.primal_weight_error_sum = 0.0;
+    t.r.primal_weight_last_error = 0.0;
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; ++j)
+      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(t.h_xt[(size_t)j]) : 0.0;
+    t.l1_den = 1.0 + acc;
+    t.do_restart = false;
+    t.n_restarts = 0;
+    t.n_evals = 0;
+    t.x64_valid = false;
+    t.termination = PDFP_UNSPECIFIED;
+    t.running = true;
+    if (t.r.total_count >= p.iteration_limit) {
+      t.termination = PDFP_ITERATION_LIMIT;
+      t.running = false;
+      t.solve_seconds = 0.0;
+    }
+  }
+
+  // The block loop of pdfp_solve! (src/pdfp.jl:252-288) for every running
+  // trajectory, in lockstep blocks of `freq` iterations, each trajectory with
+  // its own termination; returns after the first block in which one ends
+  // (until_any) or when none runs.  Returns the number still running.
+  int run(bool until_any) {
+    int freq = 0;
+    double omega = 0.0;
+    for (const Trajectory& t : tr) {
+      if (!t.running) continue;
+      if (freq == 0) {
+        freq = t.p.termination_evaluation_frequency;
+        omega = t.p.reflection_coefficient;
+      } else if (t.p.termination_evaluation_frequency != freq ||
+                 t.p.reflection_coefficient != omega) {
+        throw Error(PDFP_ERR_ARGUMENT,
+                    "running trajectories must share the evaluation frequency and reflection");
       }
     }
-    const double omega = p.reflection_coefficient;
-    const int freq = p.termination_evaluation_frequency;
-    auto any = [&] {
-      for (bool a : active)
-        if (a) return true;
-      return false;
+    std::vector<bool> active((size_t)K, false);
+    auto refresh = [&] {
+      bool any = false;
+      for (int k = 0; k < K; ++k) {
+        active[(size_t)k] = tr[(size_t)k].running;
+        any |= active[(size_t)k];
+      }
+      return any;
     };
-    while (any()) {
+    while (refresh()) {
       upload_steps(omega, active);
       bool any_restart = false;
       for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && tr[(size_t)k].do_restart;
@@ -379,9 +403,11 @@ struct Batch {
                               hipMemcpyDeviceToHost, stream));
       sync();
       const double now = wall_seconds();
+      bool ended = false;
       for (int k = 0; k < K; ++k) {
         if (!active[(size_t)k]) continue;
         Trajectory& t = tr[(size_t)k];
+        const pdfp_params& p = t.p;
         ++t.n_evals;
         if (t.do_restart) {
           t.r.fixed_point_error = read_eval(k, 1).fpe;
@@ -402,7 +428,7 @@ struct Batch {
             t.relative_l1_gap <= p.eps_optimal_relative) {
           t.termination = PDFP_OPTIMAL;
           stop = true;
-        } else if (now - t_start > p.time_limit) {
+        } else if (now - t.t_start > p.time_limit) {
           t.termination = PDFP_TIME_LIMIT;
           stop = true;
         } else {
@@ -423,12 +449,26 @@ struct Batch {
           }
         }
         if (stop) {
-          active[(size_t)k] = false;
-          t.solve_seconds = now - t_start;
+          t.running = false;
+          t.solve_seconds = now - t.t_start;
+          ended = true;
         }
       }
+      if (until_any && ended) break;
     }
     sync();
+    int n_running = 0;
+    for (const Trajectory& t : tr) n_running += t.running ? 1 : 0;
+    return n_running;
+  }
+
+  // pdfp_solve! (src/pdfp.jl:252-288) for every trajectory with active[k]:
+  // each begins with primal weight pw[k], then they run to their ends.
+  void solve(const pdfp_params& p, const double* pw, const uint8_t* act) {
+    for (Trajectory& t : tr) t.running = false;
+    for (int k = 0; k < K; ++k)
+      if (!act || act[k]) begin(k, p, pw[k]);
+    run(false);
   }
 
   void polish(int k) {
@@ -662,6 +702,28 @@ int pdfp_batch_solve(pdfp_batch* ctx, const pdfp_params* params, const double* p
     if (params) p = *params;
     else pdfp_default_params(&p);
     ctx->impl->solve(p, primal_weight, active);
+    for (int k = 0; k < ctx->impl->K; ++k) {
+      if (termination) termination[k] = ctx->impl->tr[(size_t)k].termination;
+      if (total_count) total_count[k] = ctx->impl->tr[(size_t)k].r.total_count;
+    }
+  });
+}
+
+int pdfp_batch_begin(pdfp_batch* ctx, int32_t k, const pdfp_params* params,
+                     double primal_weight) {
+  return batch_guarded(ctx, [&] {
+    pdfp_params p;
+    if (params) p = *params;
+    else pdfp_default_params(&p);
+    ctx->impl->begin(k, p, primal_weight);
+  });
+}
+
+int pdfp_batch_run(pdfp_batch* ctx, int32_t until_any, int32_t* termination,
+                   int64_t* total_count, int32_t* n_running) {
+  return batch_guarded(ctx, [&] {
+    const int left = ctx->impl->run(until_any != 0);
+    if (n_running) *n_running = left;
     for (int k = 0; k < ctx->impl->K; ++k) {
       if (termination) termination[k] = ctx->impl->tr[(size_t)k].termination;
       if (total_count) total_count[k] = ctx->impl->tr[(size_t)k].r.total_count;

@@ -513,12 +513,14 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
     processes with seeds 100k + seed, each holding its own copy of the
     instance on the GPU: here the trajectories (seeds 100k + seed, k = 1..K)
     share one root LP and one batched FP32 projector (PDFPBatch, the matrix
-    in HBM once), and each round projects every trajectory that asks for a
-    projection in the same launches, then rounds each on the host.  A
-    trajectory asked for a tolerance tighter than the FP32 form resolves moves
-    to an FP64 projector of its own, as in feasibility_pump.  The result is
-    the winning trajectory's, with ``trajectory`` and ``seed`` set; its
-    ``time`` is the time to the first feasible point of the whole run."""
+    in HBM once).  The projector runs continuously: a trajectory whose
+    projection ends is rounded on a host thread while the others keep
+    iterating, and re-enters the batch with its next target at the next
+    evaluation block.  A trajectory asked for a tolerance tighter than the
+    FP32 form resolves moves to an FP64 projector of its own, as in
+    feasibility_pump.  The result is the winning trajectory's, with
+    ``trajectory`` and ``seed`` set; its ``time`` is the time to the first
+    feasible point of the whole run."""
     if not 1 <= trajectories <= 8:
         raise ValueError("trajectories must be in 1..8")
     if batch_policy not in ("cost", "always"):
@@ -599,13 +601,26 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             if found:
                 return finish(k, True, xt)
     last = None
-    from concurrent.futures import ThreadPoolExecutor
+    from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 
+    # Continuous batching: a trajectory whose projection ends is handed to a
+    # host thread (rounding, flips, restarts) while the device runs on with
+    # the others, and re-enters the batch with its next projection at the
+    # next block boundary.
     pool = ThreadPoolExecutor(max_workers=max(1, min(K, os.cpu_count() or 1)))
-    while pending:
-        results = {}
-        groups: dict = {}
-        for k, (kind, xt, params) in pending.items():
+
+    def advance(k, result):
+        try:
+            return gens[k].send(result), None
+        except StopIteration as e:
+            return None, e.value
+
+    running: set = set()
+    futures: dict = {}
+    winner = None
+    while pending or running or futures:
+        for k in sorted(pending):
+            kind, xt, params = pending[k]
             if kind == "target":
                 target[k] = xt / D_c
             tight = min(params.eps_optimal_relative,
@@ -626,49 +641,38 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                 elif kind == "tighten":
                     _pdfp_mod.warm_start(st64)
                 _pdfp_mod.pdfp_solve(st64, params, primal_weight)
-                results[k] = (D_c * st64.pdhg_x, st64.total_count)
+                futures[k] = pool.submit(advance, k, (D_c * st64.pdhg_x, st64.total_count))
             else:
-                key = (params.eps_optimal_relative, params.eps_feasible_relative)
-                groups.setdefault(key, []).append(k)
-        for ks in groups.values():
-            active = np.zeros(K, dtype=np.uint8)
-            for k in ks:
-                kind, _, _ = pending[k]
                 if kind == "target":
                     batch.set_target(k, target[k])
                 else:
                     batch.warm_start(k)
-                active[k] = 1
-            p0 = pending[ks[0]][2]
-            params = cfg.params_cls(iteration_limit=p0.iteration_limit,
-                                    eps_optimal_relative=p0.eps_optimal_relative,
-                                    eps_feasible_relative=p0.eps_feasible_relative,
-                                    time_limit=min(pending[k][2].time_limit for k in ks))
-            batch.solve(params, primal_weight, active)
-            for k in ks:
-                results[k] = (D_c * batch.pdhg_x(k), batch.total_count(k))
-        # The trajectories round concurrently: the C++ rounder releases the
-        # interpreter while it works.  Among those that find a feasible point
-        # in the same round, the lowest index wins.
-        def advance(k):
-            try:
-                return k, gens[k].send(results[k]), None
-            except StopIteration as e:
-                return k, None, e.value
-
-        new_pending = {}
-        winner = None
-        for k, req, end in pool.map(advance, sorted(results)):
-            if end is None:
-                new_pending[k] = req
-            elif end[0]:
-                winner = winner if winner is not None else (k, end[1])
-            else:
-                last = (k, end[1])
+                batch.begin(k, params, primal_weight)
+                running.add(k)
+        pending = {}
+        if running:
+            terms, _ = batch.run(until_any=True)
+            for k in sorted(running):
+                if terms[k] != "UNSPECIFIED":
+                    running.discard(k)
+                    futures[k] = pool.submit(advance, k,
+                                             (D_c * batch.pdhg_x(k), batch.total_count(k)))
+        if futures:
+            done = [k for k, f in futures.items() if f.done()]
+            if not done and not running:
+                wait(list(futures.values()), return_when=FIRST_COMPLETED)
+                done = [k for k, f in futures.items() if f.done()]
+            for k in sorted(done):
+                req, end = futures.pop(k).result()
+                if end is None:
+                    pending[k] = req
+                elif end[0]:
+                    winner = winner if winner is not None else (k, end[1])
+                else:
+                    last = (k, end[1])
         if winner is not None:
-            pool.shutdown()
+            pool.shutdown(wait=False, cancel_futures=True)
             return finish(winner[0], True, winner[1])
-        pending = new_pending
     pool.shutdown()
     k, xt = last if last is not None else (0, x)
     return finish(k, False, xt)

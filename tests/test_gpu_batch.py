@@ -115,3 +115,46 @@ def test_batch_long_rows_and_columns():
         assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
         l1b, l1s = bt.info(k).l1_distance, s.info().l1_distance
         assert abs(l1b - l1s) <= 1e-4 * max(1.0, l1s), (k, l1b, l1s)
+
+
+def test_batch_continuous_begin_and_run():
+    """pdfp_batch_begin / pdfp_batch_run: trajectories started at different
+    times, each with its own tolerance, end as their own solves would, and a
+    trajectory that ended can start again while another still runs."""
+    print("batch continuous", flush=True)
+    m = G.mixed(400, 800, 31)
+    pp = projector_problem(m)
+    bt = _batch(pp, 3)
+    tg = [G.binary_targets(m, 900 + k) / pp.D_c for k in range(3)]
+    loose = SolverParameters(eps_optimal_relative=1e-2, eps_feasible_relative=1e-3)
+    tight = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                             iteration_limit=200_000)
+    bt.set_target(0, tg[0])
+    bt.begin(0, tight, 1.0)
+    bt.set_target(1, tg[1])
+    bt.begin(1, loose, 1.0)
+    terms, left = bt.run(until_any=True)
+    assert terms[1] == "OPTIMAL" and terms[0] == "UNSPECIFIED" and left == 1
+    n1 = bt.total_count(1)
+    bt.set_target(2, tg[2])                       # joins while 0 still runs
+    bt.begin(2, tight, 1.0)
+    while left:
+        terms, left = bt.run(until_any=True)
+    assert terms[0] == terms[2] == "OPTIMAL" and bt.total_count(1) == n1
+    for k in (0, 2):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, tg[k])
+        assert P.pdfp_solve(s, tight, 1.0) == "OPTIMAL"
+        assert abs(bt.info(k).l1_distance - s.info().l1_distance) <= \
+            1e-4 * max(1.0, s.info().l1_distance)
+    bad = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                           termination_evaluation_frequency=100)
+    bt.set_target(0, tg[0])
+    bt.begin(0, tight, 1.0)
+    bt.set_target(1, tg[1])
+    bt.begin(1, bad, 1.0)
+    from pdfp_amd._lib import LibraryError
+
+    with pytest.raises(LibraryError, match="share"):
+        bt.run()
