@@ -32,7 +32,6 @@ struct Trajectory {
   int termination = PDFP_UNSPECIFIED;
   int n_restarts = 0, n_evals = 0;
   double solve_seconds = 0.0;
-  bool x64_valid = false;
   bool do_restart = false;
   bool running = false;  // begun and not yet terminated
   pdfp_params p{};       // the parameters of its current solve
@@ -61,6 +60,7 @@ struct Batch {
   bool orig_primal = false;
   double sv = NAN, step_size = 0.0;
   std::vector<Trajectory> tr;
+  int x64_owner = -1;  // the trajectory whose polished point x64 holds, or -1
   std::string err = "no error";
 
   Batch(int K_, int64_t m_, int64_t n_, const int64_t* colptr, const int64_t* rowval,
@@ -199,7 +199,7 @@ struct Batch {
     t.r.fixed_point_error = INFINITY;
     t.r.last_trial_fixed_point_error = INFINITY;
     t.termination = PDFP_UNSPECIFIED;
-    t.x64_valid = false;
+    x64_owner = -1;
   }
 
   void set_target(int k, const double* xts) {
@@ -352,9 +352,9 @@ struct Batch {
     t.do_restart = false;
     t.n_restarts = 0;
     t.n_evals = 0;
-    t.x64_valid = false;
     t.termination = PDFP_UNSPECIFIED;
     t.running = true;
+    x64_owner = -1;
     if (t.r.total_count >= p.iteration_limit) {
       t.termination = PDFP_ITERATION_LIMIT;
       t.running = false;
@@ -380,6 +380,7 @@ struct Batch {
                     "running trajectories must share the evaluation frequency and reflection");
       }
     }
+    x64_owner = -1;  // the blocks below move the running trajectories' points
     std::vector<bool> active((size_t)K, false);
     auto refresh = [&] {
       bool any = false;
@@ -473,7 +474,7 @@ struct Batch {
 
   void polish(int k) {
     Trajectory& t = tr[(size_t)k];
-    if (t.x64_valid || n == 0) return;
+    if (x64_owner == k || n == 0) return;
     xt64.upload(t.h_xt.data(), (size_t)n, stream);
     const int g = grid_for(n);
     k_batch_polish<<<g, 256, 0, stream>>>((int)n, k, px.p, lb.p, ub.p, lb64.p, ub64.p, xt64.p,
@@ -483,7 +484,7 @@ struct Batch {
                             hipMemcpyDeviceToHost, stream));
     sync();
     t.l1_distance = h_sums.p[kBatch * 32];
-    t.x64_valid = true;
+    x64_owner = k;
   }
 
   void get_x(int k, double* out) {
@@ -491,8 +492,6 @@ struct Batch {
     polish(k);
     if (n) x64.download(out, (size_t)n, stream);
     sync();
-    // the polish buffer is shared: the next polish recomputes
-    tr[(size_t)k].x64_valid = false;
   }
 
   void get_vector(int k, int which, double* out) {
@@ -513,10 +512,7 @@ struct Batch {
   void info(int k, pdfp_info* o) {
     check_k(k);
     Trajectory& t = tr[(size_t)k];
-    if (t.termination != PDFP_UNSPECIFIED) {
-      polish(k);
-      t.x64_valid = false;
-    }
+    if (t.termination != PDFP_UNSPECIFIED) polish(k);
     std::memset(o, 0, sizeof(*o));
     o->relative_primal_residual = t.r.relative_primal_residual;
     o->relative_dual_residual = t.r.relative_dual_residual;

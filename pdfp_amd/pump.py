@@ -66,19 +66,29 @@ class _Counters:
 
 
 def _differs(a, b, idx) -> bool:
-    return any(a[j] != b[j] for j in idx)
+    idx = np.asarray(idx, dtype=np.int64)
+    return bool(np.any(np.asarray(a)[idx] != np.asarray(b)[idx]))
+
+
+def _distance(x, xt, idx) -> float:
+    """Σ_{j∈I} |x_j - x̃_j| accumulated in index order, as the reference's
+    loop (np.cumsum adds sequentially)."""
+    if len(idx) == 0:
+        return 0.0
+    d = np.abs(np.asarray(x)[idx] - np.asarray(xt)[idx])
+    return float(np.cumsum(d)[-1])
 
 
 def round_to_nearest(x, milp: MILP) -> np.ndarray:
-    """Nearest integer on I, kept inside the bounds (src/pump.jl:53-62)."""
+    """Nearest integer on I, kept inside the bounds (src/pump.jl:53-62);
+    element-wise, so vectorised exactly."""
     xt = np.array(x, dtype=np.float64)
-    for j in milp.int_idx.tolist():
-        v = jround(x[j])
-        if math.isfinite(milp.l[j]) and v < milp.l[j]:
-            v = float(math.ceil(milp.l[j]))
-        if math.isfinite(milp.u[j]) and v > milp.u[j]:
-            v = float(math.floor(milp.u[j]))
-        xt[j] = v
+    I = milp.int_idx
+    v = np.rint(xt[I])
+    lo, hi = milp.l[I], milp.u[I]
+    v = np.where(np.isfinite(lo) & (v < lo), np.ceil(lo), v)
+    v = np.where(np.isfinite(hi) & (v > hi), np.floor(hi), v)
+    xt[I] = v
     return xt
 
 
@@ -130,10 +140,12 @@ def flip_farthest(xt: np.ndarray, x, milp: MILP, TT: int) -> None:
     """FP's stalling move: flip the TT integer variables farthest from x
     (src/pump.jl:121-135)."""
     l, u = milp.l, milp.u
-    distances = [(abs(x[j] - xt[j]), j) for j in milp.int_idx.tolist()]
-    distances.sort(key=lambda t: t[0], reverse=True)           # stable, as sort!(rev=true)
-    for k in range(min(TT, len(distances))):
-        d, j = distances[k]
+    I = milp.int_idx
+    dist = np.abs(np.asarray(x)[I] - xt[I])
+    # descending, ties in index order: sort!(rev = true) is stable
+    order = np.argsort(-dist, kind="stable")
+    for k in range(min(TT, len(order))):
+        d, j = float(dist[order[k]]), int(I[order[k]])
         if d == 0.0:
             continue
         cur = xt[j]
@@ -180,19 +192,19 @@ def perturb(xt: np.ndarray, x, milp: MILP, rng, general: list, is_general: np.nd
             v = jround(hi if v > hi else (lo if v < lo else v))
             if abs(v - xt[j]) > 1e-6:
                 xt[j] = v
-    for j in milp.int_idx.tolist():
-        if is_general[j]:
-            continue
-        rho = -0.3 + float(rng.random())
-        if abs(x[j] - xt[j]) + max(rho, 0.0) > 0.5:
-            cur = xt[j]
-            direction = 1.0 if x[j] >= cur else -1.0
-            v = cur + direction
-            if v < l[j] or v > u[j]:
-                v = cur - direction
-            if v < l[j] or v > u[j]:
-                continue
-            xt[j] = v
+    # the binaries: one draw each, in index order (Generator.random(k) gives
+    # the k draws of k calls); every update is element-wise
+    J = milp.int_idx[~is_general[milp.int_idx]]
+    if len(J):
+        rho = -0.3 + rng.random(len(J))
+        xJ, cur = np.asarray(x)[J], xt[J]
+        flip = np.abs(xJ - cur) + np.maximum(rho, 0.0) > 0.5
+        direction = np.where(xJ >= cur, 1.0, -1.0)
+        v = cur + direction
+        out = (v < l[J]) | (v > u[J])
+        v = np.where(out, cur - direction, v)
+        ok = flip & ~((v < l[J]) | (v > u[J]))
+        xt[J[ok]] = v[ok]
 
 
 def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
@@ -448,9 +460,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
         if gap > cfg.gap_floor:
             gap = max(gap * cfg.gap_decay, cfg.gap_floor)
         stats.projection_time += time.perf_counter() - t
-        distance = 0.0
-        for j in int_list:
-            distance += abs(x[j] - xt[j])
+        distance = _distance(x, xt, int_idx)
 
         if is_integral(x, int_idx, tol):
             candidate = round_to_nearest(x, milp)

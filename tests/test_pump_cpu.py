@@ -144,3 +144,95 @@ def test_tight_projections_leave_the_fp32_form():
     if switched:
         assert all(prec == "fp64" for prec, _ in calls[switched[0]:])
         assert any("FP64 form" in line for line in r.log)
+
+
+# ---- the vectorised host steps against the reference's element loops ----------
+
+def _loop_round_to_nearest(x, milp):
+    """src/pump.jl:53-62, element by element."""
+    import math
+
+    xt = np.array(x, dtype=np.float64)
+    for j in milp.int_idx.tolist():
+        v = float(np.rint(x[j]))
+        if math.isfinite(milp.l[j]) and v < milp.l[j]:
+            v = float(math.ceil(milp.l[j]))
+        if math.isfinite(milp.u[j]) and v > milp.u[j]:
+            v = float(math.floor(milp.u[j]))
+        xt[j] = v
+    return xt
+
+
+def _loop_flip_farthest(xt, x, milp, TT):
+    """src/pump.jl:121-135, element by element."""
+    l, u = milp.l, milp.u
+    distances = [(abs(x[j] - xt[j]), j) for j in milp.int_idx.tolist()]
+    distances.sort(key=lambda t: t[0], reverse=True)
+    for k in range(min(TT, len(distances))):
+        d, j = distances[k]
+        if d == 0.0:
+            continue
+        cur = xt[j]
+        direction = 1.0 if x[j] >= cur else -1.0
+        v = cur + direction
+        if v < l[j] or v > u[j]:
+            v = cur - direction
+        if v < l[j] or v > u[j]:
+            continue
+        xt[j] = v
+
+
+def _loop_perturb_binaries(xt, x, milp, rng, is_general):
+    """The binary part of src/pump.jl:149-187, element by element."""
+    l, u = milp.l, milp.u
+    for j in milp.int_idx.tolist():
+        if is_general[j]:
+            continue
+        rho = -0.3 + float(rng.random())
+        if abs(x[j] - xt[j]) + max(rho, 0.0) > 0.5:
+            cur = xt[j]
+            direction = 1.0 if x[j] >= cur else -1.0
+            v = cur + direction
+            if v < l[j] or v > u[j]:
+                v = cur - direction
+            if v < l[j] or v > u[j]:
+                continue
+            xt[j] = v
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_vectorised_pump_steps_equal_element_loops(seed):
+    from pdfp_amd.pump import RestartLadder, _differs, _distance, perturb
+
+    rng = np.random.default_rng(seed)
+    m = G.general_integer(60, 120, seed)
+    m.u[::7] = np.inf                        # some unbounded integers
+    m.l[::11] = -np.inf
+    n = len(m.c)
+    I = m.int_idx
+    for _ in range(20):
+        x = rng.normal(2.0, 3.0, n)
+        x[::5] = np.round(x[::5]) + 0.5      # exact ties
+        a, b = round_to_nearest(x, m), _loop_round_to_nearest(x, m)
+        assert np.array_equal(a, b)
+        xt = np.round(rng.normal(2.0, 3.0, n))
+        d_loop = 0.0
+        for j in I.tolist():
+            d_loop += abs(x[j] - xt[j])
+        assert _distance(x, xt, I) == d_loop
+        assert _differs(a, xt, I.tolist()) == any(a[j] != xt[j] for j in I.tolist())
+        assert not _differs(a, a.copy(), I)
+        f1, f2 = xt.copy(), xt.copy()
+        flip_farthest(f1, x, m, 15)
+        _loop_flip_farthest(f2, x, m, 15)
+        assert np.array_equal(f1, f2)
+        # perturb with no general integers touches only the binary part
+        mb = G.set_cover(40, 70, seed)
+        xb = rng.random(70)
+        tb = np.round(rng.random(70))
+        isg = np.zeros(70, dtype=bool)
+        p1, p2 = tb.copy(), tb.copy()
+        r1, r2 = np.random.default_rng(seed), np.random.default_rng(seed)
+        perturb(p1, xb, mb, r1, [], isg, RestartLadder(0.0, 0, 20, 10), 5)
+        _loop_perturb_binaries(p2, xb, mb, r2, isg)
+        assert np.array_equal(p1, p2) and r1.random() == r2.random()
