@@ -13,10 +13,12 @@ KKT tolerance 1e-4, the way the pump calls the projector
 (src/pump.jl:329-359).  The scaled matrix, its transpose and every
 iterate stay in HBM; the timed region covers set_target + pdfp_solve.
 
-value = Σ PDHG iterations / Σ projection wall time of the batched product
-form: 8 trajectories sharing the matrix in HBM, FP32 data and iterates,
-fused launches, FP64 reductions (``--batch 1``: a single trajectory), the
-reference's own accounting (src/pump.jl:357-359).  The single-trajectory
+value = Σ PDHG iterations / projection wall time of the batched product
+form: 8 trajectory slots sharing the matrix in HBM, FP32 data and iterates,
+fused launches, FP64 reductions, serving the step's projections as a queue
+(a slot that finishes takes the next target while the others run on;
+``--batch 1``: a single trajectory), the reference's own accounting
+(src/pump.jl:357-359).  The single-trajectory
 FP32 form and the plain FP64 form (the reference's launch structure) run the
 same kind of projections and are reported beside it.
 
@@ -119,25 +121,41 @@ def kernel_bytes_batch(m: int, n: int, nnz: int, K: int = 8) -> tuple[float, flo
 
 
 def run_batch(bt, pp, milp, target_sets, tol):
-    """Each step: K warm-started projections, one per trajectory, solved
-    together.  Returns (Σ PDHG iterations, seconds, statuses)."""
+    """The K trajectory slots serve a queue of warm-started projections, the
+    targets of all steps in order (step s, slot k): a slot whose projection
+    ends takes the next target at the next evaluation block while the others
+    run on (pdfp_batch_begin / pdfp_batch_run), so no slot idles while
+    another finishes.  Returns (Σ PDHG iterations, seconds, statuses)."""
     from pdfp_amd.pdfp import SolverParameters
 
-    iters = 0
-    t_total = 0.0
-    statuses = []
     pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
     p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
                          iteration_limit=500_000)
-    for targets in target_sets:
-        t0 = time.perf_counter()
-        for k, xt in enumerate(targets):
-            bt.set_target(k, xt / pp.D_c)
-        st = bt.solve(p, pw)
-        t_total += time.perf_counter() - t0
-        iters += sum(bt.total_count(k) for k in range(bt.K))
-        statuses.append(st)
-    return iters, t_total, statuses
+    queue = [xt for targets in target_sets for xt in targets]
+    iters = 0
+    statuses = []
+    t0 = time.perf_counter()
+    running = set()
+    nxt = 0
+    for k in range(min(bt.K, len(queue))):
+        bt.set_target(k, queue[nxt] / pp.D_c)
+        bt.begin(k, p, pw)
+        running.add(k)
+        nxt += 1
+    while running:
+        terms, _ = bt.run(until_any=True)
+        for k in sorted(running):
+            if terms[k] == "UNSPECIFIED":
+                continue
+            running.discard(k)
+            iters += bt.last_total_count[k]
+            statuses.append(terms[k])
+            if nxt < len(queue):
+                bt.set_target(k, queue[nxt] / pp.D_c)
+                bt.begin(k, p, pw)
+                running.add(k)
+                nxt += 1
+    return iters, time.perf_counter() - t0, statuses
 
 
 def cpu_baseline(pp, milp, target, iterations: int):
@@ -354,8 +372,9 @@ def main():
         bt.close()
         head = results["batch"]
         bat, ba = kernel_bytes_batch(mm, nn, nnz)
-        steps_desc = (f"{K} warm-started l1 projections (one per trajectory, 8-slot batch "
-                      f"sharing A) to new 0/1 targets at relative KKT tolerance {args.tol:g}")
+        steps_desc = (f"{K} warm-started l1 projections to new integer targets at relative KKT "
+                      f"tolerance {args.tol:g}, served by the {K} trajectory slots of one batch "
+                      f"sharing A (a slot that finishes takes the next target)")
     else:
         head = results["fp32"]
         bat, ba = kernel_bytes_fp32(mm, nn, nnz)

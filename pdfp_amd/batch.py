@@ -104,6 +104,7 @@ class PDFPBatch:
         self._check(self._lib.pdfp_batch_run(self._ctx, int(until_any), term, total,
                                              C.byref(left)))
         self.termination_reason = [_lib.TERMINATION[int(t)] for t in term]
+        self.last_total_count = [int(t) for t in total]
         return list(self.termination_reason), int(left.value)
 
     def pdhg_x(self, k: int) -> np.ndarray:
