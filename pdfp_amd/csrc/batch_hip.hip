@@ -133,8 +133,10 @@ struct Batch {
     for (int q = 0; q < kBatch; ++q) h_sp.p[q] = BatchStep{1.0, 1.0, 1.0, 0.0, 0.0, {0, 0, 0}};
     for (auto& t : tr) t.h_xt.assign((size_t)n, 0.0);
     autotune();
-    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
-    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
+    part_t.alloc((size_t)std::max<int64_t>((n + kEvalRows - 1) / kEvalRows, 1) * kBatch *
+                 kEvalColTerms);
+    part_a.alloc((size_t)std::max<int64_t>((m + kEvalRows - 1) / kEvalRows, 1) * kBatch *
+                 kEvalRowTerms);
     PDFP_HIP(hipStreamSynchronize(stream));
   }
 
@@ -280,17 +282,26 @@ struct Batch {
     else step_t<false>(k_offset);
   }
 
-  void evaluate(int k, int slot) {
-    EpiBatchEvalCol ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, k};
-    launch_pass<kEvalColTerms>(At, GatherDeltaStrided{pz.p, rz.p, k}, ec, part_t.p, stream);
-    EpiBatchEvalRow er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr, k};
-    launch_pass<kEvalRowTerms>(A, Gather1Strided{px.p, k}, er, part_a.p, stream);
-    double* out = sums.p + (size_t)k * 32 + (size_t)slot * 16;
-    if (At.nblocks)
-      finalize_partials<256><<<1, 256, 0, stream>>>(part_t.p, At.nblocks, kEvalColTerms, 0, out);
-    if (A.nblocks)
-      finalize_partials<256><<<1, 256, 0, stream>>>(part_a.p, A.nblocks, kEvalRowTerms, 0,
-                                                    out + 8);
+  // The evaluation of every trajectory in `mask` into its slot, in one pass
+  // over each operator (the 8 trajectories share the gathers).
+  void evaluate_all(int mask, int slot) {
+    if (!mask) return;
+    const int nbt = (int)((n + kEvalRows - 1) / kEvalRows);
+    const int nba = (int)((m + kEvalRows - 1) / kEvalRows);
+    EpiBatchEvalCol ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, 0};
+    EpiBatchEvalRow er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr, 0};
+    if (nbt) {
+      k_batch_eval_col<<<nbt, 256, 0, stream>>>((int)n, At.rowptr.p, At.col.p, At.val.p, pz.p,
+                                                 rz.p, ec, mask, part_t.p);
+      k_batch_eval_finalize<<<kBatch * kEvalColTerms, 256, 0, stream>>>(
+          part_t.p, nbt, kEvalColTerms, mask, slot * 16, sums.p);
+    }
+    if (nba) {
+      k_batch_eval_row<<<nba, 256, 0, stream>>>((int)m, A.rowptr.p, A.col.p, A.val.p, px.p, er,
+                                                 mask, part_a.p);
+      k_batch_eval_finalize<<<kBatch * kEvalRowTerms, 256, 0, stream>>>(
+          part_a.p, nba, kEvalRowTerms, mask, slot * 16 + 8, sums.p);
+    }
   }
 
   struct Eval {
@@ -395,11 +406,15 @@ struct Batch {
       bool any_restart = false;
       for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && tr[(size_t)k].do_restart;
       step(1, any_restart || freq == 1);
-      for (int k = 0; k < K; ++k)
-        if (active[(size_t)k] && tr[(size_t)k].do_restart) evaluate(k, 1);
+      int restart_mask = 0, active_mask = 0;
+      for (int k = 0; k < K; ++k) {
+        if (!active[(size_t)k]) continue;
+        active_mask |= 1 << k;
+        if (tr[(size_t)k].do_restart) restart_mask |= 1 << k;
+      }
+      evaluate_all(restart_mask, 1);
       for (int i = 2; i <= freq; ++i) step(i, i == freq);
-      for (int k = 0; k < K; ++k)
-        if (active[(size_t)k]) evaluate(k, 0);
+      evaluate_all(active_mask, 0);
       PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
                               hipMemcpyDeviceToHost, stream));
       sync();

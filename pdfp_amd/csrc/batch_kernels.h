@@ -154,29 +154,7 @@ struct EpiBatchDualZ {
   }
 };
 
-// ---- evaluation of one trajectory k (strided views of the batch) ------------
-
-struct Gather1Strided {
-  static constexpr int NV = 1;
-  const float* __restrict__ x;
-  int k;
-  __device__ __forceinline__ void operator()(int c, double a, double* p) const {
-    p[0] = a * (double)x[(size_t)c * kBatch + k];
-  }
-};
-
-struct GatherDeltaStrided {
-  static constexpr int NV = 2;
-  const float* __restrict__ x;
-  const float* __restrict__ r;
-  int k;
-  __device__ __forceinline__ void operator()(int c, double a, double* p) const {
-    const size_t o = (size_t)c * kBatch + k;
-    const double xc = (double)x[o];
-    p[0] = a * xc;
-    p[1] = a * (xc - (double)r[o]);
-  }
-};
+// ---- the evaluation terms of trajectory k (views of the interleaved vectors) --
 
 // The terms of EpiEvalCol (pdfp_kernels.h) for trajectory k.
 struct EpiBatchEvalCol {
@@ -241,6 +219,94 @@ struct EpiBatchEvalRow {
     acc[3] += ez * ez;
   }
 };
+
+// ---- evaluation of several trajectories in one pass ---------------------------
+// One workgroup serves 32 consecutive rows (of Aᵀ: columns; of A: rows), lane
+// (row, q) = (tid / 8, tid % 8) forms trajectory q's sums over the row in
+// FP64, in ascending column order, and adds q's evaluation terms (the
+// EpiBatchEvalCol / EpiBatchEvalRow functors) when bit q of `mask` is set.
+// Each workgroup writes per-trajectory partials partial[block][q][t]; the
+// sums of the 8 trajectories share every 32-byte gather.
+constexpr int kEvalRows = 256 / kBatch;
+
+template <int NT>
+__device__ __forceinline__ void batch_eval_store(double (&acc)[NT], double* __restrict__ out) {
+  __shared__ double red[256 / kWave][kBatch][NT];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    double v = acc[t];
+    // lanes with the same q = lane % 8: offsets 8, 16, 32
+    for (int o = kBatch; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
+    if (lane < kBatch) red[wid][lane][t] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kBatch * NT) {
+    const int q = threadIdx.x / NT, t = threadIdx.x % NT;
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < 256 / kWave; ++w) v += red[w][q][t];
+    out[threadIdx.x] = v;  // out[q * NT + t]
+  }
+}
+
+static __global__ __launch_bounds__(256) void k_batch_eval_col(
+    int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const float* __restrict__ pz, const float* __restrict__ rz,
+    EpiBatchEvalCol ec, int mask, double* __restrict__ partial) {
+  const int q = threadIdx.x % kBatch;
+  const int j = blockIdx.x * kEvalRows + threadIdx.x / kBatch;
+  double acc[kEvalColTerms];
+#pragma unroll
+  for (int t = 0; t < kEvalColTerms; ++t) acc[t] = 0.0;
+  if (j < nrows && ((mask >> q) & 1)) {
+    double s[2] = {0.0, 0.0};
+    for (int k = ptr[j]; k < ptr[j + 1]; ++k) {
+      const double a = (double)val[k];
+      const size_t o = (size_t)col[k] * kBatch + q;
+      const double zc = (double)pz[o];
+      s[0] += a * zc;
+      s[1] += a * (zc - (double)rz[o]);
+    }
+    ec.k = q;
+    ec(j, ec.pre(j), s, acc);
+  }
+  batch_eval_store<kEvalColTerms>(acc, partial + (size_t)blockIdx.x * kBatch * kEvalColTerms);
+}
+
+static __global__ __launch_bounds__(256) void k_batch_eval_row(
+    int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const float* __restrict__ px, EpiBatchEvalRow er, int mask,
+    double* __restrict__ partial) {
+  const int q = threadIdx.x % kBatch;
+  const int i = blockIdx.x * kEvalRows + threadIdx.x / kBatch;
+  double acc[kEvalRowTerms];
+#pragma unroll
+  for (int t = 0; t < kEvalRowTerms; ++t) acc[t] = 0.0;
+  if (i < nrows && ((mask >> q) & 1)) {
+    double s = 0.0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+      s += (double)val[k] * (double)px[(size_t)col[k] * kBatch + q];
+    er.k = q;
+    er(i, er.pre(i), &s, acc);
+  }
+  batch_eval_store<kEvalRowTerms>(acc, partial + (size_t)blockIdx.x * kBatch * kEvalRowTerms);
+}
+
+// out[q·32 + offset + t] = Σ_b partial[b][q][t] in block order, for every
+// trajectory q in mask: one workgroup per (q, t).
+static __global__ __launch_bounds__(256) void k_batch_eval_finalize(
+    const double* __restrict__ partial, int nb, int nt, int mask, int offset,
+    double* __restrict__ out) {
+  __shared__ double red[256 / kWave];
+  const int q = blockIdx.x / nt, t = blockIdx.x % nt;
+  if (!((mask >> q) & 1)) return;  // uniform over the workgroup
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) v += partial[((size_t)b * kBatch + q) * nt + t];
+  v = block_sum_to0<256>(v, red);
+  if (threadIdx.x == 0) out[q * 32 + offset + t] = v;
+}
 
 // perform_restart! for trajectory k: initial ← pdhg, current ← pdhg.
 static __global__ __launch_bounds__(256) void k_batch_restart(int n, int m, int k,
