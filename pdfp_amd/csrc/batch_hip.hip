@@ -54,6 +54,9 @@ struct Batch {
   DevBuf<double> part_t, part_a, part_v, sums;
   DevBuf<BatchStep> d_sp;
   PinnedBuf<BatchStep> h_sp;
+  DevBuf<double2> d_wt;    // Halpern weights [k_offset][8] of the current block
+  PinnedBuf<double2> h_wt;
+  int wt_rows = 0;
   PinnedBuf<double> h_sums;
   std::vector<double> h_w, h_b;
   double b_norm = 0.0, b_norm_original = NAN, dual_den = 1.0;
@@ -232,10 +235,10 @@ struct Batch {
 
   template <bool MAJOR>
   void step_t(int k_offset) {
-    EpiBatchPrimalY<MAJOR> ex{d_sp.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+    EpiBatchPrimalY<MAJOR> ex{d_sp.p, d_wt.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                               rx.p, px.p, ds.p, py.p, ry.p};
     launch_batch(At, GatherBatch{z.p}, ex);
-    EpiBatchDualZ<MAJOR> ez{d_sp.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    EpiBatchDualZ<MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
     launch_batch(A, GatherBatch{rx.p}, ez);
   }
 
@@ -324,19 +327,39 @@ struct Batch {
     return e;
   }
 
-  void upload_steps(double omega, const std::vector<bool>& active) {
+  // The block's per-trajectory scalars and Halpern weights for iterations
+  // k_offset = 0 .. freq (and at least 0 .. 199, for the timing helpers).
+  void upload_steps(double omega, const std::vector<bool>& active, int freq = 200) {
+    // the pinned staging and the table are rewritten below: no earlier copy
+    // or pass may still read them (the run loop's stream is idle here anyway)
+    sync();
     for (int q = 0; q < kBatch; ++q) {
       BatchStep& s = h_sp.p[q];
-      s.step_size = step_size;
+      const double pw = q < K ? tr[(size_t)q].r.primal_weight : 1.0;
+      s.tau = step_size / pw;
+      s.sigma = step_size * pw;
       s.omega = omega;
-      if (q < K) {
-        s.primal_weight = tr[(size_t)q].r.primal_weight;
-        s.inner_count = (double)tr[(size_t)q].r.inner_count;
-      }
+      s.omw = 1.0 - omega;
       s.active = (q < K && active[(size_t)q]) ? 1.0 : 0.0;
+    }
+    const int rows = std::max(freq, 199) + 1;
+    if (rows > wt_rows) {
+      d_wt.alloc((size_t)rows * kBatch);
+      h_wt.alloc((size_t)rows * kBatch);
+      wt_rows = rows;
+    }
+    for (int q = 0; q < kBatch; ++q) {
+      const double ic = q < K ? (double)tr[(size_t)q].r.inner_count : 0.0;
+      for (int ko = 0; ko < rows; ++ko) {
+        const double k = ic + (double)ko;
+        const double w = k / (k + 1.0);
+        h_wt.p[(size_t)ko * kBatch + q] = double2{w, 1.0 - w};
+      }
     }
     PDFP_HIP(hipMemcpyAsync(d_sp.p, h_sp.p, sizeof(BatchStep) * kBatch, hipMemcpyHostToDevice,
                             stream));
+    PDFP_HIP(hipMemcpyAsync(d_wt.p, h_wt.p, sizeof(double2) * (size_t)rows * kBatch,
+                            hipMemcpyHostToDevice, stream));
   }
 
   // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for
@@ -402,7 +425,7 @@ struct Batch {
       return any;
     };
     while (refresh()) {
-      upload_steps(omega, active);
+      upload_steps(omega, active, freq);
       bool any_restart = false;
       for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && tr[(size_t)k].do_restart;
       step(1, any_restart || freq == 1);
@@ -558,11 +581,11 @@ struct Batch {
     for (auto& e : ev) PDFP_HIP(hipEventCreate(&e));
     for (int i = 0; i < iters; ++i) {
       const int k = 2 + (i % 198);
-      EpiBatchPrimalY<false> ex{d_sp.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+      EpiBatchPrimalY<false> ex{d_sp.p, d_wt.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                                 rx.p, px.p, ds.p, py.p, ry.p};
       launch_batch(At, GatherBatch{z.p}, ex,
           ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
-      EpiBatchDualZ<false> ez{d_sp.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+      EpiBatchDualZ<false> ez{d_sp.p, d_wt.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
       launch_batch(A, GatherBatch{rx.p}, ez,
           ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
     }

@@ -17,24 +17,32 @@ namespace pdfp {
 
 constexpr int kBatch = 8;
 
+// Per-trajectory scalars of the current evaluation block (FP64, formed on
+// the host with the same operations as step_scalars: τ = step/pw,
+// σ = step·pw, 1 - ω), and a table of the Halpern weights of the block's
+// iterations: wt[k_offset·8 + q] = {w_k, 1 - w_k}, w_k = k/(k+1),
+// k = inner_count_q + k_offset (src/pdfp.jl:120-122).  No division is left
+// for the epilogues.
 struct BatchStep {
-  double step_size;
-  double primal_weight;
+  double tau;
+  double sigma;
   double omega;
-  double inner_count;
+  double omw;
   double active;  // 1: iterate this trajectory, 0: leave it unchanged
   double pad[3];
 };
 
-__device__ __forceinline__ StepScalars batch_scalars(const BatchStep& sp, int k_offset) {
+__device__ __forceinline__ StepScalars batch_scalars(const BatchStep& sp,
+                                                     const double2* __restrict__ wt,
+                                                     int k_offset, int q) {
   StepScalars s;
-  s.tau = sp.step_size / sp.primal_weight;
-  s.sigma = sp.step_size * sp.primal_weight;
-  const double k = sp.inner_count + (double)k_offset;
-  s.weight = k / (k + 1.0);
+  s.tau = sp.tau;
+  s.sigma = sp.sigma;
   s.omega = sp.omega;
-  s.omw = 1.0 - s.omega;
-  s.omwt = 1.0 - s.weight;
+  s.omw = sp.omw;
+  const double2 w = wt[(size_t)k_offset * 8 + q];
+  s.weight = w.x;
+  s.omwt = w.y;
   return s;
 }
 
@@ -83,6 +91,7 @@ struct EpiBatchPrimalY {
     float x, y, lb, ub, w;
   };
   const BatchStep* sp;
+  const double2* wt;
   int k_offset;
   float* __restrict__ x;
   const float* __restrict__ x0;
@@ -106,7 +115,7 @@ struct EpiBatchPrimalY {
     const BatchStep& st = sp[q];
     if (st.active == 0.0) return;
     const size_t o = (size_t)j * kBatch + q;
-    const StepScalars c = batch_scalars(st, k_offset);
+    const StepScalars c = batch_scalars(st, wt, k_offset, q);
     const PrimalYOut r = primal_y_math(c, s, (double)p.x, (double)p.y, (double)x0[o],
                                        (double)y0[o], (double)p.lb, (double)p.ub,
                                        (double)xt[o], (double)p.w);
@@ -131,6 +140,7 @@ struct EpiBatchDualZ {
     float z;
   };
   const BatchStep* sp;
+  const double2* wt;
   int k_offset;
   float* __restrict__ z;
   const float* __restrict__ z0;
@@ -144,7 +154,7 @@ struct EpiBatchDualZ {
     const BatchStep& st = sp[q];
     if (st.active == 0.0) return;
     const size_t o = (size_t)i * kBatch + q;
-    const StepScalars c = batch_scalars(st, k_offset);
+    const StepScalars c = batch_scalars(st, wt, k_offset, q);
     const DualZOut r = dual_z_math(c, s, (double)p.z, (double)z0[o], (double)b[i], is_eq[i] != 0);
     z[o] = (float)r.z;
     if (MAJOR) {
