@@ -365,7 +365,9 @@ def main():
                                 ms_iter=ms_iter,
                                 npb={"A": bi.npb_a, "At": bi.npb_at,
                                      "A_persistent_wg_per_cu": bi.persist_a,
-                                     "At_persistent_wg_per_cu": bi.persist_at})
+                                     "At_persistent_wg_per_cu": bi.persist_at,
+                                     "A_lane_rows": bi.lane_rows_a,
+                                     "At_lane_rows": bi.lane_rows_at})
         print(f"[bench] fp32 batch of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; batched iteration "
               f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)

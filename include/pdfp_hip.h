@@ -110,6 +110,9 @@ typedef struct pdfp_info {
   int32_t persist_at;        /*   grid of k workgroups per CU (autotuned)       */
   int32_t small_path;        /* 1: the last solve ran each block of iterations  */
                              /*   in one workgroup (operators held in LDS)      */
+  int32_t lane_rows_a;       /* batch: blocks whose rows hold at most this many */
+  int32_t lane_rows_at;      /*   nonzeros are reduced one lane per (row,       */
+                             /*   trajectory); 0: none (autotuned)              */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */
@@ -226,6 +229,13 @@ int pdfp_batch_time_iterations(pdfp_batch* ctx, int32_t iterations, double* ms_o
 /* Mean milliseconds of the batched transpose pass and of the batched A pass. */
 int pdfp_batch_time_kernels(pdfp_batch* ctx, int32_t iterations, double* ms_at,
                             double* ms_a);
+/* Replaces the autotuned schedule of the batched passes (no counterpart in
+ * the reference; for tests and tuning): nonzeros per workgroup of the A and
+ * Aᵀ passes (256 or 512), persistent workgroups per CU (0: one workgroup per
+ * row block), and the row-length bound of the lane-per-(row, trajectory)
+ * reduction of each pass (0: off). */
+int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int32_t persist,
+                            int32_t lane_rows_a, int32_t lane_rows_at);
 void pdfp_batch_destroy(pdfp_batch* ctx);
 
 /* ---- root LP (src/pdlp.jl) -------------------------------------------- */

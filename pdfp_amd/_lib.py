@@ -63,6 +63,8 @@ class PdfpInfo(C.Structure):
         ("persist_a", C.c_int32),
         ("persist_at", C.c_int32),
         ("small_path", C.c_int32),
+        ("lane_rows_a", C.c_int32),
+        ("lane_rows_at", C.c_int32),
     ]
 
 
@@ -135,6 +137,8 @@ SIGNATURES = {
     "pdfp_batch_get_info": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpInfo)]),
     "pdfp_batch_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_batch_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "pdfp_batch_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32]),
     "pdfp_batch_destroy": (None, [_P]),
     "pdfp_gather_ceiling": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, _D]),
     "pdfp_scale_matrix": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32,

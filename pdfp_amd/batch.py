@@ -127,6 +127,15 @@ class PDFPBatch:
     def total_count(self, k: int) -> int:
         return int(self.info(k).total_count)
 
+    def set_schedule(self, npb_a: int, npb_at: int, persist: int = 0, lane_rows_a: int = 0,
+                     lane_rows_at: int = 0) -> None:
+        """Replace the autotuned schedule of the batched passes (tests and
+        tuning): nonzeros per workgroup of the A and Aᵀ passes (256 or 512),
+        persistent workgroups per CU, and the row-length bound of each pass's
+        lane-per-(row, trajectory) reduction (0: off)."""
+        self._check(self._lib.pdfp_batch_set_schedule(self._ctx, npb_a, npb_at, persist,
+                                                      lane_rows_a, lane_rows_at))
+
     def time_kernels(self, iterations: int) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()
         self._check(self._lib.pdfp_batch_time_kernels(self._ctx, iterations, C.byref(a),

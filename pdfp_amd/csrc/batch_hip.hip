@@ -244,7 +244,9 @@ struct Batch {
 
   // Schedule autotuner of the batched passes: time whole iterations with
   // every trajectory inactive (the iterates keep their values) for each
-  // nonzeros-per-block of A and Aᵀ and each grid form; keep the fastest.
+  // nonzeros-per-block of A and Aᵀ and each grid form, then for each
+  // operator the row-length bounds of the lane-per-(row, trajectory)
+  // reduction; keep the fastest.
   void autotune() {
     if (nnz == 0) return;
     std::vector<bool> none((size_t)K, false);
@@ -252,21 +254,26 @@ struct Batch {
     hipEvent_t e0, e1;
     PDFP_HIP(hipEventCreate(&e0));
     PDFP_HIP(hipEventCreate(&e1));
+    auto time8 = [&] {
+      for (int w = 0; w < 2; ++w) step(2, false);
+      PDFP_HIP(hipEventRecord(e0, stream));
+      for (int r = 0; r < 8; ++r) step(2, false);
+      PDFP_HIP(hipEventRecord(e1, stream));
+      PDFP_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    constexpr int kRows = kBlock / kBatch;
     float best = 1e30f;
     int bA = 256, bAt = 256, bP = 0;
     for (int na : {256, 512})
       for (int nt : {256, 512})
         for (int persist : {0, 4}) {
-          A.schedule(na, stream, kBlock / kBatch);
-          At.schedule(nt, stream, kBlock / kBatch);
+          A.schedule(na, stream, kRows);
+          At.schedule(nt, stream, kRows);
           A.persist = At.persist = persist;
-          for (int w = 0; w < 2; ++w) step(2, false);
-          PDFP_HIP(hipEventRecord(e0, stream));
-          for (int r = 0; r < 8; ++r) step(2, false);
-          PDFP_HIP(hipEventRecord(e1, stream));
-          PDFP_HIP(hipEventSynchronize(e1));
-          float ms = 0.f;
-          PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
+          const float ms = time8();
           if (ms < best) {
             best = ms;
             bA = na;
@@ -274,11 +281,35 @@ struct Batch {
             bP = persist;
           }
         }
+    A.schedule(bA, stream, kRows);
+    At.schedule(bAt, stream, kRows);
+    A.persist = At.persist = bP;
+    best = time8();
+    for (DevCsr<float>* M : {&A, &At}) {
+      int bl = 0;
+      for (int len : {32, 128}) {
+        M->schedule(M->npb, stream, kRows, len);
+        const float ms = time8();
+        if (ms < best) {
+          best = ms;
+          bl = len;
+        }
+      }
+      M->schedule(M->npb, stream, kRows, bl);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    A.schedule(bA, stream, kBlock / kBatch);
-    At.schedule(bAt, stream, kBlock / kBatch);
-    A.persist = At.persist = bP;
+  }
+
+  void set_schedule(int npb_a, int npb_at, int persist, int lane_a, int lane_at) {
+    constexpr int kRows = kBlock / kBatch;
+    if ((npb_a != 256 && npb_a != 512) || (npb_at != 256 && npb_at != 512) || persist < 0 ||
+        persist > 16 || lane_a < 0 || lane_at < 0)
+      throw Error(PDFP_ERR_ARGUMENT, "bad schedule");
+    sync();
+    A.schedule(npb_a, stream, kRows, lane_a);
+    At.schedule(npb_at, stream, kRows, lane_at);
+    A.persist = At.persist = persist;
   }
   void step(int k_offset, bool major) {
     if (major) step_t<true>(k_offset);
@@ -571,6 +602,8 @@ struct Batch {
     o->npb_at = At.npb;
     o->persist_a = A.persist;
     o->persist_at = At.persist;
+    o->lane_rows_a = A.lane_len;
+    o->lane_rows_at = At.lane_len;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched
@@ -797,6 +830,13 @@ int pdfp_batch_time_kernels(pdfp_batch* ctx, int32_t iterations, double* ms_at, 
   return batch_guarded(ctx, [&] {
     if (!ms_at || !ms_a || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
     ctx->impl->time_kernels(iterations, ms_at, ms_a);
+  });
+}
+
+int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int32_t persist,
+                            int32_t lane_rows_a, int32_t lane_rows_at) {
+  return batch_guarded(ctx, [&] {
+    ctx->impl->set_schedule(npb_a, npb_at, persist, lane_rows_a, lane_rows_at);
   });
 }
 

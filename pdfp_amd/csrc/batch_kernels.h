@@ -72,11 +72,8 @@ struct GatherBatch {
 #pragma unroll
     for (int q = 0; q < kBatch; ++q) p[q] = a * (double)f.v[q];
   }
-  __device__ __forceinline__ void raw(int c, float* v) const {
-    const float4* src = reinterpret_cast<const float4*>(x + (size_t)c * kBatch);
-    float4* dst = reinterpret_cast<float4*>(v);
-    dst[0] = src[0];
-    dst[1] = src[1];
+  __device__ __forceinline__ float4 raw4(int c, int h) const {
+    return reinterpret_cast<const float4*>(x + (size_t)c * kBatch)[h];
   }
 };
 

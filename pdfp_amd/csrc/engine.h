@@ -48,7 +48,7 @@ struct Csr {
   const int32_t* __restrict__ rowptr;  // nrows + 1
   const int32_t* __restrict__ col;     // nnz
   const TV* __restrict__ val;          // nnz
-  // nblocks + 1 records {first row, first nonzero, lanes per row, 0}; the
+  // nblocks + 1 records {first row, first nonzero, lanes per row, mode}; the
   // last record is {nrows, nnz, 0, 0}.  One 16-byte load gives a block
   // everything it needs to start streaming.
   const int4* __restrict__ sched;
@@ -94,6 +94,21 @@ __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
   return v;
+}
+
+// Keeps v's words in registers, as loaded, until this point.  The epilogue
+// operands are loaded before the row slice streams in; without this the
+// compiler converts them right after their loads and so waits for them there,
+// before the streaming starts.
+template <class T>
+__device__ __forceinline__ void hold(T& v) {
+  if constexpr (sizeof(T) % 4 == 0) {
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) asm volatile("" : "+v"(w[i]));
+    __builtin_memcpy(&v, w, sizeof(T));
+  }
 }
 
 // Sum of v over the block, returned on thread 0 (other threads: garbage).
@@ -165,9 +180,11 @@ struct GatherDelta {
 };
 
 // A gather that sets `static constexpr bool kRawStage = true` also provides
-// raw(c, float* v), the NV gathered FP32 values of column c (v is 16-byte
-// aligned); the pass then stages them with the FP32 matrix value and forms
-// the products a·v[q] in FP64 during the row reduction.
+// raw4(c, h), the gathered FP32 values 4h .. 4h+3 of column c (NV is a
+// multiple of 4); the pass then stages them with the FP32 matrix value and
+// forms the products a·v[q] in FP64 during the row reduction.  NV/4
+// neighbouring lanes load one nonzero's values, so the NV·4 contiguous bytes
+// of a column come from a single instruction (one cache-line request).
 template <class G, class = void>
 struct raw_stage : std::false_type {};
 template <class G>
@@ -192,6 +209,15 @@ struct epi_lanes : std::integral_constant<int, 0> {};
 template <class E>
 struct epi_lanes<E, decltype((void)E::kLanes)> : std::integral_constant<int, E::kLanes> {};
 
+// Mode field (the record's fourth) of a stream-mode block reduced one lane
+// per (row, gathered value): lane (r, q) sums the products of row r with
+// value q in ascending column order, then runs the epilogue of (r, q) on its
+// own sum, with no shuffles and no second barrier.  Only lane epilogues over
+// raw staging read it; every other pass over the same schedule reduces the
+// block with its `tpr` lanes per row.  The host sets it on blocks whose rows
+// are all short.
+constexpr int kLaneRows = 1;
+
 // ---- the pass ------------------------------------------------------------------
 
 // One row block b of the schedule, processed by the whole workgroup.
@@ -204,17 +230,21 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   constexpr bool kRaw = raw_stage<Gather>::value;
   static_assert(!kRaw || (std::is_same<TV, float>::value && NV % 4 == 0),
                 "raw staging: FP32 matrix values, a multiple of 4 gathered values");
-  // products: double[NV][NPB]; raw: float[NPB][NV] values then float[NPB] a_ij
-  constexpr int kStageBytes = kRaw ? NPB * (NV + 1) * 4 : NV * NPB * 8;
+  // products: double[NV][NPB + 1]; raw: float[NPB + 1][NV] values then
+  // float[NPB + 1] a_ij.  Slot NPB takes the stores of lanes past the end of
+  // the slice, so the staging stores need no branch (a branch would let the
+  // compiler sink each load into it, one load in flight at a time).
+  constexpr int NS = NPB + 1;
+  constexpr int kStageBytes = kRaw ? NS * (NV + 1) * 4 : NV * NS * 8;
   __shared__ __align__(16) unsigned char stage_mem[kStageBytes];
-  double(*stage)[NPB] = reinterpret_cast<double(*)[NPB]>(stage_mem);
+  double(*stage)[NS] = reinterpret_cast<double(*)[NS]>(stage_mem);
   float* const sraw = reinterpret_cast<float*>(stage_mem);
-  float* const sa = sraw + NPB * NV;
+  float* const sa = sraw + NS * NV;
   // A lane epilogue serves every (row, value) pair of a block in one pass of
   // the workgroup: its schedule holds at most BLOCK / kLanes rows.
   constexpr int MAXROWS = epi_lanes<Epi>::value > 0 ? BLOCK / epi_lanes<Epi>::value : BLOCK;
   __shared__ double rowsum[NV][MAXROWS];
-  __shared__ int off[MAXROWS + 1];
+  __shared__ int off[MAXROWS + 2];  // + a pad slot for lanes past the last row
   __shared__ double red[BLOCK / kWave];
 
   const int tid = threadIdx.x;
@@ -241,82 +271,130 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   for (int k = 0; k < NA; ++k) acc[k] = 0.0;
 
   if (tpr > 0) {
-    for (int i = tid; i <= nrows; i += BLOCK) off[i] = A.rowptr[r0 + i] - base;
-    for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
-      int cc[U];
-      double vv[U];
+    // row offsets: loaded now, stored to LDS after the slice's loads are in
+    // flight (stream mode holds at most MAXROWS ≤ BLOCK rows; off[nrows] = cnt)
+    const int row_off = A.rowptr[r0 + (tid < nrows ? tid : nrows)] - base;
+    if constexpr (kRaw) {
+      // SPL lanes per nonzero, lane h of them staging values 4h .. 4h+3
+      constexpr int SPL = NV / 4;
+      constexpr int PER = BLOCK / SPL;
+      constexpr int UR = U * SPL;
+      const int h = tid % SPL;
+      const int t = tid / SPL;
+      for (int k0 = 0; k0 < cnt; k0 += PER * UR) {
+        int cc[UR];
+        float aa[UR];
+        // every load of the sweep in flight at once: past the end of the
+        // slice a lane reloads its last nonzero (never stored)
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = k0 + u * BLOCK + tid;
-        cc[u] = k < cnt ? ld_stream(A.col + base + k) : 0;
-        vv[u] = k < cnt ? (double)ld_stream(A.val + base + k) : 0.0;
+        for (int u = 0; u < UR; ++u) {
+          const int k = k0 + u * PER + t;
+          const int kc = k < cnt ? k : cnt - 1;
+          cc[u] = ld_stream(A.col + base + kc);
+          aa[u] = ld_stream(A.val + base + kc);
+        }
+        float4 gv[UR];
+#pragma unroll
+        for (int u = 0; u < UR; ++u) gv[u] = g.raw4(cc[u], h);
+#pragma unroll
+        for (int u = 0; u < UR; ++u) {
+          const int k = k0 + u * PER + t;
+          const int kw = k < cnt ? k : NPB;
+          reinterpret_cast<float4*>(sraw + kw * NV)[h] = gv[u];
+          sa[kw] = aa[u];  // the SPL lanes of a nonzero store the same value
+        }
       }
+    } else {
+      for (int k0 = 0; k0 < cnt; k0 += BLOCK * U) {
+        int cc[U];
+        double vv[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = k0 + u * BLOCK + tid;
-        if (k < cnt) {
-          if constexpr (kRaw) {
-            g.raw(cc[u], sraw + k * NV);
-            sa[k] = (float)vv[u];
-          } else {
-            double p[NV];
-            g(cc[u], vv[u], p);
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u * BLOCK + tid;
+          const int kc = k < cnt ? k : cnt - 1;
+          cc[u] = ld_stream(A.col + base + kc);
+          vv[u] = (double)ld_stream(A.val + base + kc);
+        }
+        double p[U][NV];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) stage[q][k] = p[q];
-          }
+        for (int u = 0; u < U; ++u) g(cc[u], vv[u], p[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u * BLOCK + tid;
+          const int kw = k < cnt ? k : NPB;
+#pragma unroll
+          for (int q = 0; q < NV; ++q) stage[q][kw] = p[u][q];
         }
       }
     }
+    off[tid < nrows ? tid : MAXROWS + 1] = row_off;
+    if (tid == 0) off[nrows] = cnt;
     __syncthreads();
-    const int groups = BLOCK / tpr;
-    const int grp = tid / tpr;
-    const int lane = tid & (tpr - 1);
-    // Every lane of a group walks the same rows, so the group stays
-    // converged around its shuffles.
-    for (int lr = grp; lr < nrows; lr += groups) {
-      double s[NV];
+    bool lane_rows = false;
+    if constexpr (EL > 0 && kRaw) lane_rows = sb.w == kLaneRows;
+    if (lane_rows) {
+      if constexpr (EL > 0 && kRaw) {
+        if (tid < nrows * EL) {
+          const int lr = tid / EL, q = tid % EL;
+          double s = 0.0;
+          const int k1 = off[lr + 1];
+          for (int k = off[lr]; k < k1; ++k) s += (double)sa[k] * (double)sraw[k * NV + q];
+          hold(pre);
+          epi(r0 + lr, q, pre, s, acc);
+        }
+      }
+    } else {
+      const int groups = BLOCK / tpr;
+      const int grp = tid / tpr;
+      const int lane = tid & (tpr - 1);
+      // Every lane of a group walks the same rows, so the group stays
+      // converged around its shuffles.
+      for (int lr = grp; lr < nrows; lr += groups) {
+        double s[NV];
 #pragma unroll
-      for (int q = 0; q < NV; ++q) s[q] = 0.0;
-      const int k1 = off[lr + 1];
+        for (int q = 0; q < NV; ++q) s[q] = 0.0;
+        const int k1 = off[lr + 1];
 #pragma unroll 1
-      for (int k = off[lr] + lane; k < k1; k += tpr) {
-        if constexpr (kRaw) {
-          const double a = (double)sa[k];
-          const float4* v4 = reinterpret_cast<const float4*>(sraw + k * NV);
+        for (int k = off[lr] + lane; k < k1; k += tpr) {
+          if constexpr (kRaw) {
+            const double a = (double)sa[k];
+            const float4* v4 = reinterpret_cast<const float4*>(sraw + k * NV);
 #pragma unroll
-          for (int q4 = 0; q4 < NV / 4; ++q4) {
-            const float4 v = v4[q4];
-            s[4 * q4 + 0] += a * (double)v.x;
-            s[4 * q4 + 1] += a * (double)v.y;
-            s[4 * q4 + 2] += a * (double)v.z;
-            s[4 * q4 + 3] += a * (double)v.w;
+            for (int q4 = 0; q4 < NV / 4; ++q4) {
+              const float4 v = v4[q4];
+              s[4 * q4 + 0] += a * (double)v.x;
+              s[4 * q4 + 1] += a * (double)v.y;
+              s[4 * q4 + 2] += a * (double)v.z;
+              s[4 * q4 + 3] += a * (double)v.w;
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) s[q] += stage[q][k];
           }
-        } else {
+        }
+        for (int o = tpr >> 1; o > 0; o >>= 1) {
 #pragma unroll
-          for (int q = 0; q < NV; ++q) s[q] += stage[q][k];
+          for (int q = 0; q < NV; ++q) s[q] += __shfl_xor(s[q], o, tpr);
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (int q = 0; q < NV; ++q) rowsum[q][lr] = s[q];
         }
       }
-      for (int o = tpr >> 1; o > 0; o >>= 1) {
+      __syncthreads();
+      hold(pre);
+      if constexpr (EL > 0) {
+        for (int e = tid; e < nrows * EL; e += BLOCK) {
+          const int lr = e / EL, q = e % EL;
+          const typename Epi::Pre p = e == tid ? pre : epi.pre(r0 + lr, q);
+          epi(r0 + lr, q, p, rowsum[q][lr], acc);
+        }
+      } else if (tid < nrows) {
+        double s[NV];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) s[q] += __shfl_xor(s[q], o, tpr);
+        for (int q = 0; q < NV; ++q) s[q] = rowsum[q][tid];
+        epi(r0 + tid, pre, s, acc);
       }
-      if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) rowsum[q][lr] = s[q];
-      }
-    }
-    __syncthreads();
-    if constexpr (EL > 0) {
-      for (int e = tid; e < nrows * EL; e += BLOCK) {
-        const int lr = e / EL, q = e % EL;
-        const typename Epi::Pre p = e == tid ? pre : epi.pre(r0 + lr, q);
-        epi(r0 + lr, q, p, rowsum[q][lr], acc);
-      }
-    } else if (tid < nrows) {
-      double s[NV];
-#pragma unroll
-      for (int q = 0; q < NV; ++q) s[q] = rowsum[q][tid];
-      epi(r0 + tid, pre, s, acc);
     }
   } else {
     double s[NV];
@@ -328,23 +406,26 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int k = k0 + u * BLOCK + tid;
-        cc[u] = k < cnt ? ld_stream(A.col + base + k) : 0;
-        vv[u] = k < cnt ? (double)ld_stream(A.val + base + k) : 0.0;
+        const int kc = k < cnt ? k : cnt - 1;
+        cc[u] = ld_stream(A.col + base + kc);
+        vv[u] = (double)ld_stream(A.val + base + kc);
       }
+      double p[4][NV];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) g(cc[u], vv[u], p[u]);
+      // adding +0.0 past the end leaves s unchanged (s starts at +0.0 and a
+      // sum from +0.0 never becomes -0.0)
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int k = k0 + u * BLOCK + tid;
-        if (k < cnt) {
-          double p[NV];
-          g(cc[u], vv[u], p);
+        const bool in = k0 + u * BLOCK + tid < cnt;
 #pragma unroll
-          for (int q = 0; q < NV; ++q) s[q] += p[q];
-        }
+        for (int q = 0; q < NV; ++q) s[q] += in ? p[u][q] : 0.0;
       }
     }
     double t[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) t[q] = block_sum_to0<BLOCK>(s[q], red);
+    hold(pre);
     if constexpr (EL > 0) {
       if (tid == 0) {
 #pragma unroll

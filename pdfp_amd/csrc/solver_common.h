@@ -116,12 +116,14 @@ inline void start_vector(int64_t m, uint64_t seed, double* out) {
 template <typename TV>
 struct DevCsr {
   DevBuf<int32_t> rowptr, col;
-  DevBuf<int4> sched;  // per-block {first row, first nonzero, lanes per row, 0}
+  DevBuf<int4> sched;  // per-block {first row, first nonzero, lanes per row, mode}
   DevBuf<TV> val;
   int32_t nblocks = 0, nrows = 0;
   int32_t npb = 512;  // nonzeros per stream-mode block of the current schedule
   int32_t persist = 0;  // 0: one workgroup per block; k: k workgroups per CU, looping
   int32_t max_rows = kBlock;  // rows per block of the current schedule, at most
+  int32_t lane_len = 0;  // blocks whose rows all hold at most this many nonzeros
+                         // are reduced a lane per (row, value) (kLaneRows); 0: none
   int64_t nnz = 0;
   std::vector<int32_t> h_rowptr;  // kept for re-scheduling
 
@@ -141,11 +143,14 @@ struct DevCsr {
     schedule(npb, s);
   }
 
-  // Row blocks of at most kBlock rows and `max_nnz` nonzeros, with their
-  // lanes per row.
-  void schedule(int max_nnz, hipStream_t s, int rows = kBlock) {
+  // Row blocks of at most `rows` rows and `max_nnz` nonzeros, with their
+  // lanes per row.  `lane_rows` > 0 (lane epilogues over raw staging only)
+  // marks the stream-mode blocks whose rows all hold at most that many
+  // nonzeros for the lane-per-(row, value) reduction.
+  void schedule(int max_nnz, hipStream_t s, int rows = kBlock, int lane_rows = 0) {
     npb = max_nnz;
     max_rows = rows;
+    lane_len = lane_rows;
     std::vector<int32_t> blk = row_blocks(h_rowptr, rows, max_nnz);
     std::vector<uint8_t> lanes = block_lanes(h_rowptr, blk, kBlock, max_nnz);
     nblocks = (int32_t)blk.size() - 1;
@@ -155,6 +160,12 @@ struct DevCsr {
       rec[k].y = h_rowptr[(size_t)blk[k]];
       rec[k].z = k < lanes.size() ? (int)lanes[k] : 0;
       rec[k].w = 0;
+      if (lane_rows > 0 && k < lanes.size() && lanes[k] > 0) {
+        int32_t longest = 0;
+        for (int32_t r = blk[k]; r < blk[k + 1]; ++r)
+          longest = std::max(longest, h_rowptr[(size_t)r + 1] - h_rowptr[(size_t)r]);
+        if (longest <= lane_rows) rec[k].w = kLaneRows;
+      }
     }
     sched.alloc(rec.size());
     sched.upload(rec.data(), rec.size(), s);
