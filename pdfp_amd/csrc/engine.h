@@ -402,24 +402,45 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
     for (int q = 0; q < NV; ++q) s[q] = 0.0;
     for (int k0 = 0; k0 < cnt; k0 += BLOCK * 4) {
       int cc[4];
-      double vv[4];
+      TV vv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int k = k0 + u * BLOCK + tid;
         const int kc = k < cnt ? k : cnt - 1;
         cc[u] = ld_stream(A.col + base + kc);
-        vv[u] = (double)ld_stream(A.val + base + kc);
+        vv[u] = ld_stream(A.val + base + kc);
       }
-      double p[4][NV];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) g(cc[u], vv[u], p[u]);
       // adding +0.0 past the end leaves s unchanged (s starts at +0.0 and a
       // sum from +0.0 never becomes -0.0)
+      if constexpr (kRaw) {
+        // the gathered values stay FP32 in registers until their products
+        float4 gv[4][NV / 4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool in = k0 + u * BLOCK + tid < cnt;
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) s[q] += in ? p[u][q] : 0.0;
+          for (int h = 0; h < NV / 4; ++h) gv[u][h] = g.raw4(cc[u], h);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool in = k0 + u * BLOCK + tid < cnt;
+          const double a = (double)vv[u];
+#pragma unroll
+          for (int h = 0; h < NV / 4; ++h) {
+            s[4 * h + 0] += in ? a * (double)gv[u][h].x : 0.0;
+            s[4 * h + 1] += in ? a * (double)gv[u][h].y : 0.0;
+            s[4 * h + 2] += in ? a * (double)gv[u][h].z : 0.0;
+            s[4 * h + 3] += in ? a * (double)gv[u][h].w : 0.0;
+          }
+        }
+      } else {
+        double p[4][NV];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) g(cc[u], (double)vv[u], p[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool in = k0 + u * BLOCK + tid < cnt;
+#pragma unroll
+          for (int q = 0; q < NV; ++q) s[q] += in ? p[u][q] : 0.0;
+        }
       }
     }
     double t[NV];
