@@ -77,6 +77,21 @@ struct GatherBatch {
   }
 };
 
+// Trajectory q's step scalars of the launch and its active flag.  The
+// epilogues load them together with nothing else in flight behind them and
+// test the flag only before their stores, so the loads go out as one batch.
+struct BatchScalars {
+  StepScalars c;
+  double active;  // 1: iterate this trajectory, 0: leave it unchanged
+};
+
+__device__ __forceinline__ BatchScalars load_batch_scalars(const BatchStep* __restrict__ sp,
+                                                           const double2* __restrict__ wt,
+                                                           int k_offset, int q) {
+  const BatchStep& st = sp[q];
+  return {batch_scalars(st, wt, k_offset, q), st.active};
+}
+
 // Column j, trajectory q (lane q of the row's 8): the fused primal and dual-y
 // step (src/pdfp.jl:119-143) with s = (Aᵀ z_q)_j.  An inactive trajectory
 // is left untouched.  MAJOR steps also store pdhg x, the dual slack, pdhg y
@@ -85,7 +100,7 @@ template <bool MAJOR>
 struct EpiBatchPrimalY {
   static constexpr int kLanes = kBatch;
   struct Pre {
-    float x, y, lb, ub, w;
+    float x, y, x0, y0, xt, lb, ub, w;
   };
   const BatchStep* sp;
   const double2* wt;
@@ -105,17 +120,16 @@ struct EpiBatchPrimalY {
   float* __restrict__ ry;
   __device__ __forceinline__ Pre pre(int j, int q) const {
     const size_t o = (size_t)j * kBatch + q;
-    return {x[o], y[o], lb[j], ub[j], w[j]};
+    return {x[o], y[o], x0[o], y0[o], xt[o], lb[j], ub[j], w[j]};
   }
   __device__ __forceinline__ void operator()(int j, int q, const Pre& p, double s,
                                              double*) const {
-    const BatchStep& st = sp[q];
-    if (st.active == 0.0) return;
+    const BatchScalars sh = load_batch_scalars(sp, wt, k_offset, q);
     const size_t o = (size_t)j * kBatch + q;
-    const StepScalars c = batch_scalars(st, wt, k_offset, q);
-    const PrimalYOut r = primal_y_math(c, s, (double)p.x, (double)p.y, (double)x0[o],
-                                       (double)y0[o], (double)p.lb, (double)p.ub,
-                                       (double)xt[o], (double)p.w);
+    const PrimalYOut r = primal_y_math(sh.c, s, (double)p.x, (double)p.y, (double)p.x0,
+                                       (double)p.y0, (double)p.lb, (double)p.ub, (double)p.xt,
+                                       (double)p.w);
+    if (sh.active == 0.0) return;
     x[o] = (float)r.x;
     y[o] = (float)r.y;
     rx[o] = (float)r.rx;
@@ -134,7 +148,8 @@ template <bool MAJOR>
 struct EpiBatchDualZ {
   static constexpr int kLanes = kBatch;
   struct Pre {
-    float z;
+    float z, z0, b;
+    uint32_t eq;
   };
   const BatchStep* sp;
   const double2* wt;
@@ -145,14 +160,16 @@ struct EpiBatchDualZ {
   const uint8_t* __restrict__ is_eq;
   float* __restrict__ pz;
   float* __restrict__ rz;
-  __device__ __forceinline__ Pre pre(int i, int q) const { return {z[(size_t)i * kBatch + q]}; }
+  __device__ __forceinline__ Pre pre(int i, int q) const {
+    const size_t o = (size_t)i * kBatch + q;
+    return {z[o], z0[o], b[i], (uint32_t)is_eq[i]};
+  }
   __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
                                              double*) const {
-    const BatchStep& st = sp[q];
-    if (st.active == 0.0) return;
+    const BatchScalars sh = load_batch_scalars(sp, wt, k_offset, q);
     const size_t o = (size_t)i * kBatch + q;
-    const StepScalars c = batch_scalars(st, wt, k_offset, q);
-    const DualZOut r = dual_z_math(c, s, (double)p.z, (double)z0[o], (double)b[i], is_eq[i] != 0);
+    const DualZOut r = dual_z_math(sh.c, s, (double)p.z, (double)p.z0, (double)p.b, p.eq != 0);
+    if (sh.active == 0.0) return;
     z[o] = (float)r.z;
     if (MAJOR) {
       pz[o] = (float)r.pz;
