@@ -138,8 +138,9 @@ struct Batch {
     autotune();
     part_t.alloc((size_t)std::max<int64_t>((n + kEvalRows - 1) / kEvalRows, 1) * kBatch *
                  kEvalColTerms);
-    part_a.alloc((size_t)std::max<int64_t>((m + kEvalRows - 1) / kEvalRows, 1) * kBatch *
-                 kEvalRowTerms);
+    // the row terms run on A's schedule (evaluate_all grows this if a new
+    // schedule has more blocks)
+    part_a.alloc((size_t)std::max<int32_t>(A.nblocks, 1) * kBatch * kEvalRowTerms);
     PDFP_HIP(hipStreamSynchronize(stream));
   }
 
@@ -330,11 +331,18 @@ struct Batch {
       k_batch_eval_finalize<<<kBatch * kEvalColTerms, 256, 0, stream>>>(
           part_t.p, nbt, kEvalColTerms, mask, slot * 16, sums.p);
     }
-    if (nba) {
-      k_batch_eval_row<<<nba, 256, 0, stream>>>((int)m, A.rowptr.p, A.col.p, A.val.p, px.p, er,
-                                                 mask, part_a.p);
+    if (nba && A.nblocks > 0) {
+      const size_t need = (size_t)A.nblocks * kBatch * kEvalRowTerms;
+      if (part_a.n < need) part_a.alloc(need);
+      EpiBatchEvalRowLanes el{er, mask};
+      if (A.max_rows > kBlock / kBatch)
+        throw Error(PDFP_ERR_STATE, "batch schedule holds too many rows per block");
+      if (A.npb == 512)
+        launch_pass_npb<512, kEvalRowTerms, 0>(A, GatherBatch{px.p}, el, part_a.p, stream);
+      else
+        launch_pass_npb<256, kEvalRowTerms, 0>(A, GatherBatch{px.p}, el, part_a.p, stream);
       k_batch_eval_finalize<<<kBatch * kEvalRowTerms, 256, 0, stream>>>(
-          part_a.p, nba, kEvalRowTerms, mask, slot * 16 + 8, sums.p);
+          part_a.p, A.nblocks, kEvalRowTerms, mask, slot * 16 + 8, sums.p);
     }
   }
 

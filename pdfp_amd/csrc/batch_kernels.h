@@ -244,36 +244,40 @@ struct EpiBatchEvalRow {
   }
 };
 
-// ---- evaluation of several trajectories in one pass ---------------------------
-// One workgroup serves 32 consecutive rows (of Aᵀ: columns; of A: rows), lane
-// (row, q) = (tid / 8, tid % 8) forms trajectory q's sums over the row in
-// FP64, in ascending column order, and adds q's evaluation terms (the
-// EpiBatchEvalCol / EpiBatchEvalRow functors) when bit q of `mask` is set.
-// Each workgroup writes per-trajectory partials partial[block][q][t]; the
-// sums of the 8 trajectories share every 32-byte gather.
-constexpr int kEvalRows = 256 / kBatch;
+// The row terms of every trajectory in `mask` as a lane epilogue of the CSR
+// pass over A (gathering the 8 trajectories' pdhg x): lane (row, q) adds
+// trajectory q's terms; the pass writes per-trajectory partials.
+struct EpiBatchEvalRowLanes {
+  static constexpr int kLanes = kBatch;
+  using Pre = EpiBatchEvalRow::Pre;
+  EpiBatchEvalRow er;
+  int mask;
+  __device__ __forceinline__ Pre pre(int i, int q) const {
+    EpiBatchEvalRow e = er;
+    e.k = q;
+    return e.pre(i);
+  }
+  __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
+                                             double* acc) const {
+    if (!((mask >> q) & 1)) return;
+    EpiBatchEvalRow e = er;
+    e.k = q;
+    e(i, p, &s, acc);
+  }
+};
 
-template <int NT>
-__device__ __forceinline__ void batch_eval_store(double (&acc)[NT], double* __restrict__ out) {
-  __shared__ double red[256 / kWave][kBatch][NT];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wid = threadIdx.x / kWave;
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    double v = acc[t];
-    // lanes with the same q = lane % 8: offsets 8, 16, 32
-    for (int o = kBatch; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
-    if (lane < kBatch) red[wid][lane][t] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < kBatch * NT) {
-    const int q = threadIdx.x / NT, t = threadIdx.x % NT;
-    double v = 0.0;
-#pragma unroll
-    for (int w = 0; w < 256 / kWave; ++w) v += red[w][q][t];
-    out[threadIdx.x] = v;  // out[q * NT + t]
-  }
-}
+// ---- evaluation of several trajectories in one pass ---------------------------
+// The column terms: one workgroup serves 32 consecutive columns (rows of Aᵀ);
+// lane (column, q) = (tid / 8, tid % 8) forms trajectory q's two sums over
+// the column in FP64, in ascending row order (the second over the difference
+// pz - rz, formed first as the reference does), and adds q's terms
+// (EpiBatchEvalCol) when bit q of `mask` is set.  Each workgroup writes
+// per-trajectory partials partial[(q·NT + t)·nblocks + block]; the 8
+// trajectories share every 32-byte gather.  (The row terms run as a lane
+// epilogue of the CSR pass over A, EpiBatchEvalRowLanes, whose schedule
+// spreads long rows over the workgroup; columns are short in the pump's
+// instances, and a dense column costs one lane loop here.)
+constexpr int kEvalRows = 256 / kBatch;
 
 static __global__ __launch_bounds__(256) void k_batch_eval_col(
     int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
@@ -296,38 +300,21 @@ static __global__ __launch_bounds__(256) void k_batch_eval_col(
     ec.k = q;
     ec(j, ec.pre(j), s, acc);
   }
-  batch_eval_store<kEvalColTerms>(acc, partial + (size_t)blockIdx.x * kBatch * kEvalColTerms);
+  lane_store_partials<256, kBatch, kEvalColTerms>(acc, partial, gridDim.x, blockIdx.x);
 }
 
-static __global__ __launch_bounds__(256) void k_batch_eval_row(
-    int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-    const float* __restrict__ val, const float* __restrict__ px, EpiBatchEvalRow er, int mask,
-    double* __restrict__ partial) {
-  const int q = threadIdx.x % kBatch;
-  const int i = blockIdx.x * kEvalRows + threadIdx.x / kBatch;
-  double acc[kEvalRowTerms];
-#pragma unroll
-  for (int t = 0; t < kEvalRowTerms; ++t) acc[t] = 0.0;
-  if (i < nrows && ((mask >> q) & 1)) {
-    double s = 0.0;
-    for (int k = ptr[i]; k < ptr[i + 1]; ++k)
-      s += (double)val[k] * (double)px[(size_t)col[k] * kBatch + q];
-    er.k = q;
-    er(i, er.pre(i), &s, acc);
-  }
-  batch_eval_store<kEvalRowTerms>(acc, partial + (size_t)blockIdx.x * kBatch * kEvalRowTerms);
-}
-
-// out[q·32 + offset + t] = Σ_b partial[b][q][t] in block order, for every
-// trajectory q in mask: one workgroup per (q, t).
+// out[q·32 + offset + t] = Σ_b partial[(q·nt + t)·nb + b] in block order,
+// for every trajectory q in mask: one workgroup per (q, t), reading its
+// series contiguously.
 static __global__ __launch_bounds__(256) void k_batch_eval_finalize(
     const double* __restrict__ partial, int nb, int nt, int mask, int offset,
     double* __restrict__ out) {
   __shared__ double red[256 / kWave];
   const int q = blockIdx.x / nt, t = blockIdx.x % nt;
   if (!((mask >> q) & 1)) return;  // uniform over the workgroup
+  const double* __restrict__ series = partial + (size_t)blockIdx.x * nb;
   double v = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) v += partial[((size_t)b * kBatch + q) * nt + t];
+  for (int b = threadIdx.x; b < nb; b += 256) v += series[b];
   v = block_sum_to0<256>(v, red);
   if (threadIdx.x == 0) out[q * 32 + offset + t] = v;
 }

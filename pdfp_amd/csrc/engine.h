@@ -30,8 +30,9 @@
 // reference's CSC product.
 //
 // Per-block FP64 partials (NACC terms) of the epilogues' reductions are
-// written to partials[block·NACC + k] and combined by finalize_partials in
-// block order, so every reduction is deterministic run to run.
+// written to partials[block·NACC + k] (a lane epilogue's, per gathered value
+// q, to partials[(q·NACC + k)·nblocks + block]) and combined in block order,
+// so every reduction is deterministic run to run.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -151,6 +152,33 @@ __device__ __forceinline__ void block_store_partials(const double (&acc)[NACC],
     for (int w = 0; w < kW; ++w)
       s = is_max ? fmax(s, red[w][threadIdx.x]) : s + red[w][threadIdx.x];
     out[threadIdx.x] = s;
+  }
+}
+
+// Per-value partials of a lane epilogue: thread t's terms belong to gathered
+// value q = t % EL, and out[(q·NACC + k)·nb + b] receives the sum of term k
+// over the block's threads of class q (sums only), so that a finalize reads
+// each (q, k) series contiguously.
+template <int BLOCK, int EL, int NACC>
+__device__ __forceinline__ void lane_store_partials(const double (&acc)[NACC],
+                                                    double* __restrict__ out, int nb, int b) {
+  static_assert(kWave % EL == 0, "lane classes must tile a wavefront");
+  __shared__ double red[BLOCK / kWave][EL][NACC];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
+    double v = acc[k];
+    for (int o = EL; o < kWave; o <<= 1) v += __shfl_xor(v, o, kWave);
+    if (lane < EL) red[wid][lane][k] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < EL * NACC) {
+    const int q = threadIdx.x / NACC, k = threadIdx.x % NACC;
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / kWave; ++w) v += red[w][q][k];
+    out[(size_t)(q * NACC + k) * nb + b] = v;
   }
 }
 
@@ -459,7 +487,12 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
     }
   }
   if constexpr (NACC > 0) {
-    block_store_partials<BLOCK, NACC, NMAX>(acc, partials + (size_t)b * NACC);
+    if constexpr (EL > 0) {
+      static_assert(NMAX == 0, "lane partials are sums");
+      lane_store_partials<BLOCK, EL, NACC>(acc, partials, A.nblocks, b);
+    } else {
+      block_store_partials<BLOCK, NACC, NMAX>(acc, partials + (size_t)b * NACC);
+    }
   }
 }
 
