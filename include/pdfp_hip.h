@@ -238,6 +238,51 @@ int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int3
                             int32_t lane_rows_a, int32_t lane_rows_at);
 void pdfp_batch_destroy(pdfp_batch* ctx);
 
+/* ---- pack of independent instances ------------------------------------ */
+/* The reference solves independent instances in separate processes
+ * (scripts/solve.jl:121-148).  A pdfp_pack holds K projector instances of
+ * uneven sizes as one block-diagonal A (and Aᵀ) in HBM and iterates them
+ * together in the FP32 form's fused launches; each instance has its own
+ * target, iterates, step size (its own power iteration), primal weight,
+ * restart state and termination, and follows pdfp_solve!
+ * (src/pdfp.jl:252-288) on its own.  Instance k is given as pdfp_create's
+ * arguments (its CSC and vectors: element k of each pointer array). */
+
+typedef struct pdfp_pack pdfp_pack;
+
+const char* pdfp_pack_last_error(const pdfp_pack* ctx);
+
+/* As K calls of pdfp_create (FP32 form), into one pack. */
+int pdfp_pack_create(pdfp_pack** out, int32_t K, const int64_t* m, const int64_t* n,
+                     const int64_t* const* colptr, const int64_t* const* rowval,
+                     const double* const* nzval, int32_t index_base, const double* const* b,
+                     const uint8_t* const* is_eq, const uint8_t* const* is_int,
+                     const double* const* var_lb, const double* const* var_ub,
+                     const double* const* w, int32_t device);
+/* Instance k's original_primal_residual! (src/pdfp.jl:85-90), copyto! of
+ * current_x/initial_x (src/pump.jl:305-306), set_target! (src/pdfp.jl:98-101)
+ * and warm_start! (src/pdfp.jl:104-114). */
+int pdfp_pack_set_original_primal_residual(pdfp_pack* ctx, int32_t k, const double* D_r);
+int pdfp_pack_set_x(pdfp_pack* ctx, int32_t k, const double* x_scaled);
+int pdfp_pack_set_target(pdfp_pack* ctx, int32_t k, const double* x_tilde_scaled);
+int pdfp_pack_warm_start(pdfp_pack* ctx, int32_t k);
+/* pdfp_solve! for every k with active[k] (all when active is NULL), with
+ * primal_weight[k]; returns when each of them has terminated.  The running
+ * instances share the evaluation frequency and the reflection coefficient
+ * (one params for all).  termination / total_count: K entries. */
+int pdfp_pack_solve(pdfp_pack* ctx, const pdfp_params* params, const double* primal_weight,
+                    const uint8_t* active, int32_t* termination, int64_t* total_count);
+/* Instance k's projection (FP64, polished), vectors (ids as in
+ * pdfp_get_vector) and scalars. */
+int pdfp_pack_get_x(pdfp_pack* ctx, int32_t k, double* x_scaled_out);
+int pdfp_pack_get_vector(pdfp_pack* ctx, int32_t k, int32_t which, double* out);
+int pdfp_pack_get_info(pdfp_pack* ctx, int32_t k, pdfp_info* out);
+/* Mean milliseconds of one pack iteration (every instance), and of its
+ * transpose pass and A pass. */
+int pdfp_pack_time_iterations(pdfp_pack* ctx, int32_t iterations, double* ms_out);
+int pdfp_pack_time_kernels(pdfp_pack* ctx, int32_t iterations, double* ms_at, double* ms_a);
+void pdfp_pack_destroy(pdfp_pack* ctx);
+
 /* ---- root LP (src/pdlp.jl) -------------------------------------------- */
 
 typedef struct pdlp_ctx pdlp_ctx;

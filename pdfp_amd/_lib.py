@@ -139,6 +139,23 @@ SIGNATURES = {
     "pdfp_batch_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_batch_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32]),
+    "pdfp_pack_last_error": (C.c_char_p, [_P]),
+    "pdfp_pack_create": (C.c_int, [C.POINTER(_P), C.c_int32, _I64, _I64, C.POINTER(_I64),
+                                   C.POINTER(_I64), C.POINTER(_D), C.c_int32, C.POINTER(_D),
+                                   C.POINTER(_U8), C.POINTER(_U8), C.POINTER(_D), C.POINTER(_D),
+                                   C.POINTER(_D), C.c_int32]),
+    "pdfp_pack_set_original_primal_residual": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_pack_set_x": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_pack_set_target": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_pack_warm_start": (C.c_int, [_P, C.c_int32]),
+    "pdfp_pack_solve": (C.c_int, [_P, C.POINTER(PdfpParams), _D, _U8, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int64)]),
+    "pdfp_pack_get_x": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_pack_get_vector": (C.c_int, [_P, C.c_int32, C.c_int32, _D]),
+    "pdfp_pack_get_info": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpInfo)]),
+    "pdfp_pack_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_pack_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "pdfp_pack_destroy": (None, [_P]),
     "pdfp_batch_destroy": (None, [_P]),
     "pdfp_gather_ceiling": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, _D]),
     "pdfp_scale_matrix": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32,

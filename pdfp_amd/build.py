@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpdfp_hip.so")
-SOURCES = ["pdfp_hip.hip", "pdlp_hip.hip", "batch_hip.hip", "bench_hip.hip", "scale_hip.hip", "check_hip.hip", "rounder.cpp"]
+SOURCES = ["pdfp_hip.hip", "pdlp_hip.hip", "batch_hip.hip", "pack_hip.hip", "bench_hip.hip", "scale_hip.hip", "check_hip.hip", "rounder.cpp"]
 
 # -ffp-contract=off: no a*b+c contraction, so the FP64 form performs the
 # reference's operations one by one.  The packed-FP32 target feature is

@@ -157,16 +157,14 @@ inline HostCsr csc_as_transpose_csr(int64_t m, int64_t n, const int64_t* colptr,
 // Row-block schedule: rows [blocks[k], blocks[k+1]) form block k.  A block
 // holds at most `max_rows` rows and at most `max_nnz` nonzeros, except that a
 // row with more than `max_nnz` nonzeros forms a block of its own.
-inline std::vector<int32_t> row_blocks(const std::vector<int32_t>& rowptr, int max_rows,
-                                       int max_nnz) {
-  const int64_t m = (int64_t)rowptr.size() - 1;
-  std::vector<int32_t> blocks;
-  blocks.push_back(0);
-  int64_t r = 0;
-  while (r < m) {
+// Appends the block ends of rows [r0, r1) to `blocks` (whose last entry is r0).
+inline void row_blocks_range(const std::vector<int32_t>& rowptr, int64_t r0, int64_t r1,
+                             int max_rows, int max_nnz, std::vector<int32_t>& blocks) {
+  int64_t r = r0;
+  while (r < r1) {
     const int64_t start = r;
     int64_t acc = 0;
-    while (r < m && r - start < max_rows) {
+    while (r < r1 && r - start < max_rows) {
       const int64_t len = rowptr[(size_t)r + 1] - rowptr[(size_t)r];
       if (acc + len > max_nnz) break;
       acc += len;
@@ -175,6 +173,13 @@ inline std::vector<int32_t> row_blocks(const std::vector<int32_t>& rowptr, int m
     if (r == start) ++r;  // a single row longer than max_nnz
     blocks.push_back((int32_t)r);
   }
+}
+
+inline std::vector<int32_t> row_blocks(const std::vector<int32_t>& rowptr, int max_rows,
+                                       int max_nnz) {
+  std::vector<int32_t> blocks;
+  blocks.push_back(0);
+  row_blocks_range(rowptr, 0, (int64_t)rowptr.size() - 1, max_rows, max_nnz, blocks);
   return blocks;
 }
 

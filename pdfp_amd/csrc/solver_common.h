@@ -126,6 +126,12 @@ struct DevCsr {
                          // are reduced a lane per (row, value) (kLaneRows); 0: none
   int64_t nnz = 0;
   std::vector<int32_t> h_rowptr;  // kept for re-scheduling
+  // Independent row segments (the instances of a block-diagonal pack): row
+  // offsets, K + 1 entries, or empty for one segment.  No block of a
+  // schedule crosses a segment boundary; seg_blocks[k] is the first block of
+  // segment k (seg_blocks[K] = nblocks), so per-block partials sum per segment.
+  std::vector<int64_t> segments;
+  std::vector<int32_t> seg_blocks;
 
   void upload(const HostCsr& h, hipStream_t s) {
     nrows = (int32_t)h.nrows;
@@ -151,7 +157,17 @@ struct DevCsr {
     npb = max_nnz;
     max_rows = rows;
     lane_len = lane_rows;
-    std::vector<int32_t> blk = row_blocks(h_rowptr, rows, max_nnz);
+    std::vector<int32_t> blk;
+    if (segments.empty()) {
+      blk = row_blocks(h_rowptr, rows, max_nnz);
+    } else {
+      blk.assign(1, (int32_t)segments[0]);
+      seg_blocks.assign(1, 0);
+      for (size_t k = 0; k + 1 < segments.size(); ++k) {
+        row_blocks_range(h_rowptr, segments[k], segments[k + 1], rows, max_nnz, blk);
+        seg_blocks.push_back((int32_t)blk.size() - 1);
+      }
+    }
     std::vector<uint8_t> lanes = block_lanes(h_rowptr, blk, kBlock, max_nnz);
     nblocks = (int32_t)blk.size() - 1;
     std::vector<int4> rec(blk.size());

@@ -1,0 +1,105 @@
+"""GPU tests of the block-diagonal pack of independent instances (pdfp_pack_*).
+
+Each instance must behave as its own FP32 projection: its KKT residuals,
+re-evaluated in FP64 by the oracle, meet the tolerance (×10 for FP32
+rounding), its ℓ1 distance matches the single FP32 projector's to 1e-4
+relative, and instances that are not active are left unchanged."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pdfp_amd import generators as G
+from pdfp_amd import pdfp as P
+from pdfp_amd.pack import PDFPPack
+from pdfp_amd.pdfp import SolverParameters, projector_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_kkt(pp, xt_scaled, x, y, z, ds):
+    o = O.OraclePDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    O.set_target(o, xt_scaled)
+    o.initialize(None, 1.0)
+    return o.residual_of(x, y, z, ds)
+
+
+def _single(pp, target, p, pw, D_r=None):
+    s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                    precision="fp32")
+    if D_r is not None:
+        P.original_primal_residual(s, D_r)
+    P.set_target(s, target)
+    assert P.pdfp_solve(s, p, pw) == "OPTIMAL"
+    l1 = s.info().l1_distance
+    s.close()
+    return l1
+
+
+def _instances():
+    # uneven sizes, both pure-binary and mixed (continuous columns, w = 0)
+    return [G.set_cover(133, 201, 1), G.mixed(400, 800, 3), G.set_cover(700, 1500, 4),
+            G.mixed(60, 90, 5), G.set_cover(2000, 3500, 6)]
+
+
+def test_pack_instances_are_independent_projections():
+    print("pack independent", flush=True)
+    milps = _instances()
+    pps = [projector_problem(m) for m in milps]
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    pk = PDFPPack(pps)
+    targets = [G.binary_targets(m, 50 + k) / pp.D_c for k, (m, pp) in enumerate(zip(milps, pps))]
+    for k, t in enumerate(targets):
+        pk.set_target(k, t)
+    pw = np.linspace(0.7, 1.4, len(pps))
+    assert pk.solve(p, pw) == ["OPTIMAL"] * len(pps)
+    for k, pp in enumerate(pps):
+        info = pk.info(k)
+        assert info.total_count > 0 and info.step_size > 0
+        r = _oracle_kkt(pp, targets[k], pk.pdhg_x(k), pk.vector(k, "pdhg_y"),
+                        pk.vector(k, "pdhg_z"), pk.vector(k, "dual_slack"))
+        assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (k, r)
+        l1s = _single(pp, targets[k], p, float(pw[k]))
+        assert abs(info.l1_distance - l1s) <= 1e-4 * max(1.0, l1s), (k, info.l1_distance, l1s)
+
+
+def test_pack_inactive_instance_is_untouched_and_warm_starts():
+    print("pack inactive", flush=True)
+    milps = [G.set_cover(133, 201, 4), G.set_cover(150, 260, 7), G.mixed(80, 120, 2)]
+    pps = [projector_problem(m) for m in milps]
+    p = SolverParameters(eps_optimal_relative=1e-6, eps_feasible_relative=1e-6)
+    pk = PDFPPack(pps)
+    for k, pp in enumerate(pps):
+        pk.set_target(k, np.zeros(pp.A.shape[1]))
+    assert pk.solve(p, 1.0) == ["OPTIMAL"] * 3
+    before = [pk.pdhg_x(k) for k in range(3)]
+    n1 = pk.total_count(1)
+    pk.set_target(0, np.ones(pps[0].A.shape[1]) / pps[0].D_c)  # feasible target: distance 0
+    terms = pk.solve(p, 1.0, active=[1, 0, 0])
+    assert terms[0] == "OPTIMAL"
+    assert pk.info(0).l1_distance <= 1e-5
+    for k in (1, 2):
+        assert np.array_equal(pk.pdhg_x(k), before[k])
+    assert pk.total_count(1) == n1
+
+
+def test_pack_original_primal_residual():
+    """original_primal_residual! on one mixed instance of the pack: the
+    primal residual in original units, as the single projector computes it."""
+    print("pack orig primal", flush=True)
+    milps = [G.mixed(300, 600, 8), G.set_cover(200, 300, 9)]
+    pps = [projector_problem(m) for m in milps]
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    pk = PDFPPack(pps)
+    pk.original_primal_residual(0, pps[0].D_r)
+    targets = [G.binary_targets(m, 11 + k) / pp.D_c for k, (m, pp) in enumerate(zip(milps, pps))]
+    for k, t in enumerate(targets):
+        pk.set_target(k, t)
+    assert pk.solve(p, 1.0) == ["OPTIMAL", "OPTIMAL"]
+    for k, pp in enumerate(pps):
+        l1s = _single(pp, targets[k], p, 1.0, D_r=pp.D_r if k == 0 else None)
+        l1p = pk.info(k).l1_distance
+        assert abs(l1p - l1s) <= 1e-4 * max(1.0, l1s), (k, l1p, l1s)
