@@ -1,13 +1,14 @@
 """Benchmark of the PDHG projector on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c4|c5]
 
 Workload (default, "c2"): the mixed MILP with 50,000 rows × 100,000 columns
 (70 % binary, 30 % continuous in [0, 10]), 1,000,000 nonzeros, row lengths
 uniform in 10-30, values N(0,1), b = A x0 + |N(0,1)| for a random feasible
 x0 (seed 1).  ``--workload c3``: 500,000 × 1,000,000 with exactly 5,000,000
 nonzeros in power-law rows; ``c5``: a general-integer, equality-heavy MILP of
-200,000 × 400,000 with 4M nonzeros.  A step is one warm-started ℓ1
+200,000 × 400,000 with 4M nonzeros; ``c4``: 64 set-covering instances of
+uneven sizes in one block-diagonal pack (15.5M nonzeros in all).  A step is one warm-started ℓ1
 projection toward a new random integer target (0/1 on binaries) at relative
 KKT tolerance 1e-4, the way the pump calls the projector
 (src/pump.jl:329-359).  The scaled matrix, its transpose and every
@@ -257,6 +258,133 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 10):
           flush=True)
 
 
+def bench_pack(args) -> None:
+    """``--workload c4``: 64 set-covering instances of uneven sizes (seeds
+    1..64; rows e^U[ln 2k, ln 40k], columns e^U[ln 4k, ln 80k]) in one
+    block-diagonal pack.  A step is one warm-started ℓ1 projection of every
+    instance to a new 0/1 target at the tolerance, all solved together; the
+    value is Σ PDHG iterations over all instances / wall time.  Beside it,
+    the same projections one instance at a time with the single FP32 and the
+    plain FP64 projector."""
+    from pdfp_amd import generators as G
+    from pdfp_amd import pdfp as P
+    from pdfp_amd.pack import PDFPPack
+
+    t0 = time.perf_counter()
+    milps = G.uneven_set_covers(64)
+    pps = [P.projector_problem(m) for m in milps]
+    mm = sum(m.A.shape[0] for m in milps)
+    nn = sum(m.A.shape[1] for m in milps)
+    nnz = sum(m.A.nnz for m in milps)
+    desc = (f"64 set-covering instances of uneven sizes (seeds 1..64, rows e^U[ln 2k, ln 40k], "
+            f"columns e^U[ln 4k, ln 80k], each column covering 3-15 rows): {mm} rows, {nn} "
+            f"columns, {nnz} nnz in total, one block-diagonal pack")
+    print(f"[bench] workload c4: {mm}x{nn}, nnz {nnz}, built in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    K = len(milps)
+    p = P.SolverParameters(eps_optimal_relative=args.tol, eps_feasible_relative=args.tol,
+                           iteration_limit=500_000)
+    pws = [float(np.sqrt(1.0 + len(m.int_idx) / max(len(m.b), 1))) for m in milps]
+    sets_w = [[G.binary_targets(m, 3000 + 100 * s + k) for k, m in enumerate(milps)]
+              for s in range(args.warmup)]
+    sets = [[G.binary_targets(m, 4000 + 100 * s + k) for k, m in enumerate(milps)]
+            for s in range(args.steps)]
+
+    t0 = time.perf_counter()
+    pk = PDFPPack(pps)
+    print(f"[bench] pack of {K} created in {time.perf_counter() - t0:.2f}s", file=sys.stderr,
+          flush=True)
+
+    def pack_steps(target_sets):
+        iters, dt = 0, 0.0
+        statuses = []
+        for ts in target_sets:
+            t = time.perf_counter()
+            for k, xt in enumerate(ts):
+                pk.set_target(k, xt / pps[k].D_c)
+            st = pk.solve(p, pws)
+            dt += time.perf_counter() - t
+            iters += sum(pk.last_total_count)
+            statuses.append(sorted(set(st)))
+        return iters, dt, statuses
+
+    t0 = time.perf_counter()
+    pack_steps(sets_w)
+    print(f"[bench] pack warmup {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+    it, dt, st = pack_steps(sets)
+    ms_at, ms_a = pk.time_kernels(100)
+    ms_iter = pk.time_iterations(200)
+    info = pk.info(0)
+    print(f"[bench] fp32 pack of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; kernels "
+          f"AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; pack iteration {ms_iter * 1e3:.1f}us; "
+          f"{st}", file=sys.stderr, flush=True)
+    npb = {"A": info.npb_a, "At": info.npb_at, "A_persistent_wg_per_cu": info.persist_a,
+           "At_persistent_wg_per_cu": info.persist_at}
+    pk.close()
+
+    # the same projections one instance at a time
+    seq = {}
+    for prec in (["fp32"] if args.no_fp64 else ["fp32", "fp64"]):
+        iters, secs = 0, 0.0
+        t0 = time.perf_counter()
+        for k, (m, pp) in enumerate(zip(milps, pps)):
+            s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                            precision=prec)
+            run_projections(s, pp, m, [ts[k] for ts in sets_w], args.tol)
+            i, d, _ = run_projections(s, pp, m, [ts[k] for ts in sets], args.tol)
+            iters += i
+            secs += d
+            s.close()
+        seq[prec] = iters / secs
+        print(f"[bench] {prec} one instance at a time: {iters} iterations in {secs:.3f}s = "
+              f"{iters / secs:.0f} it/s ({time.perf_counter() - t0:.1f}s with set-up)",
+              file=sys.stderr, flush=True)
+
+    value = it / dt
+    bat, ba = kernel_bytes_fp32(mm, nn, nnz)
+    bat += 4 * nn  # the column's instance index
+    ba += 4 * mm   # the row's instance index
+    keys = ("EpiPackPrimalY<false>", "EpiPackDualZ<false>")
+    dom_bytes, dom_ms, dom_name, dom_key = max(
+        (bat, ms_at, "transpose pass (A^T z + x/y update)", keys[0]),
+        (ba, ms_a, "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "PDHG iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc,
+                   "step": (f"one warm-started l1 projection of each of the {K} instances to a "
+                            f"new 0/1 target at relative KKT tolerance {args.tol:g}, solved "
+                            f"together"),
+                   "instances": K,
+                   "precision": "FP32 data/iterates, FP64 reductions and polish",
+                   "iterations_timed": it},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("c4", dom_key), "algorithmic_bytes": dom_bytes,
+                     "kernel_ms": dom_ms},
+        "iteration_us": ms_iter * 1e3,
+        "kernel_us": {"transpose_pass": ms_at * 1e3, "row_pass": ms_a * 1e3},
+        "schedule_nnz_per_block": npb,
+        "fp32_one_at_a_time_value": seq["fp32"],
+        "speedup_vs_fp32_one_at_a_time": value / seq["fp32"],
+    }
+    if "fp64" in seq:
+        out["fp64_plain_one_at_a_time_value"] = seq["fp64"]
+        out["speedup_vs_fp64_plain"] = value / seq["fp64"]
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(workload: str, kernel_key: str):
     """HBM bytes per launch of a kernel from the committed counter summary
     (profiles/pmc_traffic_<workload>.md, its ```json block), or None."""
@@ -294,6 +422,10 @@ def main():
         print(f"bench.py measures one GPU: the projector has no multi-GPU path in this build "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c4":
+        bench_pack(args)
+        return
 
     from pdfp_amd import generators as G
     from pdfp_amd import pdfp as P

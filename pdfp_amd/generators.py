@@ -117,6 +117,21 @@ def _power_law_lengths(rng, m: int, nnz: int, exponent: float, max_len: int,
     return L
 
 
+def uneven_set_covers(count: int = 64, *, m_range=(2_000, 40_000), n_range=(4_000, 80_000),
+                      first_seed: int = 1) -> list:
+    """``count`` set-covering instances of uneven sizes, seeds first_seed..:
+    instance s has m = round(e^U[ln m_lo, ln m_hi]) rows and
+    n = round(e^U[ln n_lo, ln n_hi]) columns, drawn with its own seed, and is
+    ``set_cover(m, n, s)``."""
+    out = []
+    for s in range(first_seed, first_seed + count):
+        rng = _rng(s)
+        m = int(round(float(np.exp(rng.uniform(np.log(m_range[0]), np.log(m_range[1]))))))
+        n = int(round(float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
+        out.append(set_cover(m, n, s))
+    return out
+
+
 def mixed(m: int = 50_000, n: int = 100_000, seed: int = 1, *, min_len: int = 10,
           max_len: int = 30, nnz: int | None = None, power_law: bool = False,
           zipf_exponent: float = 1.5, binary_fraction: float = 0.7,
