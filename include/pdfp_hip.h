@@ -272,6 +272,16 @@ int pdfp_pack_warm_start(pdfp_pack* ctx, int32_t k);
  * (one params for all).  termination / total_count: K entries. */
 int pdfp_pack_solve(pdfp_pack* ctx, const pdfp_params* params, const double* primal_weight,
                     const uint8_t* active, int32_t* termination, int64_t* total_count);
+/* The continuous form (no counterpart in the reference; a pump over the
+ * pack's instances uses it), as pdfp_batch_begin / pdfp_batch_run: instance
+ * k starts a solve with its own parameters and primal weight; run iterates
+ * every begun, unfinished instance until one ends (until_any = 1) or none
+ * runs, and reports termination[K] (0 while running), total_count[K] and
+ * the number still running. */
+int pdfp_pack_begin(pdfp_pack* ctx, int32_t k, const pdfp_params* params,
+                    double primal_weight);
+int pdfp_pack_run(pdfp_pack* ctx, int32_t until_any, int32_t* termination,
+                  int64_t* total_count, int32_t* n_running);
 /* Instance k's projection (FP64, polished), vectors (ids as in
  * pdfp_get_vector) and scalars. */
 int pdfp_pack_get_x(pdfp_pack* ctx, int32_t k, double* x_scaled_out);

@@ -150,6 +150,9 @@ SIGNATURES = {
     "pdfp_pack_warm_start": (C.c_int, [_P, C.c_int32]),
     "pdfp_pack_solve": (C.c_int, [_P, C.POINTER(PdfpParams), _D, _U8, C.POINTER(C.c_int32),
                                   C.POINTER(C.c_int64)]),
+    "pdfp_pack_begin": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpParams), C.c_double]),
+    "pdfp_pack_run": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int32)]),
     "pdfp_pack_get_x": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_pack_get_vector": (C.c_int, [_P, C.c_int32, C.c_int32, _D]),
     "pdfp_pack_get_info": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpInfo)]),

@@ -415,8 +415,9 @@ struct Pack {
 
   // The block loop of pdfp_solve! (src/pdfp.jl:252-288) for every running
   // instance, in lockstep blocks of `freq` iterations, each instance with
-  // its own termination, until none runs.
-  void run() {
+  // its own termination; returns after the first block in which one ends
+  // (until_any) or when none runs.  Returns the number still running.
+  int run(bool until_any) {
     int freq = 0;
     double omega = 0.0;
     for (const PackInstance& I : in) {
@@ -451,7 +452,7 @@ struct Pack {
                               hipMemcpyDeviceToHost, stream));
       sync();
       const double now = wall_seconds();
-      bool any_flag = false;
+      bool any_flag = false, ended = false;
       for (int k = 0; k < K; ++k) {
         h_flag.p[k] = 0;
         if (!active[(size_t)k]) continue;
@@ -498,6 +499,7 @@ struct Pack {
         if (stop) {
           I.running = false;
           I.solve_seconds = now - I.t_start;
+          ended = true;
         }
       }
       if (any_flag) {
@@ -508,15 +510,19 @@ struct Pack {
             y0.p, z.p, z0.p);
         sync();  // h_flag is rewritten in the next block
       }
+      if (until_any && ended) break;
     }
     sync();
+    int n_running = 0;
+    for (const PackInstance& I : in) n_running += I.running ? 1 : 0;
+    return n_running;
   }
 
   void solve(const pdfp_params& p, const double* pw, const uint8_t* act) {
     for (PackInstance& I : in) I.running = false;
     for (int k = 0; k < K; ++k)
       if (!act || act[k]) begin(k, p, pw[k]);
-    run();
+    run(false);
   }
 
   // The FP64 polish of instance k's projection and its ℓ1 distance.
@@ -767,6 +773,28 @@ int pdfp_pack_solve(pdfp_pack* ctx, const pdfp_params* params, const double* pri
       if (termination) termination[k] = P.in[(size_t)k].termination;
       if (total_count) total_count[k] = P.in[(size_t)k].r.total_count;
     }
+  });
+}
+
+int pdfp_pack_begin(pdfp_pack* ctx, int32_t k, const pdfp_params* params,
+                    double primal_weight) {
+  return pack_guarded(ctx, [&] {
+    if (!params) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null params");
+    ctx->impl->begin(k, *params, primal_weight);
+  });
+}
+
+int pdfp_pack_run(pdfp_pack* ctx, int32_t until_any, int32_t* termination,
+                  int64_t* total_count, int32_t* n_running) {
+  return pack_guarded(ctx, [&] {
+    pdfp::Pack& P = *ctx->impl;
+    const int left = P.run(until_any != 0);
+    for (int k = 0; k < P.K; ++k) {
+      const pdfp::PackInstance& I = P.in[(size_t)k];
+      if (termination) termination[k] = I.running ? PDFP_UNSPECIFIED : I.termination;
+      if (total_count) total_count[k] = I.r.total_count;
+    }
+    if (n_running) *n_running = left;
   });
 }
 

@@ -112,6 +112,26 @@ class PDFPPack:
         self.last_total_count = [int(t) for t in total]
         return list(self.termination_reason)
 
+    def begin(self, k: int, p: SolverParameters, primal_weight: float) -> None:
+        """Start instance k's solve with its own parameters (after set_target
+        or warm_start); it runs in the next run()."""
+        cp = p.to_c()
+        self._check(self._lib.pdfp_pack_begin(self._ctx, k, C.byref(cp), float(primal_weight)))
+        self.termination_reason[k] = "UNSPECIFIED"
+
+    def run(self, until_any: bool = True) -> tuple[list, int]:
+        """Run the begun instances in lockstep blocks until one ends
+        (until_any) or none runs.  Returns (termination reasons, with
+        "UNSPECIFIED" for those still running; the number still running)."""
+        term = (C.c_int32 * self.K)()
+        total = (C.c_int64 * self.K)()
+        left = C.c_int32()
+        self._check(self._lib.pdfp_pack_run(self._ctx, int(until_any), term, total,
+                                            C.byref(left)))
+        self.termination_reason = [_lib.TERMINATION[int(t)] for t in term]
+        self.last_total_count = [int(t) for t in total]
+        return list(self.termination_reason), int(left.value)
+
     def pdhg_x(self, k: int) -> np.ndarray:
         out = np.empty(self.n[k])
         self._check(self._lib.pdfp_pack_get_x(self._ctx, k, dptr(out)))

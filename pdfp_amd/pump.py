@@ -688,6 +688,159 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
     return finish(k, False, xt)
 
 
+def feasibility_pump_pack(milps, *, seed: int = 1, time_limit: float = 1800.0,
+                          verbose: bool = False, flip_parameter: int = 20,
+                          restart_period: int = 100, swap_iteration: int = 10,
+                          gap_start: float = 1e-2, gap_decay: float = 0.98,
+                          gap_floor: float = 1e-4, residual_ratio: float = 0.1,
+                          root_lp_tolerance: float = 1e-2,
+                          projection_iteration_limit: int = 500_000,
+                          batch_policy: str = "cost", push_threshold: float = 0.01,
+                          violation_threshold: float = 5000.0,
+                          propagation_budget: int = 100_000,
+                          feasibility_tolerance: float = 1e-6, device: int = 0,
+                          scaling: str = "device") -> list:
+    """The pumps of K independent instances on one GPU, one trajectory each
+    (``feasibility_pump``'s, seed ``seed``): the root LPs are solved one by
+    one, then every instance's projections run in one block-diagonal pack
+    (PDFPPack) that iterates continuously: an instance whose projection
+    ends is rounded on a host thread while the others keep iterating, and
+    re-enters the pack with its next target at the next evaluation block.
+    An instance asked for a tolerance tighter than the FP32 form resolves
+    moves to an FP64 projector of its own.  Returns one PumpResult per
+    instance; each ``time`` counts from the start of the whole run."""
+    if batch_policy not in ("cost", "always"):
+        raise ValueError("batch_policy must be 'cost' or 'always'")
+    from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+
+    from . import pdfp as _pdfp_mod
+    from . import pdlp as _pdlp_mod
+    from .pack import PDFPPack
+
+    K = len(milps)
+    t0 = time.perf_counter()
+
+    def solve_lp(milp_, **kw):
+        return _pdlp_mod.solve_lp_relaxation(milp_, scaling=scaling, device=device, **kw)
+
+    cfgs, stats, logs, gens = [], [], [], [None] * K
+    results: list = [None] * K
+    starts = []
+    for k, milp in enumerate(milps):
+        cfg = _Config(time_limit, flip_parameter, restart_period, swap_iteration, gap_start,
+                      gap_decay, gap_floor, residual_ratio, projection_iteration_limit,
+                      batch_policy, push_threshold, violation_threshold,
+                      propagation_budget, feasibility_tolerance,
+                      _pdfp_mod.SolverParameters, t0)
+        cfg.violation = DeviceViolationCheck(milp, device)
+        st = _Counters()
+        lg: list = []
+        emit = _emitter(lg, verbose and k == 0)
+        x, dual_slack, done = _root(milp, cfg, solve_lp, root_lp_tolerance, st, emit)
+        cfgs.append(cfg)
+        stats.append(st)
+        logs.append(lg)
+        starts.append(x)
+        if done is not None:
+            results[k] = _finish(milp, True, done, st, cfg.elapsed(), emit, lg)
+        else:
+            gens[k] = _trajectory(milp, cfg, seed, x, dual_slack, st, emit)
+
+    live = [k for k in range(K) if gens[k] is not None]
+    if not live:
+        return results
+    pps = [_pdfp_mod.projector_problem(milps[k], scaling=scaling, device=device)
+           for k in range(K)]
+    pack = PDFPPack(pps, device=device)
+    mixed = [num_continuous(m) > 0 for m in milps]
+    for k in range(K):
+        if mixed[k]:
+            pack.original_primal_residual(k, pps[k].D_r)
+        pack.set_x(k, starts[k] / pps[k].D_c)
+    pw = [math.sqrt(1.0 + (len(m.int_idx) / num_constraints(m)
+                           if num_constraints(m) > 0 else 0.0)) for m in milps]
+    fp64: list = [None] * K
+    target: list = [None] * K
+    last: list = [None] * K
+
+    def finish(k, found, xt):
+        results[k] = _finish(milps[k], found, xt, stats[k], cfgs[k].elapsed(),
+                             _emitter(logs[k], False), logs[k])
+
+    def advance(k, result):
+        try:
+            return gens[k].send(result), None
+        except StopIteration as e:
+            return None, e.value
+
+    pending = {}
+    for k in live:
+        try:
+            pending[k] = gens[k].send(None)
+        except StopIteration as e:
+            finish(k, e.value[0], e.value[1])
+    pool = ThreadPoolExecutor(max_workers=max(1, min(len(live), os.cpu_count() or 1)))
+    running: set = set()
+    futures: dict = {}
+    while pending or running or futures:
+        for k in sorted(pending):
+            kind, xt, params = pending[k]
+            pp = pps[k]
+            if kind == "target":
+                target[k] = xt / pp.D_c
+            tight = min(params.eps_optimal_relative,
+                        params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR
+            if fp64[k] is None and tight:
+                st64 = _pdfp_mod.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb,
+                                           pp.var_ub, pp.w, precision="fp64", device=device)
+                if mixed[k]:
+                    _pdfp_mod.original_primal_residual(st64, pp.D_r)
+                st64.set_x(pack.pdhg_x(k))
+                _pdfp_mod.set_target(st64, target[k])
+                fp64[k] = st64
+                kind = "started"
+            if fp64[k] is not None:
+                st64 = fp64[k]
+                if kind == "target":
+                    _pdfp_mod.set_target(st64, target[k])
+                elif kind == "tighten":
+                    _pdfp_mod.warm_start(st64)
+                _pdfp_mod.pdfp_solve(st64, params, pw[k])
+                futures[k] = pool.submit(advance, k, (pp.D_c * st64.pdhg_x,
+                                                      st64.total_count))
+            else:
+                if kind == "target":
+                    pack.set_target(k, target[k])
+                else:
+                    pack.warm_start(k)
+                pack.begin(k, params, pw[k])
+                running.add(k)
+        pending = {}
+        if running:
+            terms, _ = pack.run(until_any=True)
+            for k in sorted(running):
+                if terms[k] != "UNSPECIFIED":
+                    running.discard(k)
+                    futures[k] = pool.submit(advance, k, (pps[k].D_c * pack.pdhg_x(k),
+                                                          pack.total_count(k)))
+        if futures:
+            done = [k for k, f in futures.items() if f.done()]
+            if not done and not running:
+                wait(list(futures.values()), return_when=FIRST_COMPLETED)
+                done = [k for k, f in futures.items() if f.done()]
+            for k in sorted(done):
+                req, end = futures.pop(k).result()
+                if end is None:
+                    pending[k] = req
+                else:
+                    finish(k, end[0], end[1])
+    pool.shutdown()
+    for k in range(K):
+        if results[k] is None:        # a generator that never ran to an end
+            finish(k, False, last[k] if last[k] is not None else starts[k])
+    return results
+
+
 def _print_iteration(emit, iteration, distance, violation, inner, gap, action, t):
     emit(f"{iteration:6d} {distance:12.6g} {violation:10.3g} {inner:10d} {gap:9.2e} "
          f"{action:>8s} {t:8.2f}s\n")

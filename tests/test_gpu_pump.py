@@ -66,3 +66,19 @@ def test_batched_pump_mixed_model_takes_fp64_projections():
     m = G.mixed(2000, 4000, 1)
     r = feasibility_pump_batch(m, trajectories=4, time_limit=120.0)
     assert r.found and is_feasible(m, r.x)
+
+
+def test_pack_pump_finds_feasible_for_every_instance():
+    """The four example instances and a mixed model, each with its own pump,
+    all projections in one block-diagonal pack (PDFPPack)."""
+    from pdfp_amd import generators as G
+    from pdfp_amd.pump import feasibility_pump_pack
+
+    print("pack pump", flush=True)
+    milps = [read_mps(example_path(n)) for n in ("p0201", "gen-ip054", "pk1", "markshare2")]
+    milps.append(G.mixed(2000, 4000, 2))
+    rs = feasibility_pump_pack(milps, time_limit=120.0)
+    assert len(rs) == len(milps)
+    for m, r in zip(milps, rs):
+        assert r.found and is_feasible(m, r.x), m.name
+        assert abs(r.objective - (float(m.c @ r.x) + m.c0)) <= 1e-9 * (1 + abs(r.objective))
