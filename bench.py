@@ -203,7 +203,9 @@ def time_to_first_feasible(runs: int, cpu: bool):
     (seed 1): median PumpResult.time over `runs` runs after one warm-up run,
     as the reference times it (scripts/solve.jl:88-97); FP32 and FP64 single
     trajectories and 8 concurrent trajectories on the batched projector
-    (first feasible wins, as scripts/solve.jl:121-148)."""
+    (first feasible wins, as scripts/solve.jl:121-148); and the four example
+    instances' pumps together on one pack, against the same pumps one after
+    another."""
     from pdfp_amd import generators as G
     from pdfp_amd.milp import is_feasible, read_mps
     from pdfp_amd.pump import feasibility_pump, feasibility_pump_batch
@@ -240,6 +242,31 @@ def time_to_first_feasible(runs: int, cpu: bool):
                                  "kind": "port"}
         out[name] = rec
     out["p0201"]["reference_h100_seconds"] = 0.12   # README.md:47-48
+    # The four example instances at once: their pumps on one block-diagonal
+    # pack, against the same pumps run one after another (single FP32).
+    from pdfp_amd.pump import feasibility_pump_pack
+
+    names = ("p0201", "gen-ip054", "pk1", "markshare2")
+    ex = [read_mps(os.path.join(ROOT, "tests", "golden", "examples", f"{n}.mps.gz"))
+          for n in names]
+    feasibility_pump_pack(ex[:2], time_limit=60.0)                   # warm-up
+    packs = [feasibility_pump_pack(ex, time_limit=60.0) for _ in range(runs)]
+    alls = sorted(max(r.time for r in rs) for rs in packs)
+    seqs = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        rs = [feasibility_pump(m, time_limit=60.0, precision="fp32") for m in ex]
+        seqs.append((time.perf_counter() - t, rs))
+    seqs.sort(key=lambda v: v[0])
+    out["examples_4_pack"] = {
+        "instances": list(names),
+        "pack_all_found_seconds": alls[len(alls) // 2],
+        "pack_found": all(r.found and is_feasible(m, r.x) for rs in packs
+                          for m, r in zip(ex, rs)),
+        "pack_instance_seconds": {n: r.time for n, r in zip(names, packs[0])},
+        "sequential_fp32_seconds": seqs[len(seqs) // 2][0],
+        "sequential_found": all(r.found for _, rs in seqs for r in rs),
+    }
     return out
 
 
