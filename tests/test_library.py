@@ -88,3 +88,58 @@ def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(ImportError):
         _lib.load()
+
+
+def test_pack_create_rejects_bad_arguments_before_touching_a_device():
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    colptr = np.array([0, 1, 2], dtype=np.int64)
+    rowval = np.array([0, 0], dtype=np.int64)
+    vals = np.array([1.0, 1.0])
+    b = np.array([1.0])
+    eq = np.zeros(1, dtype=np.uint8)
+    ii = np.ones(2, dtype=np.uint8)
+    lb, ub, w = np.zeros(2), np.ones(2), np.ones(2)
+    m = np.array([1, 1], dtype=np.int64)
+    n = np.array([2, 2], dtype=np.int64)
+
+    def arr(ct, a):
+        out = (C.POINTER(ct) * 2)()
+        out[0] = out[1] = a.ctypes.data_as(C.POINTER(ct))
+        return out
+
+    args = [C.byref(ctx), 2, m.ctypes.data_as(C.POINTER(C.c_int64)),
+            n.ctypes.data_as(C.POINTER(C.c_int64)), arr(C.c_int64, colptr),
+            arr(C.c_int64, rowval), arr(C.c_double, vals), 0, arr(C.c_double, b),
+            arr(C.c_uint8, eq), arr(C.c_uint8, ii), arr(C.c_double, lb), arr(C.c_double, ub),
+            arr(C.c_double, w), 0]
+    bad = list(args)
+    bad[1] = 0                                  # no instance
+    assert lib.pdfp_pack_create(*bad) == 1
+    bad = list(args)
+    bad[7] = 2                                  # index base
+    assert lib.pdfp_pack_create(*bad) == 1
+    bad = list(args)
+    bad[8] = None                               # null b array
+    assert lib.pdfp_pack_create(*bad) == 1
+    neg = np.array([1, -1], dtype=np.int64)
+    bad = list(args)
+    bad[2] = neg.ctypes.data_as(C.POINTER(C.c_int64))
+    assert lib.pdfp_pack_create(*bad) == 1
+    assert b"negative" in lib.pdfp_pack_last_error(None)
+    assert not ctx.value
+    assert lib.pdfp_pack_warm_start(None, 0) == 1
+    lib.pdfp_pack_destroy(None)
+
+
+def test_uneven_set_covers_follow_their_recipe():
+    """Sizes e^U[ln lo, ln hi] per seed, each instance a set cover of them."""
+    from pdfp_amd import generators as G
+
+    ms = G.uneven_set_covers(3, m_range=(20, 60), n_range=(40, 120), first_seed=5)
+    for s, m in zip(range(5, 8), ms):
+        rng = np.random.Generator(np.random.PCG64(s))
+        mm = int(round(float(np.exp(rng.uniform(np.log(20), np.log(60))))))
+        nn = int(round(float(np.exp(rng.uniform(np.log(40), np.log(120))))))
+        assert m.A.shape == (mm, nn)
+        assert np.array_equal(m.A.toarray(), G.set_cover(mm, nn, s).A.toarray())
