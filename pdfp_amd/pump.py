@@ -126,14 +126,11 @@ def total_integer_row_violation(milp: MILP, x) -> float:
     skip = np.zeros(m, dtype=bool)
     cont_entries = ~is_int[cols]
     skip[A.indices[cont_entries]] = True
-    total = 0.0
-    b = milp.b
     le = milp.row_sense == "le"
-    for i in range(m):
-        if skip[i]:
-            continue
-        total += max(0.0, activity[i] - b[i]) if le[i] else abs(activity[i] - b[i])
-    return total
+    d = activity - milp.b
+    viol = np.where(le, np.maximum(d, 0.0), np.abs(d))[~skip]
+    # the reference's left-to-right sum over the rows (cumsum adds in order)
+    return float(np.cumsum(viol)[-1]) if len(viol) else 0.0
 
 
 def flip_farthest(xt: np.ndarray, x, milp: MILP, TT: int) -> None:

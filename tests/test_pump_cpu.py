@@ -236,3 +236,32 @@ def test_vectorised_pump_steps_equal_element_loops(seed):
         perturb(p1, xb, mb, r1, [], isg, RestartLadder(0.0, 0, 20, 10), 5)
         _loop_perturb_binaries(p2, xb, mb, r2, isg)
         assert np.array_equal(p1, p2) and r1.random() == r2.random()
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_total_integer_row_violation_equals_row_loop(seed):
+    """src/pump.jl:98-118: the CSC activity and the left-to-right sum over
+    the rows without continuous variables, as a plain loop restates it."""
+    from pdfp_amd.pump import total_integer_row_violation
+
+    m = G.mixed(80, 160, seed) if seed == 1 else G.general_integer(80, 160, seed)
+    rng = np.random.default_rng(seed)
+    is_int = m.is_int
+    A = m.A.tocsc()
+    for _ in range(5):
+        x = np.round(rng.normal(0.5, 2.0, A.shape[1]))
+        act = np.zeros(A.shape[0])
+        skip = np.zeros(A.shape[0], dtype=bool)
+        for j in range(A.shape[1]):
+            for k in range(A.indptr[j], A.indptr[j + 1]):
+                i = A.indices[k]
+                act[i] += A.data[k] * x[j]
+                if not is_int[j]:
+                    skip[i] = True
+        total = 0.0
+        for i in range(A.shape[0]):
+            if skip[i]:
+                continue
+            d = act[i] - m.b[i]
+            total += max(0.0, d) if m.row_sense[i] == "le" else abs(d)
+        assert total_integer_row_violation(m, x) == total
