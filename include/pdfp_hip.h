@@ -312,10 +312,18 @@ typedef struct pdlp_info {
   int64_t total_count;
   int32_t termination_reason;
   int32_t n_restarts;
+  int32_t small_path;        /* 1: the last solve ran each block of iterations  */
+                             /*   in one workgroup (operators held in LDS)      */
 } pdlp_info;
 
 /* PDLPState(sp; use_gpu)  src/pdlp.jl:82-96 on an already scaled problem
  * (c, var_lb/ub, con_lb/ub of the ScaledProblem, src/preconditioner.jl:13-24). */
+/* Kernel path of the root LP (no counterpart in the reference): 0 (the
+ * default) runs each block of iterations in one workgroup when the instance
+ * fits in LDS (24 nnz + 52 n + 36 m + 8 bytes ≤ 62 KiB), else the general
+ * passes; 1 always runs the general passes. */
+int pdlp_set_kernel_path(pdlp_ctx* ctx, int32_t path);
+
 int pdlp_create(pdlp_ctx** out, int64_t m, int64_t n, int64_t nnz,
                 const int64_t* colptr, const int64_t* rowval,
                 const double* nzval, int32_t index_base, const double* c,

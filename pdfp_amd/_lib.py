@@ -81,6 +81,7 @@ class PdlpInfo(C.Structure):
         ("total_count", C.c_int64),
         ("termination_reason", C.c_int32),
         ("n_restarts", C.c_int32),
+        ("small_path", C.c_int32),
     ]
 
 
@@ -108,6 +109,7 @@ SIGNATURES = {
                               C.c_int32, _D, _D, _D, _D, _D, C.c_double, C.c_double, C.c_int32]),
     "pdlp_set_cupdlpx_termination": (C.c_int, [_P, _D, _D]),
     "pdlp_solve": (C.c_int, [_P, C.POINTER(PdfpParams), C.POINTER(C.c_int32), _I64]),
+    "pdlp_set_kernel_path": (C.c_int, [_P, C.c_int32]),
     "pdlp_get_primal": (C.c_int, [_P, _D]),
     "pdlp_get_dual_slack": (C.c_int, [_P, _D]),
     "pdlp_get_info": (C.c_int, [_P, C.POINTER(PdlpInfo)]),

@@ -250,6 +250,13 @@ struct Batch {
   // reduction; keep the fastest.
   void autotune() {
     if (nnz == 0) return;
+    constexpr int kRows0 = kBlock / kBatch;
+    if (nnz < kAutotuneMinNnz) {  // see kAutotuneMinNnz
+      A.schedule(512, stream, kRows0, 32);
+      At.schedule(512, stream, kRows0, 32);
+      A.persist = At.persist = 0;
+      return;
+    }
     std::vector<bool> none((size_t)K, false);
     upload_steps(1.0, none);
     hipEvent_t e0, e1;

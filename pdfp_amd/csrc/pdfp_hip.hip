@@ -177,7 +177,7 @@ struct Projector final : ProjectorBase {
   // optimum away from the plain pass's; coordinate-wise, Aᵀ first.  The
   // iterates are zeroed afterwards.
   void tune_fused() {
-    if (nnz == 0) return;
+    if (nnz < kAutotuneMinNnz) return;
     h_sp.p[0] = StepParams{1e-3, 1.0, 1.0, 0.0};
     PDFP_HIP(hipMemcpyAsync(d_sp.p, h_sp.p, sizeof(StepParams), hipMemcpyHostToDevice, stream));
     hipEvent_t e0, e1;

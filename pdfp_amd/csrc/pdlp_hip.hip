@@ -18,6 +18,7 @@
 #include "host_util.h"
 #include "pdlp_kernels.h"
 #include "small_kernels.h"
+#include "small_lp_kernels.h"
 #include "solver_common.h"
 
 namespace pdfp {
@@ -28,6 +29,7 @@ struct LpBase {
   virtual int solve(const pdfp_params& p) = 0;
   virtual void get(int which, double* out) = 0;
   virtual void info(pdlp_info* out) = 0;
+  virtual void set_path(int path) = 0;
 };
 
 template <typename T>
@@ -57,6 +59,12 @@ struct LpSolver final : LpBase {
   int termination = PDFP_UNSPECIFIED;
   int n_restarts = 0;
   double solve_seconds = 0.0;
+  // FP64 on a small instance: whole blocks in one workgroup
+  // (small_lp_kernels.h); path 1 forces the general passes.
+  bool small_ok = false;
+  int small_lanes_a = 1, small_lanes_t = 1;
+  int path = 0;
+  bool small_used = false;
 
   LpSolver(int64_t m_, int64_t n_, const int64_t* colptr, const int64_t* rowval,
            const double* nzval, int32_t base, const double* c_, const double* lb_,
@@ -73,6 +81,13 @@ struct LpSolver final : LpBase {
     // pick the nonzeros-per-block of each operator's schedule by timing
     autotune_schedule(A, n, stream);
     autotune_schedule(At, m, stream);
+    if constexpr (std::is_same<T, double>::value) {
+      small_ok = m > 0 && n > 0 && nnz > 0 && small_lp_lds_bytes(m, n, nnz) <= kSmallLdsMax;
+      if (small_ok) {
+        small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
+        small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);
+      }
+    }
     h_c.assign(c_, c_ + n);
     h_clb.assign(clb_, clb_ + m);
     h_cub.assign(cub_, cub_ + m);
@@ -157,6 +172,51 @@ struct LpSolver final : LpBase {
   void evaluate(int slot) {
     if (cupdlpx) evaluate_t<true>(slot);
     else evaluate_t<false>(slot);
+  }
+
+  void set_path(int p) override {
+    if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
+    path = p;
+  }
+
+  // One block of `freq` iterations and its evaluation(s) in one workgroup.
+  void small_block(int freq, bool restart_eval) {
+    if constexpr (std::is_same<T, double>::value) {
+      SmallLpArgs a;
+      a.A = SmallCsrT<double>{A.rowptr.p, A.col.p, A.val.p};
+      a.At = SmallCsrT<double>{At.rowptr.p, At.col.p, At.val.p};
+      a.m = (int)m;
+      a.n = (int)n;
+      a.nnz = (int)nnz;
+      a.lanes_a = small_lanes_a;
+      a.lanes_t = small_lanes_t;
+      a.freq = freq;
+      a.restart_eval = restart_eval ? 1 : 0;
+      a.sp = d_sp.p;
+      a.x = x.p;
+      a.y = y.p;
+      a.x0 = x0.p;
+      a.y0 = y0.p;
+      a.c = c.p;
+      a.lb = lb.p;
+      a.ub = ub.p;
+      a.clb = clb.p;
+      a.cub = cub.p;
+      a.rx = rx.p;
+      a.px = px.p;
+      a.ds = ds.p;
+      a.py = py.p;
+      a.ry = ry.p;
+      a.col_unscale = cupdlpx ? col_unscale.p : nullptr;
+      a.row_unscale = cupdlpx ? row_unscale.p : nullptr;
+      a.sums = sums.p;
+      const size_t lds = (size_t)small_lp_lds_bytes(m, n, nnz);
+      if (cupdlpx)
+        k_small_lp_block<kSmallBlock, true><<<1, kSmallBlock, lds, stream>>>(a);
+      else
+        k_small_lp_block<kSmallBlock, false><<<1, kSmallBlock, lds, stream>>>(a);
+      PDFP_HIP(hipGetLastError());
+    }
   }
 
   // initialize_step_size_and_primal_weight!  src/pdlp.jl:292-316
@@ -282,14 +342,19 @@ struct LpSolver final : LpBase {
     bool do_restart = false;
     n_restarts = 0;
     termination = PDFP_UNSPECIFIED;
+    small_used = small_ok && path == 0;
     while (r.total_count < p.iteration_limit) {
       upload_step_params(omega);
       const bool restart_eval = do_restart;
-      step(1, true);
-      if (restart_eval) evaluate(1);
-      for (int i = 2; i <= freq - 1; ++i) step(i, false);
-      step(freq, true);
-      evaluate(0);
+      if (small_used) {
+        small_block(freq, restart_eval);
+      } else {
+        step(1, true);
+        if (restart_eval) evaluate(1);
+        for (int i = 2; i <= freq - 1; ++i) step(i, false);
+        step(freq, true);
+        evaluate(0);
+      }
       PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 64, hipMemcpyDeviceToHost,
                               stream));
       sync();
@@ -366,6 +431,7 @@ struct LpSolver final : LpBase {
     o->total_count = r.total_count;
     o->termination_reason = termination;
     o->n_restarts = n_restarts;
+    o->small_path = small_used ? 1 : 0;
   }
 };
 
@@ -403,6 +469,13 @@ extern "C" {
 const char* pdlp_last_error(const pdlp_ctx* ctx) {
   if (!ctx) return pdfp::g_create_error.c_str();
   return ctx->err.c_str();
+}
+
+int pdlp_set_kernel_path(pdlp_ctx* ctx, int32_t path) {
+  return lp_guarded(ctx, [&] {
+    if (!ctx->impl) throw pdfp::Error(PDFP_ERR_STATE, "no solver");
+    ctx->impl->set_path(path);
+  });
 }
 
 int pdlp_create(pdlp_ctx** out, int64_t m, int64_t n, int64_t nnz, const int64_t* colptr,

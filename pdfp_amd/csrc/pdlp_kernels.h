@@ -19,6 +19,40 @@ namespace pdfp {
 
 __device__ __forceinline__ bool finite(double v) { return fabs(v) < INFINITY; }
 
+// ---- the per-element math of one PDLP iteration (FP64) ---------------------
+// Shared by the fused epilogues below and by the one-workgroup block kernel
+// (small_lp_kernels.h), so that both run the same operations.
+
+struct LpPrimalOut {
+  double temp, p, r, x;  // dual slack (major steps): (p - temp) / τ
+};
+
+// K4 (src/pdlp_kernels.jl:4-21) for one column with s = (Aᵀy)_j.
+__device__ __forceinline__ LpPrimalOut lp_primal_math(const StepScalars& sc, double s, double cx,
+                                                      double x0, double c, double lb, double ub) {
+  LpPrimalOut o;
+  o.temp = cx - sc.tau * (c - s);
+  o.p = jclamp(o.temp, lb, ub);
+  o.r = 2.0 * o.p - cx;
+  o.x = sc.weight * (sc.omega * o.r + sc.omw * cx) + sc.omwt * x0;
+  return o;
+}
+
+struct LpDualOut {
+  double p, r, y;
+};
+
+// K5 (src/pdlp_kernels.jl:35-51) for one row with s = (A x̄)_i.
+__device__ __forceinline__ LpDualOut lp_dual_math(const StepScalars& sc, double s, double cy,
+                                                  double y0, double clb, double cub) {
+  LpDualOut o;
+  const double t = cy / sc.sigma - s;
+  o.p = (t - jclamp(t, -cub, -clb)) * sc.sigma;
+  o.r = 2.0 * o.p - cy;
+  o.y = sc.weight * (sc.omega * o.r + sc.omw * cy) + sc.omwt * y0;
+  return o;
+}
+
 // Transpose pass of one iteration (K4, src/pdlp_kernels.jl:4-21).
 template <typename T, bool MAJOR>
 struct EpiLpPrimal {
@@ -39,15 +73,13 @@ struct EpiLpPrimal {
   __device__ __forceinline__ void operator()(int j, const Pre& q, const double* s,
                                              double*) const {
     const StepScalars sc = step_scalars(sp, k_offset);
-    const double cx = (double)q.x;
-    const double temp = cx - sc.tau * ((double)q.c - s[0]);
-    const double p = jclamp(temp, (double)q.lb, (double)q.ub);
-    const double r = 2.0 * p - cx;
-    rx[j] = (T)r;
-    x[j] = (T)(sc.weight * (sc.omega * r + sc.omw * cx) + sc.omwt * (double)q.x0);
+    const LpPrimalOut o = lp_primal_math(sc, s[0], (double)q.x, (double)q.x0, (double)q.c,
+                                         (double)q.lb, (double)q.ub);
+    rx[j] = (T)o.r;
+    x[j] = (T)o.x;
     if (MAJOR) {
-      px[j] = (T)p;
-      ds[j] = (T)((p - temp) / sc.tau);
+      px[j] = (T)o.p;
+      ds[j] = (T)((o.p - o.temp) / sc.tau);
     }
   }
 };
@@ -70,14 +102,12 @@ struct EpiLpDual {
   __device__ __forceinline__ void operator()(int i, const Pre& q, const double* s,
                                              double*) const {
     const StepScalars sc = step_scalars(sp, k_offset);
-    const double cy = (double)q.y;
-    const double t = cy / sc.sigma - s[0];
-    const double p = (t - jclamp(t, -(double)q.cub, -(double)q.clb)) * sc.sigma;
-    const double r = 2.0 * p - cy;
-    y[i] = (T)(sc.weight * (sc.omega * r + sc.omw * cy) + sc.omwt * (double)q.y0);
+    const LpDualOut o = lp_dual_math(sc, s[0], (double)q.y, (double)q.y0, (double)q.clb,
+                                     (double)q.cub);
+    y[i] = (T)o.y;
     if (MAJOR) {
-      py[i] = (T)p;
-      ry[i] = (T)r;
+      py[i] = (T)o.p;
+      ry[i] = (T)o.r;
     }
   }
 };

@@ -25,6 +25,10 @@ constexpr uint64_t kSvSeed = 20240517ull;
 constexpr double kJuliaEps = 2.220446049250313e-16;
 // Nonzeros staged per workgroup: the autotuner picks one per operator.
 constexpr int kNpbChoices[] = {256, 512, 1024, 2048};
+// Below this many nonzeros a pass is a few microseconds whatever its
+// schedule, and timing the choices (a few milliseconds) would cost more than
+// any of them saves: such operators keep NPB 512 and one workgroup per block.
+constexpr int64_t kAutotuneMinNnz = 20000;
 
 extern std::string g_create_error;  // message of the last failed *_create
 
@@ -322,6 +326,11 @@ struct EpiStoreDot {
 template <typename TV>
 int autotune_schedule(DevCsr<TV>& M, int64_t ncols, hipStream_t s) {
   if (M.nblocks == 0 || M.nnz == 0) return M.npb;
+  if (M.nnz < kAutotuneMinNnz) {
+    M.schedule(512, s);
+    M.persist = 0;
+    return 512;
+  }
   DevBuf<TV> x, y;
   x.alloc((size_t)std::max<int64_t>(ncols, 1));
   y.alloc((size_t)std::max<int32_t>(M.nrows, 1));

@@ -73,6 +73,15 @@ class PDLPState:
     def total_count(self) -> int:
         return int(self.info().total_count)
 
+    def set_kernel_path(self, path: str) -> None:
+        """"auto" runs each block of iterations in one workgroup when the
+        instance fits in LDS (the default); "general" always runs the
+        general passes."""
+        codes = {"auto": 0, "general": 1}
+        if path not in codes:
+            raise ValueError("kernel path must be 'auto' or 'general'")
+        self._check(self._lib.pdlp_set_kernel_path(self._ctx, codes[path]))
+
 
 def cupdlpx_termination(s: PDLPState, D_r, D_c) -> PDLPState:
     """src/pdlp.jl:105-110."""
@@ -95,13 +104,16 @@ def pdlp_solve(s: PDLPState, p: SolverParameters) -> str:
 
 
 def solve_lp_relaxation(milp: MILP, *, tolerance: float, time_limit: float,
-                        cupdlpx_termination: bool, device: int = 0, scaling: str = "device"):
+                        cupdlpx_termination: bool, device: int = 0, scaling: str = "device",
+                        kernel_path: str = "auto"):
     """src/pdlp.jl:381-393: the LP relaxation by PDLP.  Returns
-    (x in original units, dual_slack in scaled units, status, iterations)."""
+    (x in original units, dual_slack in scaled units, status, iterations).
+    ``kernel_path``: see PDLPState.set_kernel_path."""
     con_lb = np.where(milp.row_sense == "le", -np.inf, milp.b)
     sp = scale_problem(milp.A, milp.c.copy(), milp.l.copy(), milp.u.copy(), con_lb,
                        milp.b.copy(), backend=scaling, device=device)
     s = PDLPState(sp, device=device)
+    s.set_kernel_path(kernel_path)
     if cupdlpx_termination:
         _cupdlpx_termination(s, sp.D_r, sp.D_c)
     p = SolverParameters(iteration_limit=5_000_000, eps_optimal_relative=tolerance,

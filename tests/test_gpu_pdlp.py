@@ -19,13 +19,14 @@ from .helpers import build_instance, root_lp_fixtures
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("path", ["auto", "general"])
 @pytest.mark.parametrize("case", root_lp_fixtures(), ids=lambda c: c["name"])
-def test_root_lp_matches_highs_and_oracle(case):
-    print("pdlp", case["name"], flush=True)
+def test_root_lp_matches_highs_and_oracle(case, path):
+    print("pdlp", case["name"], path, flush=True)
     milp = build_instance(case["spec"])
     mixed = num_continuous(milp) > 0
     x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-6, time_limit=60.0,
-                                          cupdlpx_termination=mixed)
+                                          cupdlpx_termination=mixed, kernel_path=path)
     xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-6, cupdlpx_termination=mixed)
     assert st == so == "OPTIMAL"
     assert it == ito
@@ -38,14 +39,38 @@ def test_root_lp_matches_highs_and_oracle(case):
     assert np.all(np.sign(ds[big]) == np.sign(dso[big]))
 
 
-def test_root_lp_p0201_at_pump_tolerance():
+@pytest.mark.parametrize("path", ["auto", "general"])
+def test_root_lp_p0201_at_pump_tolerance(path):
     """README.md:38-39: p0201's root LP at tolerance 1e-2."""
     milp = build_instance({"family": "example", "name": "p0201"})
     x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-2, time_limit=60.0,
-                                          cupdlpx_termination=False)
+                                          cupdlpx_termination=False, kernel_path=path)
     xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-2)
     assert st == so == "OPTIMAL" and it == ito
     assert abs(float(milp.c @ x) - 6875.0) <= 0.05 * 6875.0
+
+
+def test_root_lp_small_path_runs_on_small_instances():
+    """p0201 fits in LDS (24·nnz + 52n + 36m + 8 bytes ≤ 62 KiB): the
+    default path runs whole blocks in one workgroup, and agrees with the
+    general passes on the iteration count and, to FP64 summation-order
+    rounding, on the point."""
+    milp = build_instance({"family": "example", "name": "p0201"})
+    con_lb = np.where(milp.row_sense == "le", -np.inf, milp.b)
+    sp = scale_problem(milp.A, milp.c.copy(), milp.l.copy(), milp.u.copy(), con_lb,
+                       milp.b.copy())
+    p = SolverParameters(eps_optimal_relative=1e-6, eps_feasible_relative=1e-6,
+                         iteration_limit=100_000)
+    out = {}
+    for path in ("auto", "general"):
+        s = L.PDLPState(sp)
+        s.set_kernel_path(path)
+        assert L.pdlp_solve(s, p) == "OPTIMAL"
+        out[path] = (s.info(), s.pdhg_primal)
+        s.close()
+    assert out["auto"][0].small_path == 1 and out["general"][0].small_path == 0
+    assert out["auto"][0].total_count == out["general"][0].total_count
+    np.testing.assert_allclose(out["auto"][1], out["general"][1], rtol=0, atol=1e-9)
 
 
 def test_root_lp_infeasible_detected():
