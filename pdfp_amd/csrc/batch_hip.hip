@@ -78,8 +78,8 @@ struct Batch {
     nnz = (int64_t)ha.col.size();
     A.upload(ha, stream);
     At.upload(ht, stream);
-    // eight gathered values per nonzero: the staged products need 64 bytes
-    // each, so blocks hold 256 or 512 nonzeros (autotune() picks)
+    // eight gathered FP32 values and a_ij staged per nonzero (36 bytes), 32
+    // rows per block: blocks hold 256 or 512 nonzeros (autotune() picks)
     A.schedule(256, stream, kBlock / kBatch);
     At.schedule(256, stream, kBlock / kBatch);
     tr.resize((size_t)K);
@@ -215,8 +215,8 @@ struct Batch {
     warm_start(k);
   }
 
-  // The batched passes stage 8 products per nonzero: 256 or 512 nonzeros
-  // per block (48 KB of LDS at 512).
+  // The batched passes stage 8 FP32 values and a_ij per nonzero: 256 or 512
+  // nonzeros per block (about 11 or 21 KB of LDS with the row arrays).
   template <class G, class E>
   void launch_batch(const DevCsr<float>& M, G g, E e, hipEvent_t t0 = nullptr,
                     hipEvent_t t1 = nullptr) {
