@@ -578,7 +578,12 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": dom_bytes,
                      "kernel_ms": dom_ms, "gather_ceiling_us": dom_ceil,
-                     "frac_of_gather_ceiling": dom_ceil / (dom_ms * 1e3)},
+                     "frac_of_gather_ceiling": dom_ceil / (dom_ms * 1e3),
+                     # the counted bytes (line fills of the gathers included) per second
+                     "traffic_gbs": (None if traffic is None
+                                     else traffic / (dom_ms * 1e-3) / 1e9),
+                     "traffic_frac": (None if traffic is None
+                                      else traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS)},
         "iteration_us": head["ms_iter"] * 1e3,
         "kernel_us": {"transpose_pass": head["ms_at"] * 1e3, "row_pass": head["ms_a"] * 1e3},
         "gather_ceiling_us": ceil,
