@@ -199,9 +199,12 @@ __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
   }
   __syncthreads();
 
+  // the block's step parameters, read once (the loop's stores to HBM would
+  // otherwise make the compiler reload them every iteration)
+  const StepParams sp = *a.sp;
   for (int it = 1; it <= a.freq; ++it) {
     const bool major = it == a.freq || (it == 1 && a.first_major);
-    const StepScalars c = step_scalars(a.sp, it);
+    const StepScalars c = step_scalars(&sp, it);
     // columns: s_j = (Aᵀ z)_j, then the x / y steps
     const int Lt = a.lanes_t, gt = BLOCK / Lt, lt = tid & (Lt - 1);
     for (int j0 = 0; j0 < n; j0 += gt) {

@@ -170,9 +170,10 @@ __global__ __launch_bounds__(BLOCK) void k_small_lp_block(SmallLpArgs a) {
   }
   __syncthreads();
 
+  const StepParams sp = *a.sp;  // read once (see k_small_block)
   for (int it = 1; it <= a.freq; ++it) {
     const bool major = it == 1 || it == a.freq;
-    const StepScalars sc = step_scalars(a.sp, it);
+    const StepScalars sc = step_scalars(&sp, it);
     // columns: s_j = (Aᵀ y)_j, then K4
     const int Lt = a.lanes_t, gt = BLOCK / Lt, lt = tid & (Lt - 1);
     for (int j0 = 0; j0 < n; j0 += gt) {
