@@ -97,6 +97,7 @@ def run_projections(s, pp, milp, targets, tol):
     iters = 0
     t_total = 0.0
     statuses = []
+    quality = []   # per projection: ℓ1 distance and FP64-evaluated KKT terms
     pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
     for xt in targets:
         p = P.SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
@@ -107,6 +108,10 @@ def run_projections(s, pp, milp, targets, tol):
         t_total += time.perf_counter() - t0
         iters += s.total_count
         statuses.append(st)
+        inf = s.info()
+        quality.append({"l1": inf.l1_distance, "rel_primal": inf.relative_primal_residual,
+                        "rel_dual": inf.relative_dual_residual, "rel_gap": inf.relative_l1_gap})
+    run_projections.quality = quality
     return iters, t_total, statuses
 
 
@@ -488,11 +493,12 @@ def main():
         print(f"[bench] {prec} warmup {time.perf_counter() - t0:.2f}s", file=sys.stderr,
               flush=True)
         it, dt, st = run_projections(s, pp, milp, targets, args.tol)
+        quality = run_projections.quality
         ms_at, ms_a = s.time_kernels(200)
         ms_iter = s.time_iterations(400)
         info = s.info()
         results[prec] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
-                             ms_iter=ms_iter, info=info,
+                             ms_iter=ms_iter, info=info, quality=quality,
                              npb={"A": info.npb_a, "At": info.npb_at,
                                   "A_persistent_wg_per_cu": info.persist_a,
                                   "At_persistent_wg_per_cu": info.persist_at})
@@ -603,6 +609,15 @@ def main():
         out["speedup_vs_fp64_plain"] = value / v64
         out["speedup_fp32_single_vs_fp64_plain"] = out["fp32_single_value"] / v64
         out["fp64_iteration_us"] = r64["ms_iter"] * 1e3
+        # each FP32 projection against the FP64 form's projection of the same
+        # target from the same start (the FP64 form follows the reference's
+        # arithmetic and matches the C oracle's iterations)
+        out["fp32_vs_fp64_per_projection"] = [
+            {"l1_fp32": a["l1"], "l1_fp64": b["l1"],
+             "l1_rel_diff": abs(a["l1"] - b["l1"]) / max(1.0, abs(b["l1"])),
+             "fp32_rel_primal": a["rel_primal"], "fp32_rel_dual": a["rel_dual"],
+             "fp32_rel_gap": a["rel_gap"]}
+            for a, b in zip(r32["quality"], r64["quality"])]
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(pp, milp, targets[0], args.cpu_iterations)
