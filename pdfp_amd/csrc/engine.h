@@ -366,7 +366,19 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
           const int lr = tid / EL, q = tid % EL;
           double s = 0.0;
           const int k1 = off[lr + 1];
-          for (int k = off[lr]; k < k1; ++k) s += (double)sa[k] * (double)sraw[k * NV + q];
+          // four nonzeros' LDS loads in flight at a time; the sum keeps the
+          // ascending order
+          int k = off[lr];
+          for (; k + 4 <= k1; k += 4) {
+            const float a0 = sa[k], a1 = sa[k + 1], a2 = sa[k + 2], a3 = sa[k + 3];
+            const float v0 = sraw[k * NV + q], v1 = sraw[(k + 1) * NV + q];
+            const float v2 = sraw[(k + 2) * NV + q], v3 = sraw[(k + 3) * NV + q];
+            s += (double)a0 * (double)v0;
+            s += (double)a1 * (double)v1;
+            s += (double)a2 * (double)v2;
+            s += (double)a3 * (double)v3;
+          }
+          for (; k < k1; ++k) s += (double)sa[k] * (double)sraw[k * NV + q];
           hold(pre);
           epi(r0 + lr, q, pre, s, acc);
         }
