@@ -394,8 +394,23 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
 #pragma unroll
         for (int q = 0; q < NV; ++q) s[q] = 0.0;
         const int k1 = off[lr + 1];
+        int k = off[lr] + lane;
+        if constexpr (!kRaw) {
+          // four strides' products in flight at a time; the sum keeps its order
+          for (; k + 3 * tpr < k1; k += 4 * tpr) {
+            double p[4][NV];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int q = 0; q < NV; ++q) p[u][q] = stage[q][k + u * tpr];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+              for (int q = 0; q < NV; ++q) s[q] += p[u][q];
+          }
+        }
 #pragma unroll 1
-        for (int k = off[lr] + lane; k < k1; k += tpr) {
+        for (; k < k1; k += tpr) {
           if constexpr (kRaw) {
             const double a = (double)sa[k];
             const float4* v4 = reinterpret_cast<const float4*>(sraw + k * NV);
