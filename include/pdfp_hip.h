@@ -113,6 +113,8 @@ typedef struct pdfp_info {
   int32_t lane_rows_a;       /* batch: blocks whose rows hold at most this many */
   int32_t lane_rows_at;      /*   nonzeros are reduced one lane per (row,       */
                              /*   trajectory); 0: none (autotuned)              */
+  int32_t block_graph;       /* 1: the last solve replayed each block of the    */
+                             /*   general passes as a captured HIP graph        */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */
@@ -236,6 +238,12 @@ int pdfp_batch_time_kernels(pdfp_batch* ctx, int32_t iterations, double* ms_at,
  * reduction of each pass (0: off). */
 int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int32_t persist,
                             int32_t lane_rows_a, int32_t lane_rows_at);
+
+/* Block graphs of the batch (no counterpart in the reference), as
+ * pdfp_set_block_graph: 1 (the default) replays each block of iterations,
+ * with its evaluations and readback, as one captured HIP graph; 0 launches
+ * every kernel on the stream.  Both give identical results. */
+int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on);
 void pdfp_batch_destroy(pdfp_batch* ctx);
 
 /* ---- pack of independent instances ------------------------------------ */
@@ -384,6 +392,23 @@ int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
  * general passes; 1 always runs the general passes.  The FP64 form ignores
  * it. */
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);
+
+/* Block graphs of the FP32 form (no counterpart in the reference): 1 (the
+ * default) captures each block of `termination_evaluation_frequency`
+ * iterations of the general passes, with its evaluation and the readback of
+ * its sums, as one HIP graph on first use and replays it for every later
+ * block of the same shape; 0 launches every kernel on the stream.  Both give
+ * identical results.  The one-workgroup path and the FP64 form ignore it. */
+int pdfp_set_block_graph(pdfp_ctx* ctx, int32_t on);
+
+/* Replaces the autotuned schedule of the two passes (no counterpart in the
+ * reference; for tests and tuning): nonzeros per workgroup of the A and Aᵀ
+ * passes (256, 512, 1024 or 2048) and persistent workgroups per CU of each
+ * (0: one workgroup per row block; at most 16).  The schedule sets the order
+ * of each row's sum, so two states give bit-identical iterates only on the
+ * same schedule. */
+int pdfp_set_schedule(pdfp_ctx* ctx, int32_t npb_a, int32_t npb_at, int32_t persist_a,
+                      int32_t persist_at);
 
 /* ---- the pump's rounding (host) -------------------------------------------
  * Constraint-propagation rounding (FP 2.0's CPR) and the guarded two-phase

@@ -65,6 +65,7 @@ class PdfpInfo(C.Structure):
         ("small_path", C.c_int32),
         ("lane_rows_a", C.c_int32),
         ("lane_rows_at", C.c_int32),
+        ("block_graph", C.c_int32),
     ]
 
 
@@ -121,6 +122,8 @@ SIGNATURES = {
     "pdfp_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_device_bytes": (C.c_int64, [_P]),
     "pdfp_set_kernel_path": (C.c_int, [_P, C.c_int32]),
+    "pdfp_set_block_graph": (C.c_int, [_P, C.c_int32]),
+    "pdfp_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "pdfp_batch_last_error": (C.c_char_p, [_P]),
     "pdfp_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int64, C.c_int64, C.c_int64,
                                     _I64, _I64, _D, C.c_int32, _D, _U8, _U8, _D, _D, _D,
@@ -141,6 +144,7 @@ SIGNATURES = {
     "pdfp_batch_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_batch_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32]),
+    "pdfp_batch_set_block_graph": (C.c_int, [_P, C.c_int32]),
     "pdfp_pack_last_error": (C.c_char_p, [_P]),
     "pdfp_pack_create": (C.c_int, [C.POINTER(_P), C.c_int32, _I64, _I64, C.POINTER(_I64),
                                    C.POINTER(_I64), C.POINTER(_D), C.c_int32, C.POINTER(_D),

@@ -127,6 +127,12 @@ class PDFPBatch:
     def total_count(self, k: int) -> int:
         return int(self.info(k).total_count)
 
+    def set_block_graph(self, on: bool) -> None:
+        """Replay each block of iterations (with its evaluations) as one
+        captured HIP graph (the default), or launch every kernel on the
+        stream.  Both give identical results."""
+        self._check(self._lib.pdfp_batch_set_block_graph(self._ctx, int(bool(on))))
+
     def set_schedule(self, npb_a: int, npb_at: int, persist: int = 0, lane_rows_a: int = 0,
                      lane_rows_at: int = 0) -> None:
         """Replace the autotuned schedule of the batched passes (tests and

@@ -57,6 +57,21 @@ struct Batch {
   DevBuf<double2> d_wt;    // Halpern weights [k_offset][8] of the current block
   PinnedBuf<double2> h_wt;
   int wt_rows = 0;
+  // the evaluation masks of the current block: [0] the running
+  // trajectories, [1] those evaluated after a restart
+  DevBuf<int32_t> d_mask;
+  PinnedBuf<int32_t> h_mask;
+  // Each block of iterations, its evaluations and the readback of its sums
+  // captured once as a HIP graph and replayed (graph_on), one graph per
+  // (freq, restart evaluation) shape; the launches read the block's scalars,
+  // weights and masks from device memory that the host writes before each
+  // replay.
+  struct BlockGraph {
+    hipGraphExec_t exec = nullptr;
+    int freq = 0;
+  };
+  BlockGraph graphs[2];
+  bool graph_on = true;
   PinnedBuf<double> h_sums;
   std::vector<double> h_w, h_b;
   double b_norm = 0.0, b_norm_original = NAN, dual_den = 1.0;
@@ -133,12 +148,14 @@ struct Batch {
     sums.upload(h_sums.p, (size_t)kBatch * 32 + 16, stream);
     d_sp.alloc(kBatch);
     h_sp.alloc(kBatch);
+    d_mask.alloc(2);
+    h_mask.alloc(2);
     for (int q = 0; q < kBatch; ++q) h_sp.p[q] = BatchStep{1.0, 1.0, 1.0, 0.0, 0.0, {0, 0, 0}};
     for (auto& t : tr) t.h_xt.assign((size_t)n, 0.0);
     autotune();
     part_t.alloc((size_t)std::max<int64_t>((n + kEvalRows - 1) / kEvalRows, 1) * kBatch *
                  kEvalColTerms);
-    // the row terms run on A's schedule (evaluate_all grows this if a new
+    // the row terms run on A's schedule (set_schedule grows this if a new
     // schedule has more blocks)
     part_a.alloc((size_t)std::max<int32_t>(A.nblocks, 1) * kBatch * kEvalRowTerms);
     PDFP_HIP(hipStreamSynchronize(stream));
@@ -147,7 +164,17 @@ struct Batch {
   ~Batch() {
     if (stream) {
       (void)hipStreamSynchronize(stream);
+      drop_graphs();
       (void)hipStreamDestroy(stream);
+    }
+  }
+
+  // The captured blocks hold kernel arguments (buffers, schedules, flags):
+  // dropped whenever one of them changes.
+  void drop_graphs() {
+    for (BlockGraph& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
     }
   }
 
@@ -181,6 +208,7 @@ struct Batch {
       acc += v * v;
     }
     b_norm_original = std::sqrt(acc);
+    drop_graphs();  // the evaluation's epilogue takes row_unscale
     row_unscale.alloc((size_t)std::max<int64_t>(m, 1));
     row_unscale.upload(inv.data(), (size_t)m, stream);
     sync();
@@ -315,19 +343,22 @@ struct Batch {
         persist > 16 || lane_a < 0 || lane_at < 0)
       throw Error(PDFP_ERR_ARGUMENT, "bad schedule");
     sync();
+    drop_graphs();
     A.schedule(npb_a, stream, kRows, lane_a);
     At.schedule(npb_at, stream, kRows, lane_at);
     A.persist = At.persist = persist;
+    const size_t need = (size_t)std::max<int32_t>(A.nblocks, 1) * kBatch * kEvalRowTerms;
+    if (part_a.n < need) part_a.alloc(need);
   }
   void step(int k_offset, bool major) {
     if (major) step_t<true>(k_offset);
     else step_t<false>(k_offset);
   }
 
-  // The evaluation of every trajectory in `mask` into its slot, in one pass
-  // over each operator (the 8 trajectories share the gathers).
-  void evaluate_all(int mask, int slot) {
-    if (!mask) return;
+  // The evaluation of every trajectory in the device mask *mask into its
+  // slot, in one pass over each operator (the 8 trajectories share the
+  // gathers).
+  void evaluate_all(const int32_t* mask, int slot) {
     const int nbt = (int)((n + kEvalRows - 1) / kEvalRows);
     const int nba = (int)((m + kEvalRows - 1) / kEvalRows);
     EpiBatchEvalCol ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, 0};
@@ -339,8 +370,8 @@ struct Batch {
           part_t.p, nbt, kEvalColTerms, mask, slot * 16, sums.p);
     }
     if (nba && A.nblocks > 0) {
-      const size_t need = (size_t)A.nblocks * kBatch * kEvalRowTerms;
-      if (part_a.n < need) part_a.alloc(need);
+      if (part_a.n < (size_t)A.nblocks * kBatch * kEvalRowTerms)
+        throw Error(PDFP_ERR_STATE, "row-term partials smaller than the schedule");
       EpiBatchEvalRowLanes el{er, mask};
       if (A.max_rows > kBlock / kBatch)
         throw Error(PDFP_ERR_STATE, "batch schedule holds too many rows per block");
@@ -390,6 +421,7 @@ struct Batch {
     }
     const int rows = std::max(freq, 199) + 1;
     if (rows > wt_rows) {
+      drop_graphs();  // the passes take d_wt
       d_wt.alloc((size_t)rows * kBatch);
       h_wt.alloc((size_t)rows * kBatch);
       wt_rows = rows;
@@ -406,6 +438,43 @@ struct Batch {
                             stream));
     PDFP_HIP(hipMemcpyAsync(d_wt.p, h_wt.p, sizeof(double2) * (size_t)rows * kBatch,
                             hipMemcpyHostToDevice, stream));
+  }
+
+  // One block: iteration 1 (major after a restart; the restarted
+  // trajectories evaluated into slot 1), iterations 2 .. freq (freq major),
+  // the evaluation of the running trajectories into slot 0, the readback.
+  void general_block(int freq, bool any_restart) {
+    step(1, any_restart || freq == 1);
+    if (any_restart) evaluate_all(d_mask.p + 1, 1);
+    for (int i = 2; i <= freq; ++i) step(i, i == freq);
+    evaluate_all(d_mask.p, 0);
+    PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
+                            hipMemcpyDeviceToHost, stream));
+  }
+
+  // general_block as a captured graph, captured on first use of its shape
+  // and replayed after; it holds no memset node.
+  void block_graph(int freq, bool any_restart) {
+    BlockGraph& g = graphs[any_restart ? 1 : 0];
+    if (!g.exec || g.freq != freq) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
+      hipGraph_t graph = nullptr;
+      PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        general_block(freq, any_restart);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      PDFP_HIP(hipStreamEndCapture(stream, &graph));
+      const hipError_t st = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PDFP_HIP(st);
+      g.freq = freq;
+    }
+    PDFP_HIP(hipGraphLaunch(g.exec, stream));
   }
 
   // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for
@@ -474,18 +543,18 @@ struct Batch {
       upload_steps(omega, active, freq);
       bool any_restart = false;
       for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && tr[(size_t)k].do_restart;
-      step(1, any_restart || freq == 1);
       int restart_mask = 0, active_mask = 0;
       for (int k = 0; k < K; ++k) {
         if (!active[(size_t)k]) continue;
         active_mask |= 1 << k;
         if (tr[(size_t)k].do_restart) restart_mask |= 1 << k;
       }
-      evaluate_all(restart_mask, 1);
-      for (int i = 2; i <= freq; ++i) step(i, i == freq);
-      evaluate_all(active_mask, 0);
-      PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
-                              hipMemcpyDeviceToHost, stream));
+      h_mask.p[0] = active_mask;  // upload_steps synchronised: no copy reads it
+      h_mask.p[1] = restart_mask;
+      PDFP_HIP(hipMemcpyAsync(d_mask.p, h_mask.p, sizeof(int32_t) * 2, hipMemcpyHostToDevice,
+                              stream));
+      if (graph_on) block_graph(freq, any_restart);
+      else general_block(freq, any_restart);
       sync();
       const double now = wall_seconds();
       bool ended = false;
@@ -619,6 +688,7 @@ struct Batch {
     o->persist_at = At.persist;
     o->lane_rows_a = A.lane_len;
     o->lane_rows_at = At.lane_len;
+    o->block_graph = graph_on ? 1 : 0;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched
@@ -852,6 +922,13 @@ int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int3
                             int32_t lane_rows_a, int32_t lane_rows_at) {
   return batch_guarded(ctx, [&] {
     ctx->impl->set_schedule(npb_a, npb_at, persist, lane_rows_a, lane_rows_at);
+  });
+}
+
+int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on) {
+  return batch_guarded(ctx, [&] {
+    if (on != 0 && on != 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "block graph switch must be 0 or 1");
+    ctx->impl->graph_on = on != 0;
   });
 }
 

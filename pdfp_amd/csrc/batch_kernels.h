@@ -246,14 +246,16 @@ struct EpiBatchEvalRow {
   }
 };
 
-// The row terms of every trajectory in `mask` as a lane epilogue of the CSR
-// pass over A (gathering the 8 trajectories' pdhg x): lane (row, q) adds
-// trajectory q's terms; the pass writes per-trajectory partials.
+// The row terms of every trajectory in the mask (*mask, a device word the
+// host writes before the launch, so a captured launch serves any mask) as a
+// lane epilogue of the CSR pass over A (gathering the 8 trajectories' pdhg
+// x): lane (row, q) adds trajectory q's terms; the pass writes
+// per-trajectory partials.
 struct EpiBatchEvalRowLanes {
   static constexpr int kLanes = kBatch;
   using Pre = EpiBatchEvalRow::Pre;
   EpiBatchEvalRow er;
-  int mask;
+  const int32_t* mask;
   __device__ __forceinline__ Pre pre(int i, int q) const {
     EpiBatchEvalRow e = er;
     e.k = q;
@@ -261,7 +263,7 @@ struct EpiBatchEvalRowLanes {
   }
   __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
                                              double* acc) const {
-    if (!((mask >> q) & 1)) return;
+    if (!((*mask >> q) & 1)) return;
     EpiBatchEvalRow e = er;
     e.k = q;
     e(i, p, &s, acc);
@@ -273,7 +275,7 @@ struct EpiBatchEvalRowLanes {
 // lane (column, q) = (tid / 8, tid % 8) forms trajectory q's two sums over
 // the column in FP64, in ascending row order (the second over the difference
 // pz - rz, formed first as the reference does), and adds q's terms
-// (EpiBatchEvalCol) when bit q of `mask` is set.  Each workgroup writes
+// (EpiBatchEvalCol) when bit q of *mask is set.  Each workgroup writes
 // per-trajectory partials partial[(q·NT + t)·nblocks + block]; the 8
 // trajectories share every 32-byte gather.  (The row terms run as a lane
 // epilogue of the CSR pass over A, EpiBatchEvalRowLanes, whose schedule
@@ -284,7 +286,8 @@ constexpr int kEvalRows = 256 / kBatch;
 static __global__ __launch_bounds__(256) void k_batch_eval_col(
     int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const float* __restrict__ pz, const float* __restrict__ rz,
-    EpiBatchEvalCol ec, int mask, double* __restrict__ partial) {
+    EpiBatchEvalCol ec, const int32_t* __restrict__ maskp, double* __restrict__ partial) {
+  const int mask = *maskp;
   const int q = threadIdx.x % kBatch;
   const int j = blockIdx.x * kEvalRows + threadIdx.x / kBatch;
   double acc[kEvalColTerms];
@@ -306,14 +309,14 @@ static __global__ __launch_bounds__(256) void k_batch_eval_col(
 }
 
 // out[q·32 + offset + t] = Σ_b partial[(q·nt + t)·nb + b] in block order,
-// for every trajectory q in mask: one workgroup per (q, t), reading its
+// for every trajectory q in *maskp: one workgroup per (q, t), reading its
 // series contiguously.
 static __global__ __launch_bounds__(256) void k_batch_eval_finalize(
-    const double* __restrict__ partial, int nb, int nt, int mask, int offset,
-    double* __restrict__ out) {
+    const double* __restrict__ partial, int nb, int nt, const int32_t* __restrict__ maskp,
+    int offset, double* __restrict__ out) {
   __shared__ double red[256 / kWave];
   const int q = blockIdx.x / nt, t = blockIdx.x % nt;
-  if (!((mask >> q) & 1)) return;  // uniform over the workgroup
+  if (!((*maskp >> q) & 1)) return;  // uniform over the workgroup
   const double* __restrict__ series = partial + (size_t)blockIdx.x * nb;
   double v = 0.0;
   for (int b = threadIdx.x; b < nb; b += 256) v += series[b];

@@ -45,6 +45,8 @@ struct ProjectorBase {
   virtual void time_kernels(int iters, double* ms_at, double* ms_a) = 0;
   virtual int64_t device_bytes() const = 0;
   virtual void set_path(int path) = 0;
+  virtual void set_block_graph(int on) = 0;
+  virtual void set_schedule(int npb_a, int npb_at, int persist_a, int persist_at) = 0;
 };
 
 template <typename T>
@@ -83,6 +85,16 @@ struct Projector final : ProjectorBase {
   int small_lanes_a = 1, small_lanes_t = 1;
   int path = 0;
   bool small_used = false;
+  // FP32 form, general passes: each block of iterations, its evaluation and
+  // the readback of its sums captured once as a HIP graph and replayed
+  // (block_graph_on), one graph per (freq, restart evaluation) shape.
+  struct BlockGraph {
+    hipGraphExec_t exec = nullptr;
+    int freq = 0;
+  };
+  BlockGraph graphs[2];  // [restart evaluation]
+  bool block_graph_on = true;
+  bool graph_used = false;
   int n_restarts = 0, n_evals = 0;
   double solve_seconds = 0.0;
   bool x64_valid = false;
@@ -226,8 +238,44 @@ struct Projector final : ProjectorBase {
   ~Projector() override {
     if (stream) {
       (void)hipStreamSynchronize(stream);
+      drop_graphs();
       (void)hipStreamDestroy(stream);
     }
+  }
+
+  // The captured blocks hold kernel arguments (buffers, schedules, flags):
+  // dropped whenever one of them changes.
+  void drop_graphs() {
+    for (BlockGraph& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
+    }
+  }
+
+  // Replaces the autotuned schedule of both operators.
+  void set_schedule(int npb_a, int npb_at, int persist_a, int persist_at) override {
+    auto npb_ok = [](int v) {
+      for (int c : kNpbChoices)
+        if (v == c) return true;
+      return false;
+    };
+    if (!npb_ok(npb_a) || !npb_ok(npb_at) || persist_a < 0 || persist_a > 16 || persist_at < 0 ||
+        persist_at > 16)
+      throw Error(PDFP_ERR_ARGUMENT, "bad schedule");
+    sync();
+    drop_graphs();
+    A.schedule(npb_a, stream);
+    At.schedule(npb_at, stream);
+    A.persist = persist_a;
+    At.persist = persist_at;
+    part_t.alloc((size_t)std::max(At.nblocks, 1) * kEvalColTerms);
+    part_a.alloc((size_t)std::max(A.nblocks, 1) * kEvalRowTerms);
+    sync();
+  }
+
+  void set_block_graph(int on) override {
+    if (on != 0 && on != 1) throw Error(PDFP_ERR_ARGUMENT, "block graph switch must be 0 or 1");
+    block_graph_on = on != 0;
   }
 
   int64_t device_bytes() const override {
@@ -266,6 +314,7 @@ struct Projector final : ProjectorBase {
       acc += v * v;
     }
     b_norm_original = std::sqrt(acc);
+    drop_graphs();  // the evaluation's epilogue takes row_unscale
     row_unscale.alloc((size_t)std::max<int64_t>(m, 1));
     row_unscale.upload(inv.data(), (size_t)m, stream);
     sync();
@@ -382,6 +431,48 @@ struct Projector final : ProjectorBase {
     }
   }
 
+  void read_sums() {
+    PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 32, hipMemcpyDeviceToHost,
+                            stream));
+  }
+
+  // One block of the general passes: iteration 1 (major, and evaluated into
+  // slot 1 after a restart), iterations 2 .. freq (freq major), the
+  // evaluation into slot 0, and the readback of both slots.
+  void general_block(int freq, bool restart_eval) {
+    step(1, restart_eval || freq == 1);
+    if (restart_eval) evaluate(1);
+    for (int i = 2; i <= freq; ++i) step(i, i == freq);
+    evaluate(0);
+    read_sums();
+  }
+
+  // general_block as a captured graph.  Its launches read the block's
+  // scalars from d_sp, which the host writes before each replay, so one
+  // capture serves every block of its shape.  The block has no memset node.
+  void block_graph(int freq, bool restart_eval) {
+    BlockGraph& g = graphs[restart_eval ? 1 : 0];
+    if (!g.exec || g.freq != freq) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
+      hipGraph_t graph = nullptr;
+      PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        general_block(freq, restart_eval);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      PDFP_HIP(hipStreamEndCapture(stream, &graph));
+      const hipError_t st = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PDFP_HIP(st);
+      g.freq = freq;
+    }
+    PDFP_HIP(hipGraphLaunch(g.exec, stream));
+  }
+
   void set_path(int p) override {
     if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
     path = p;
@@ -483,19 +574,18 @@ struct Projector final : ProjectorBase {
     n_evals = 0;
     x64_valid = false;
     small_used = small_ok && path == 0;
+    graph_used = kFused && !small_used && block_graph_on;
     while (r.total_count < p.iteration_limit) {
       upload_step_params(omega);
       const bool restart_eval = do_restart;
       if (small_used) {
         small_block(freq, restart_eval);
+        read_sums();
+      } else if (graph_used) {
+        block_graph(freq, restart_eval);
       } else {
-        step(1, restart_eval || freq == 1);
-        if (restart_eval) evaluate(1);
-        for (int i = 2; i <= freq; ++i) step(i, i == freq);
-        evaluate(0);
+        general_block(freq, restart_eval);
       }
-      PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 32, hipMemcpyDeviceToHost,
-                              stream));
       sync();
       ++n_evals;
       if (restart_eval) {
@@ -600,6 +690,7 @@ struct Projector final : ProjectorBase {
     o->persist_a = A.persist;
     o->persist_at = At.persist;
     o->small_path = small_used ? 1 : 0;
+    o->block_graph = graph_used ? 1 : 0;
   }
 
   void spmv(int transpose, const double* xin, double* yout) override {
@@ -894,6 +985,15 @@ int64_t pdfp_device_bytes(const pdfp_ctx* ctx) {
 
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path) {
   return guarded(ctx, [&] { ctx->impl->set_path(path); });
+}
+
+int pdfp_set_block_graph(pdfp_ctx* ctx, int32_t on) {
+  return guarded(ctx, [&] { ctx->impl->set_block_graph(on); });
+}
+
+int pdfp_set_schedule(pdfp_ctx* ctx, int32_t npb_a, int32_t npb_at, int32_t persist_a,
+                      int32_t persist_at) {
+  return guarded(ctx, [&] { ctx->impl->set_schedule(npb_a, npb_at, persist_a, persist_at); });
 }
 
 }  // extern "C"

@@ -195,6 +195,18 @@ class PDFPState:
             raise ValueError("kernel path must be 'auto' or 'general'")
         self._check(self._lib.pdfp_set_kernel_path(self._ctx, codes[path]))
 
+    def set_block_graph(self, on: bool) -> None:
+        """FP32 form, general passes: replay each block of iterations (with
+        its evaluation) as one captured HIP graph (the default), or launch
+        every kernel on the stream.  Both give identical results."""
+        self._check(self._lib.pdfp_set_block_graph(self._ctx, int(bool(on))))
+
+    def set_schedule(self, npb_a: int, npb_at: int, persist_a: int = 0,
+                     persist_at: int = 0) -> None:
+        """Replace the autotuned schedule of the A and Aᵀ passes."""
+        self._check(self._lib.pdfp_set_schedule(self._ctx, npb_a, npb_at, persist_a,
+                                                persist_at))
+
 
 def original_primal_residual(s: PDFPState, D_r) -> PDFPState:
     """src/pdfp.jl:85-90."""

@@ -197,3 +197,53 @@ def test_batch_schedules_agree():
             assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (sched, k, r)
             l1b = bt.info(k).l1_distance
             assert abs(l1b - ref[k]) <= 1e-4 * max(1.0, ref[k]), (sched, k, l1b, ref[k])
+
+
+def test_batch_block_graph_replays_equal_stream_launches():
+    """Blocks replayed as captured graphs give the same trajectories, bit for
+    bit, as the same launches on the stream: trajectories that start, end,
+    restart and start again at different blocks (so the device masks and
+    the step tables change under the replayed graph), each run on a fresh
+    batch with the same schedule."""
+    print("batch block graph", flush=True)
+    m = G.mixed(2000, 4000, 1)
+    pp = projector_problem(m)
+    K = 8
+    tg = [G.binary_targets(m, 700 + k) / pp.D_c for k in range(12)]
+    loose = SolverParameters(eps_optimal_relative=1e-3, eps_feasible_relative=1e-4)
+    tight = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                             iteration_limit=200_000)
+    runs = []
+    for graph in (False, True, True):
+        bt = _batch(pp, K)
+        bt.set_schedule(256, 256, 0, 32, 32)
+        bt.set_block_graph(graph)
+        queue = list(range(12))
+        ended = []
+        running = [True] * K
+        for k in range(K):
+            bt.set_target(k, tg[queue.pop(0)])
+            bt.begin(k, tight if k % 2 else loose, 0.6 + 0.1 * k)
+        while any(running):
+            terms, _ = bt.run(until_any=True)
+            for k in range(K):
+                if not running[k] or terms[k] == "UNSPECIFIED":
+                    continue
+                assert terms[k] == "OPTIMAL"
+                info = bt.info(k)
+                assert info.block_graph == int(graph)
+                ended.append((k, info.total_count, info.n_restarts, info.l1_distance,
+                              bt.vector(k, "pdhg_x"), bt.vector(k, "pdhg_z")))
+                running[k] = False
+                if queue:
+                    bt.set_target(k, tg[queue.pop(0)])
+                    bt.begin(k, loose, 1.0)
+                    running[k] = True
+        bt.close()
+        runs.append(ended)
+    assert len(runs[0]) == 12
+    for run in runs[1:]:
+        assert len(run) == len(runs[0])
+        for a, b in zip(runs[0], run):
+            assert a[:4] == b[:4]
+            assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])

@@ -248,3 +248,45 @@ def test_edge_cases(precision):
     pp, s, st = project_with(P, P.PDFPState, m, np.zeros(201), tol=1e-14, iteration_limit=400,
                              precision=precision)
     assert st == "ITERATION_LIMIT" and s.total_count == 400
+
+
+@pytest.mark.parametrize("name,maker", [
+    ("mixed_2000x4000", lambda: G.mixed(2000, 4000, 1)),
+    ("mixed_50000x100000", lambda: G.mixed(50_000, 100_000, 1)),
+])
+def test_block_graph_replays_equal_stream_launches(name, maker):
+    """The FP32 general passes replayed as one captured graph per block give
+    the same iterates, bit for bit, as the same launches on the stream: a
+    warm-started sequence of projections (many replays, restarts, host reads
+    and uploads between them), each run on a fresh state with the same
+    schedule (the schedule sets each row's summation order)."""
+    say("block graph", name)
+    m = maker()
+    pp = projector_problem(m)
+    n = m.A.shape[1]
+    targets = [G.binary_targets(m, 90 + k) / pp.D_c for k in range(3)]
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-5,
+                         iteration_limit=100_000)
+    runs = []
+    for graph in (False, True, True):
+        s = hip_state(pp, "fp32")
+        s.set_kernel_path("general")
+        s.set_schedule(512, 512, 0, 0)
+        s.set_block_graph(graph)
+        s.set_x(np.zeros(n))
+        seq = []
+        for xt in targets:
+            P.set_target(s, xt)
+            assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
+            info = s.info()
+            assert info.block_graph == int(graph)
+            assert (info.npb_a, info.npb_at, info.persist_a, info.persist_at) == (512, 512, 0, 0)
+            seq.append((info.total_count, info.n_restarts, info.l1_distance,
+                        s.vector("pdhg_x"), s.vector("pdhg_z")))
+        s.close()
+        runs.append(seq)
+    assert sum(r[0] for r in runs[1]) > 3 * 200  # several replays per projection
+    for seq in runs[1:]:
+        for a, b in zip(runs[0], seq):
+            assert a[:3] == b[:3]
+            assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
