@@ -295,6 +295,11 @@ int pdfp_pack_run(pdfp_pack* ctx, int32_t until_any, int32_t* termination,
 int pdfp_pack_get_x(pdfp_pack* ctx, int32_t k, double* x_scaled_out);
 int pdfp_pack_get_vector(pdfp_pack* ctx, int32_t k, int32_t which, double* out);
 int pdfp_pack_get_info(pdfp_pack* ctx, int32_t k, pdfp_info* out);
+/* Block graphs of the pack (no counterpart in the reference), as
+ * pdfp_set_block_graph: 1 (the default) replays each block of iterations,
+ * with its evaluations and readback, as one captured HIP graph; 0 launches
+ * every kernel on the stream.  Both give identical results. */
+int pdfp_pack_set_block_graph(pdfp_pack* ctx, int32_t on);
 /* Mean milliseconds of one pack iteration (every instance), and of its
  * transpose pass and A pass. */
 int pdfp_pack_time_iterations(pdfp_pack* ctx, int32_t iterations, double* ms_out);

@@ -64,6 +64,16 @@ struct Pack {
   PinnedBuf<uint8_t> h_flag;
   PinnedBuf<double> h_sums;
   bool any_orig = false;
+  // Each block of iterations, its evaluations and the readback of its sums
+  // captured once as a HIP graph and replayed (graph_on), one graph per
+  // (freq, restart evaluation) shape; the launches read every instance's
+  // scalars from the step table that the host writes before each replay.
+  struct BlockGraph {
+    hipGraphExec_t exec = nullptr;
+    int freq = 0;
+  };
+  BlockGraph graphs[2];
+  bool graph_on = true;
   std::vector<PackInstance> in;
   std::string err = "no error";
 
@@ -203,7 +213,17 @@ struct Pack {
   ~Pack() {
     if (stream) {
       (void)hipStreamSynchronize(stream);
+      drop_graphs();
       (void)hipStreamDestroy(stream);
+    }
+  }
+
+  // The captured blocks hold kernel arguments (buffers, schedules, flags):
+  // dropped whenever one of them changes.
+  void drop_graphs() {
+    for (BlockGraph& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
     }
   }
 
@@ -247,6 +267,7 @@ struct Pack {
     check_k(k);
     PackInstance& I = in[(size_t)k];
     if (!any_orig) {
+      drop_graphs();  // the evaluation's epilogue takes row_unscale from now on
       std::vector<double> ones((size_t)std::max<int64_t>(M, 1), 1.0);
       row_unscale.alloc(ones.size());
       row_unscale.upload(ones.data(), ones.size(), stream);
@@ -413,6 +434,43 @@ struct Pack {
     }
   }
 
+  // One block: iteration 1 (major after a restart, and every instance
+  // evaluated into slot 1), iterations 2 .. freq (freq major), the
+  // evaluation into slot 0, the readback of the sums.
+  void general_block(int freq, bool any_restart) {
+    step(1, any_restart || freq == 1);
+    if (any_restart) evaluate(1);
+    for (int i = 2; i <= freq; ++i) step(i, i == freq);
+    evaluate(0);
+    PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)K * 32,
+                            hipMemcpyDeviceToHost, stream));
+  }
+
+  // general_block as a captured graph, captured on first use of its shape
+  // and replayed after; it holds no memset node.
+  void block_graph(int freq, bool any_restart) {
+    BlockGraph& g = graphs[any_restart ? 1 : 0];
+    if (!g.exec || g.freq != freq) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
+      hipGraph_t graph = nullptr;
+      PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        general_block(freq, any_restart);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      PDFP_HIP(hipStreamEndCapture(stream, &graph));
+      const hipError_t st = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PDFP_HIP(st);
+      g.freq = freq;
+    }
+    PDFP_HIP(hipGraphLaunch(g.exec, stream));
+  }
+
   // The block loop of pdfp_solve! (src/pdfp.jl:252-288) for every running
   // instance, in lockstep blocks of `freq` iterations, each instance with
   // its own termination; returns after the first block in which one ends
@@ -444,12 +502,8 @@ struct Pack {
       upload_steps(omega, active);
       bool any_restart = false;
       for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && in[(size_t)k].do_restart;
-      step(1, any_restart || freq == 1);
-      if (any_restart) evaluate(1);
-      for (int i = 2; i <= freq; ++i) step(i, i == freq);
-      evaluate(0);
-      PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)K * 32,
-                              hipMemcpyDeviceToHost, stream));
+      if (graph_on) block_graph(freq, any_restart);
+      else general_block(freq, any_restart);
       sync();
       const double now = wall_seconds();
       bool any_flag = false, ended = false;
@@ -593,6 +647,7 @@ struct Pack {
     o->npb_at = At.npb;
     o->persist_a = A.persist;
     o->persist_at = At.persist;
+    o->block_graph = graph_on ? 1 : 0;
   }
 
   // Mean milliseconds of one pack iteration (every instance active, step
@@ -809,6 +864,13 @@ int pdfp_pack_get_vector(pdfp_pack* ctx, int32_t k, int32_t which, double* out) 
   return pack_guarded(ctx, [&] {
     if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
     ctx->impl->get_vector(k, which, out);
+  });
+}
+
+int pdfp_pack_set_block_graph(pdfp_pack* ctx, int32_t on) {
+  return pack_guarded(ctx, [&] {
+    if (on != 0 && on != 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "block graph switch must be 0 or 1");
+    ctx->impl->graph_on = on != 0;
   });
 }
 

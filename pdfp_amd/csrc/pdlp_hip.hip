@@ -65,6 +65,15 @@ struct LpSolver final : LpBase {
   int small_lanes_a = 1, small_lanes_t = 1;
   int path = 0;
   bool small_used = false;
+  // General passes: each block of iterations, its evaluation(s) and the
+  // readback of its sums captured once as a HIP graph and replayed, one
+  // graph per (freq, restart evaluation) shape; the launches read the
+  // block's scalars from d_sp, which the host writes before each replay.
+  struct BlockGraph {
+    hipGraphExec_t exec = nullptr;
+    int freq = 0;
+  };
+  BlockGraph graphs[2];
 
   LpSolver(int64_t m_, int64_t n_, const int64_t* colptr, const int64_t* rowval,
            const double* nzval, int32_t base, const double* c_, const double* lb_,
@@ -120,7 +129,17 @@ struct LpSolver final : LpBase {
   ~LpSolver() override {
     if (stream) {
       (void)hipStreamSynchronize(stream);
+      drop_graphs();
       (void)hipStreamDestroy(stream);
+    }
+  }
+
+  // The captured blocks hold kernel arguments (buffers, flags): dropped
+  // whenever one of them changes.
+  void drop_graphs() {
+    for (BlockGraph& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
     }
   }
 
@@ -130,6 +149,7 @@ struct LpSolver final : LpBase {
   void set_cupdlpx(const double* D_r, const double* D_c) override {
     h_ru.resize((size_t)m);
     h_cu.resize((size_t)n);
+    drop_graphs();  // the evaluation's epilogues take the unscaling vectors
     for (int64_t i = 0; i < m; ++i) h_ru[(size_t)i] = 1.0 / D_r[i];
     for (int64_t j = 0; j < n; ++j) h_cu[(size_t)j] = 1.0 / D_c[j];
     row_unscale.alloc((size_t)std::max<int64_t>(m, 1));
@@ -331,6 +351,48 @@ struct LpSolver final : LpBase {
            max_primal_ray_inf / (-primal_ray_lin_obj) <= eps_infeasible;
   }
 
+  void read_sums() {
+    PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 64, hipMemcpyDeviceToHost,
+                            stream));
+  }
+
+  // One block of the general passes: iterations 1 and freq major, the
+  // restart evaluation after iteration 1, the block's evaluation, the
+  // readback of the sums.
+  void general_block(int freq, bool restart_eval) {
+    step(1, true);
+    if (restart_eval) evaluate(1);
+    for (int i = 2; i <= freq - 1; ++i) step(i, false);
+    step(freq, true);
+    evaluate(0);
+    read_sums();
+  }
+
+  // general_block as a captured graph, captured on first use of its shape
+  // and replayed after; it holds no memset node.
+  void block_graph(int freq, bool restart_eval) {
+    BlockGraph& g = graphs[restart_eval ? 1 : 0];
+    if (!g.exec || g.freq != freq) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = BlockGraph{};
+      hipGraph_t graph = nullptr;
+      PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        general_block(freq, restart_eval);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      PDFP_HIP(hipStreamEndCapture(stream, &graph));
+      const hipError_t st = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PDFP_HIP(st);
+      g.freq = freq;
+    }
+    PDFP_HIP(hipGraphLaunch(g.exec, stream));
+  }
+
   // pdlp_solve!  src/pdlp.jl:325-372
   int solve(const pdfp_params& p) override {
     const double t_start = wall_seconds();
@@ -348,15 +410,10 @@ struct LpSolver final : LpBase {
       const bool restart_eval = do_restart;
       if (small_used) {
         small_block(freq, restart_eval);
+        read_sums();
       } else {
-        step(1, true);
-        if (restart_eval) evaluate(1);
-        for (int i = 2; i <= freq - 1; ++i) step(i, false);
-        step(freq, true);
-        evaluate(0);
+        block_graph(freq, restart_eval);
       }
-      PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * 64, hipMemcpyDeviceToHost,
-                              stream));
       sync();
       if (restart_eval) {
         r.fixed_point_error = fpe_of(1);

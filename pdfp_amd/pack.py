@@ -132,6 +132,12 @@ class PDFPPack:
         self.last_total_count = [int(t) for t in total]
         return list(self.termination_reason), int(left.value)
 
+    def set_block_graph(self, on: bool) -> None:
+        """Replay each block of iterations (with its evaluations) as one
+        captured HIP graph (the default), or launch every kernel on the
+        stream.  Both give identical results."""
+        self._check(self._lib.pdfp_pack_set_block_graph(self._ctx, int(bool(on))))
+
     def pdhg_x(self, k: int) -> np.ndarray:
         out = np.empty(self.n[k])
         self._check(self._lib.pdfp_pack_get_x(self._ctx, k, dptr(out)))

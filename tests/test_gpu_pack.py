@@ -103,3 +103,51 @@ def test_pack_original_primal_residual():
         l1s = _single(pp, targets[k], p, 1.0, D_r=pp.D_r if k == 0 else None)
         l1p = pk.info(k).l1_distance
         assert abs(l1p - l1s) <= 1e-4 * max(1.0, l1s), (k, l1p, l1s)
+
+
+def test_pack_block_graph_replays_equal_stream_launches():
+    """Blocks replayed as captured graphs give the same projections, bit for
+    bit, as the same launches on the stream, with instances that end and
+    start again at different blocks.  Below the autotuner's threshold (15k
+    nonzeros in all) every pack gets the same schedule."""
+    print("pack block graph", flush=True)
+    milps = [G.set_cover(133, 201, 1), G.mixed(300, 600, 3), G.mixed(60, 90, 5),
+             G.set_cover(400, 700, 4)]
+    pps = [projector_problem(m) for m in milps]
+    loose = SolverParameters(eps_optimal_relative=1e-3, eps_feasible_relative=1e-4)
+    tight = SolverParameters(eps_optimal_relative=1e-6, eps_feasible_relative=1e-6,
+                             iteration_limit=200_000)
+    runs = []
+    for graph in (False, True, True):
+        pk = PDFPPack(pps)
+        pk.set_block_graph(graph)
+        rec = []
+        restarted = [False] * len(pps)
+        for k, (m, pp) in enumerate(zip(milps, pps)):
+            pk.set_target(k, G.binary_targets(m, 20 + k) / pp.D_c)
+            pk.begin(k, tight if k % 2 else loose, 0.8 + 0.2 * k)
+        running = [True] * len(pps)
+        while any(running):
+            terms, _ = pk.run(until_any=True)
+            for k in range(len(pps)):
+                if not running[k] or terms[k] == "UNSPECIFIED":
+                    continue
+                assert terms[k] == "OPTIMAL"
+                info = pk.info(k)
+                assert info.block_graph == int(graph)
+                rec.append((k, info.total_count, info.n_restarts, info.l1_distance,
+                            pk.pdhg_x(k), pk.vector(k, "pdhg_z")))
+                running[k] = False
+                if not restarted[k]:       # each instance once more, to a new target
+                    restarted[k] = True
+                    pk.set_target(k, G.binary_targets(milps[k], 40 + k) / pps[k].D_c)
+                    pk.begin(k, loose, 1.0)
+                    running[k] = True
+        pk.close()
+        runs.append(rec)
+    assert len(runs[0]) == 2 * len(pps)
+    for run in runs[1:]:
+        assert len(run) == len(runs[0])
+        for a, b in zip(runs[0], run):
+            assert a[:4] == b[:4]
+            assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
