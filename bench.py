@@ -444,6 +444,9 @@ def main():
     ap.add_argument("--no-ttff", action="store_true")
     ap.add_argument("--batch", type=int, default=8,
                     help="trajectories per step of the headline measurement (1: single)")
+    ap.add_argument("--batch-schedule", default=None,
+                    help="npb_a,npb_at,persist,lane_rows_a,lane_rows_at: the batched passes' "
+                         "schedule instead of the autotuner's (to profile a given form)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
@@ -512,6 +515,8 @@ def main():
         from pdfp_amd.batch import PDFPBatch
 
         bt = PDFPBatch(K, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+        if args.batch_schedule:
+            bt.set_schedule(*[int(v) for v in args.batch_schedule.split(",")])
         for k in range(K):
             bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]

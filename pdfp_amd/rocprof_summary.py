@@ -1,11 +1,16 @@
 """Short text summaries of rocprofv3 output for the committed profiles.
 
     python -m pdfp_amd.rocprof_summary stats KERNEL_STATS_CSV [--top N]
+    python -m pdfp_amd.rocprof_summary window KERNEL_TRACE_CSV --last N \
+        EPILOGUE [EPILOGUE ...]
     python -m pdfp_amd.rocprof_summary traffic --fetch CSV --write CSV \\
         --copy-fetch CSV --copy-write CSV
 
 ``stats`` turns a ``--kernel-trace --stats`` kernel_stats.csv into a
-markdown table.  ``traffic`` reads four ``--pmc`` counter_collection.csv
+markdown table.  ``window`` reads a ``--kernel-trace`` kernel_trace.csv and
+gives, per CSR pass epilogue, the durations of its last N dispatches (in
+start order): bench.py's own kernel timing launches each pass last, so
+these are the launches its roofline figure averages.  ``traffic`` reads four ``--pmc`` counter_collection.csv
 files (FETCH_SIZE and WRITE_SIZE of the benchmark, and of the 1 GiB plain
 copy of ``bench.py --copy-calibration``), scales each counter by what it
 reads for the copy of known size, and prints a table plus a JSON block of
@@ -64,6 +69,26 @@ def stats(path: str, top: int) -> str:
         out.append(f"| `{short_name(r['Name'])}` | {r['Calls']} | "
                    f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
                    f"{float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def window(path: str, keys: list, last: int) -> str:
+    runs = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        key = pass_key(r["Kernel_Name"])
+        if key in keys:
+            runs[key].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
+                              short_name(r["Kernel_Name"])))
+    out = [f"| epilogue | kernel | last dispatches | mean us | min us | max us |",
+           "|---|---|---:|---:|---:|---:|"]
+    for key in keys:
+        d = sorted(runs.get(key, []))[-last:]
+        if not d:
+            continue
+        us = [(e - s) / 1e3 for s, e, _ in d]
+        names = sorted({n for _, _, n in d})
+        out.append(f"| `{key}` | `{'`, `'.join(names)}` | {len(d)} | "
+                   f"{sum(us) / len(us):.2f} | {min(us):.2f} | {max(us):.2f} |")
     return "\n".join(out)
 
 
@@ -127,12 +152,18 @@ def main():
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--top", type=int, default=15)
+    w = sub.add_parser("window")
+    w.add_argument("csv")
+    w.add_argument("keys", nargs="+")
+    w.add_argument("--last", type=int, required=True)
     t = sub.add_parser("traffic")
     for a in ("--fetch", "--write", "--copy-fetch", "--copy-write"):
         t.add_argument(a, required=True)
     a = ap.parse_args()
     if a.cmd == "stats":
         print(stats(a.csv, a.top))
+    elif a.cmd == "window":
+        print(window(a.csv, a.keys, a.last))
     else:
         print(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write))
 
