@@ -300,6 +300,13 @@ int pdfp_pack_get_info(pdfp_pack* ctx, int32_t k, pdfp_info* out);
  * with its evaluations and readback, as one captured HIP graph; 0 launches
  * every kernel on the stream.  Both give identical results. */
 int pdfp_pack_set_block_graph(pdfp_pack* ctx, int32_t on);
+/* Kernel path of the pack (no counterpart in the reference), as
+ * pdfp_set_kernel_path: 0 (the default) runs each block as one launch with
+ * one workgroup per running instance when every instance fits in LDS (each
+ * iterates exactly as the single FP32 projector's one-workgroup path, its
+ * σ_max included), else the general passes; 1 always runs the general
+ * passes.  pdfp_info's small_path says which ran. */
+int pdfp_pack_set_kernel_path(pdfp_pack* ctx, int32_t path);
 /* Mean milliseconds of one pack iteration (every instance), and of its
  * transpose pass and A pass. */
 int pdfp_pack_time_iterations(pdfp_pack* ctx, int32_t iterations, double* ms_out);

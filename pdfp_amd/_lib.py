@@ -162,6 +162,7 @@ SIGNATURES = {
     "pdfp_pack_get_x": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_pack_get_vector": (C.c_int, [_P, C.c_int32, C.c_int32, _D]),
     "pdfp_pack_set_block_graph": (C.c_int, [_P, C.c_int32]),
+    "pdfp_pack_set_kernel_path": (C.c_int, [_P, C.c_int32]),
     "pdfp_pack_get_info": (C.c_int, [_P, C.c_int32, C.POINTER(PdfpInfo)]),
     "pdfp_pack_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_pack_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),

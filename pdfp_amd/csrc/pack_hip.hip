@@ -23,6 +23,7 @@
 #include "host_util.h"
 #include "pack_kernels.h"
 #include "pdfp_kernels.h"
+#include "small_kernels.h"
 #include "solver_common.h"
 
 namespace pdfp {
@@ -44,6 +45,9 @@ struct PackInstance {
   bool x64_valid = false;
   std::vector<double> h_xt, h_w, h_b;
   HostCsr ha, ht;  // this instance's operators, kept until its σ_max is known
+  // its local CSRs in the one-workgroup path's arrays (element offsets)
+  int64_t sa_ptr = 0, sa_nz = 0, st_ptr = 0, st_nz = 0;
+  int lanes_a = 1, lanes_t = 1;
 };
 
 struct Pack {
@@ -74,6 +78,20 @@ struct Pack {
   };
   BlockGraph graphs[2];
   bool graph_on = true;
+  // Every instance fits in LDS (small_kernels.h): each block runs as one
+  // launch with one workgroup per running instance, each iterating its own
+  // instance as the single FP32 projector's one-workgroup path does; path 1
+  // forces the general passes.
+  bool small_ok = false;
+  int path = 0;
+  bool small_used = false;
+  int64_t small_lds = 0;
+  DevBuf<int32_t> s_ptr_a, s_col_a, s_ptr_t, s_col_t;
+  DevBuf<float> s_val_a, s_val_t;
+  DevBuf<StepParams> d_ssp;
+  PinnedBuf<StepParams> h_ssp;
+  DevBuf<SmallArgs> d_sargs;
+  PinnedBuf<SmallArgs> h_sargs;
   std::vector<PackInstance> in;
   std::string err = "no error";
 
@@ -129,6 +147,11 @@ struct Pack {
     }
     A.segments = rseg;
     At.segments = cseg;
+    small_ok = K > 0;
+    for (const PackInstance& I : in)
+      small_ok = small_ok && I.m > 0 && I.n > 0 && I.nnz > 0 &&
+                 small_lds_bytes(I.m, I.n, I.nnz) <= kSmallLdsMax;
+    if (small_ok) upload_small();
     A.upload(ha, stream);
     At.upload(ht, stream);
     autotune_schedule(A, N, stream);
@@ -208,6 +231,105 @@ struct Pack {
     h_flag.alloc((size_t)K);
     upload_segments();
     PDFP_HIP(hipStreamSynchronize(stream));
+  }
+
+  // The instances' own CSRs (local indices, FP32 values) for the
+  // one-workgroup path, concatenated.
+  void upload_small() {
+    std::vector<int32_t> pa, ca, pt, ct;
+    std::vector<float> va, vt;
+    for (PackInstance& I : in) {
+      I.sa_ptr = (int64_t)pa.size();
+      I.sa_nz = (int64_t)ca.size();
+      I.st_ptr = (int64_t)pt.size();
+      I.st_nz = (int64_t)ct.size();
+      pa.insert(pa.end(), I.ha.rowptr.begin(), I.ha.rowptr.end());
+      ca.insert(ca.end(), I.ha.col.begin(), I.ha.col.end());
+      for (double v : I.ha.val) va.push_back((float)v);
+      pt.insert(pt.end(), I.ht.rowptr.begin(), I.ht.rowptr.end());
+      ct.insert(ct.end(), I.ht.col.begin(), I.ht.col.end());
+      for (double v : I.ht.val) vt.push_back((float)v);
+      I.lanes_a = small_lanes(I.ha.rowptr, kSmallBlock);
+      I.lanes_t = small_lanes(I.ht.rowptr, kSmallBlock);
+      small_lds = std::max(small_lds, small_lds_bytes(I.m, I.n, I.nnz));
+    }
+    auto up = [&](auto& d, const auto& h) {
+      d.alloc(h.size());
+      d.upload(h.data(), h.size(), stream);
+    };
+    up(s_ptr_a, pa);
+    up(s_col_a, ca);
+    up(s_val_a, va);
+    up(s_ptr_t, pt);
+    up(s_col_t, ct);
+    up(s_val_t, vt);
+    d_ssp.alloc((size_t)K);
+    h_ssp.alloc((size_t)K);
+    d_sargs.alloc((size_t)K);
+    h_sargs.alloc((size_t)K);
+    sync();
+  }
+
+  // One block of `freq` iterations for every running instance: one launch,
+  // one workgroup per running instance.
+  void small_block(int freq, const std::vector<bool>& active) {
+    int na = 0;
+    for (int k = 0; k < K; ++k) {
+      if (!active[(size_t)k]) continue;
+      const PackInstance& I = in[(size_t)k];
+      h_ssp.p[k] = StepParams{I.step_size, I.r.primal_weight, I.p.reflection_coefficient,
+                              (double)I.r.inner_count};
+      const int64_t r0 = I.row0, c0 = I.col0;
+      SmallArgs& a = h_sargs.p[na++];
+      a.A = SmallCsr{s_ptr_a.p + I.sa_ptr, s_col_a.p + I.sa_nz, s_val_a.p + I.sa_nz};
+      a.At = SmallCsr{s_ptr_t.p + I.st_ptr, s_col_t.p + I.st_nz, s_val_t.p + I.st_nz};
+      a.m = (int)I.m;
+      a.n = (int)I.n;
+      a.nnz = (int)I.nnz;
+      a.lanes_a = I.lanes_a;
+      a.lanes_t = I.lanes_t;
+      a.freq = freq;
+      a.first_major = I.do_restart ? 1 : 0;
+      a.sp = d_ssp.p + k;
+      a.x = x.p + c0;
+      a.y = y.p + c0;
+      a.z = z.p + r0;
+      a.x0 = x0.p + c0;
+      a.y0 = y0.p + c0;
+      a.z0 = z0.p + r0;
+      a.lb = lb.p + c0;
+      a.ub = ub.p + c0;
+      a.xt = xt.p + c0;
+      a.w = w.p + c0;
+      a.b = b.p + r0;
+      a.is_eq = is_eq.p + r0;
+      a.rx = rx.p + c0;
+      a.px = px.p + c0;
+      a.ds = ds.p + c0;
+      a.py = py.p + c0;
+      a.ry = ry.p + c0;
+      a.pz = pz.p + r0;
+      a.rz = rz.p + r0;
+      a.ec = EpiEvalCol<float>{px.p + c0, rx.p + c0, py.p + c0, ry.p + c0, ds.p + c0,
+                               xt.p + c0, w.p + c0, x0.p + c0, y0.p + c0};
+      a.er = EpiEvalRow<float>{pz.p + r0, rz.p + r0, b.p + r0, z0.p + r0, is_eq.p + r0,
+                               I.orig_primal ? row_unscale.p + r0 : nullptr};
+      a.sums = sums.p + (size_t)k * 32;
+    }
+    if (na == 0) return;
+    PDFP_HIP(hipMemcpyAsync(d_ssp.p, h_ssp.p, sizeof(StepParams) * (size_t)K,
+                            hipMemcpyHostToDevice, stream));
+    PDFP_HIP(hipMemcpyAsync(d_sargs.p, h_sargs.p, sizeof(SmallArgs) * (size_t)na,
+                            hipMemcpyHostToDevice, stream));
+    k_small_pack<kSmallBlock><<<na, kSmallBlock, (size_t)small_lds, stream>>>(d_sargs.p);
+    PDFP_HIP(hipGetLastError());
+    PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)K * 32,
+                            hipMemcpyDeviceToHost, stream));
+  }
+
+  void set_path(int p) {
+    if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
+    path = p;
   }
 
   ~Pack() {
@@ -394,7 +516,13 @@ struct Pack {
       DevCsr<float> a, t;
       a.upload(I.ha, stream);
       t.upload(I.ht, stream);
-      I.sv = power_iteration(a, t, I.m, I.n, p.sv_max_iterations, p.sv_tolerance, stream);
+      // as the single FP32 projector: in one workgroup when the pack's
+      // blocks run that way
+      I.sv = small_ok && path == 0
+                 ? small_sigma(a, t, I.m, I.n, p.sv_max_iterations, p.sv_tolerance, stream)
+                 : NAN;
+      if (std::isnan(I.sv))
+        I.sv = power_iteration(a, t, I.m, I.n, p.sv_max_iterations, p.sv_tolerance, stream);
       sync();
     }
     I.ha = HostCsr();
@@ -502,7 +630,9 @@ struct Pack {
       upload_steps(omega, active);
       bool any_restart = false;
       for (int k = 0; k < K; ++k) any_restart |= active[(size_t)k] && in[(size_t)k].do_restart;
-      if (graph_on) block_graph(freq, any_restart);
+      small_used = small_ok && path == 0;
+      if (small_used) small_block(freq, active);
+      else if (graph_on) block_graph(freq, any_restart);
       else general_block(freq, any_restart);
       sync();
       const double now = wall_seconds();
@@ -647,7 +777,8 @@ struct Pack {
     o->npb_at = At.npb;
     o->persist_a = A.persist;
     o->persist_at = At.persist;
-    o->block_graph = graph_on ? 1 : 0;
+    o->block_graph = graph_on && !small_used ? 1 : 0;
+    o->small_path = small_used ? 1 : 0;
   }
 
   // Mean milliseconds of one pack iteration (every instance active, step
@@ -865,6 +996,10 @@ int pdfp_pack_get_vector(pdfp_pack* ctx, int32_t k, int32_t which, double* out) 
     if (!out) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null output");
     ctx->impl->get_vector(k, which, out);
   });
+}
+
+int pdfp_pack_set_kernel_path(pdfp_pack* ctx, int32_t path) {
+  return pack_guarded(ctx, [&] { ctx->impl->set_path(path); });
 }
 
 int pdfp_pack_set_block_graph(pdfp_pack* ctx, int32_t on) {

@@ -147,10 +147,11 @@ __device__ __forceinline__ void small_evaluate(const SmallArgs& a, double* __res
   }
 }
 
+// One block of a.freq iterations of one instance in the calling workgroup;
+// smem holds small_lds_bytes(m, n, nnz) bytes, red the reduction scratch.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
+__device__ __forceinline__ void small_block_body(const SmallArgs& a, unsigned char* smem,
+                                                 double* red) {
   const int tid = threadIdx.x;
   const int n = a.n, m = a.m, nnz = a.nnz;
   int32_t* t_ptr = reinterpret_cast<int32_t*>(smem);
@@ -265,6 +266,23 @@ __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
     a.x[j] = s_x[j];
     a.y[j] = s_y[j];
   }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
+  small_block_body<BLOCK>(a, smem, red);
+}
+
+// Independent small instances, one workgroup each (workgroup b runs the
+// instance of args[b]); the launch's dynamic LDS is the largest instance's.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_small_pack(const SmallArgs* __restrict__ args) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
+  const SmallArgs a = args[blockIdx.x];
+  small_block_body<BLOCK>(a, smem, red);
 }
 
 // ---- the power iteration in one workgroup ----------------------------------

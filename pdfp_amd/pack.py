@@ -132,6 +132,15 @@ class PDFPPack:
         self.last_total_count = [int(t) for t in total]
         return list(self.termination_reason), int(left.value)
 
+    def set_kernel_path(self, path: str) -> None:
+        """"auto" (the default): when every instance fits in LDS, each block
+        runs as one launch with one workgroup per running instance;
+        "general": always the general passes."""
+        codes = {"auto": 0, "general": 1}
+        if path not in codes:
+            raise ValueError("kernel path must be 'auto' or 'general'")
+        self._check(self._lib.pdfp_pack_set_kernel_path(self._ctx, codes[path]))
+
     def set_block_graph(self, on: bool) -> None:
         """Replay each block of iterations (with its evaluations) as one
         captured HIP graph (the default), or launch every kernel on the

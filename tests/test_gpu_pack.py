@@ -10,8 +10,11 @@ import pytest
 from oracle import oracle as O
 from pdfp_amd import generators as G
 from pdfp_amd import pdfp as P
+from pdfp_amd.milp import read_mps
 from pdfp_amd.pack import PDFPPack
 from pdfp_amd.pdfp import SolverParameters, projector_problem
+
+from .conftest import example_path
 
 pytestmark = pytest.mark.gpu
 
@@ -151,3 +154,65 @@ def test_pack_block_graph_replays_equal_stream_launches():
         for a, b in zip(runs[0], run):
             assert a[:4] == b[:4]
             assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
+
+
+def test_pack_small_instances_equal_single_projector_bitwise():
+    """When every instance fits in LDS, the pack runs one workgroup per
+    running instance, each iterating as the single FP32 projector's
+    one-workgroup path: every instance's projections (a cold one, then a
+    warm-started one begun at another block) equal the single projector's
+    bit for bit, and the general passes (path forced) reach the same ℓ1
+    distances to the tolerance."""
+    print("pack small", flush=True)
+    milps = [read_mps(example_path("p0201")), read_mps(example_path("pk1")),
+             G.set_cover(133, 201, 2), G.mixed(60, 90, 5), read_mps(example_path("markshare2"))]
+    pps = [projector_problem(m) for m in milps]
+    tol = 1e-6
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=100_000)
+    pw = [1.0 + 0.1 * k for k in range(len(pps))]
+    t1 = [G.binary_targets(m, 60 + k) / pp.D_c for k, (m, pp) in enumerate(zip(milps, pps))]
+    t2 = [G.binary_targets(m, 80 + k) / pp.D_c for k, (m, pp) in enumerate(zip(milps, pps))]
+    # the single projector, one instance at a time
+    ref = []
+    for k, pp in enumerate(pps):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        seq = []
+        for t in (t1[k], t2[k]):
+            P.set_target(s, t)
+            assert P.pdfp_solve(s, p, pw[k]) == "OPTIMAL"
+            assert s.info().small_path == 1
+            seq.append((s.total_count, s.info().l1_distance, s.pdhg_x.copy()))
+        s.close()
+        ref.append(seq)
+    pk = PDFPPack(pps)
+    for k in range(len(pps)):
+        pk.set_target(k, t1[k])
+        pk.begin(k, p, pw[k])
+    running, second = [True] * len(pps), [False] * len(pps)
+    got = [[] for _ in pps]
+    while any(running):
+        terms, _ = pk.run(until_any=True)
+        for k in range(len(pps)):
+            if not running[k] or terms[k] == "UNSPECIFIED":
+                continue
+            assert terms[k] == "OPTIMAL" and pk.info(k).small_path == 1
+            got[k].append((pk.total_count(k), pk.info(k).l1_distance, pk.pdhg_x(k)))
+            running[k] = False
+            if not second[k]:
+                second[k] = running[k] = True
+                pk.set_target(k, t2[k])
+                pk.begin(k, p, pw[k])
+    for k in range(len(pps)):
+        for a, b in zip(ref[k], got[k]):
+            assert a[0] == b[0] and a[1] == b[1], (k, a[:2], b[:2])
+            assert np.array_equal(a[2], b[2]), k
+    pk.set_kernel_path("general")
+    for k in range(len(pps)):
+        pk.set_target(k, t1[k])
+    assert pk.solve(p, np.asarray(pw)) == ["OPTIMAL"] * len(pps)
+    for k in range(len(pps)):
+        assert pk.info(k).small_path == 0
+        l1g, l1s = pk.info(k).l1_distance, ref[k][0][1]
+        assert abs(l1g - l1s) <= 1e-4 * max(1.0, l1s), (k, l1g, l1s)
