@@ -244,6 +244,13 @@ int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int3
  * with its evaluations and readback, as one captured HIP graph; 0 launches
  * every kernel on the stream.  Both give identical results. */
 int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on);
+/* Kernel path of the batch (no counterpart in the reference), as
+ * pdfp_set_kernel_path: 0 (the default) runs each block as one launch with
+ * one workgroup per running trajectory when the instance fits in LDS (each
+ * iterates exactly as the single FP32 projector's one-workgroup path, its
+ * σ_max included), else the batched passes; 1 always runs the batched
+ * passes.  pdfp_info's small_path says which ran. */
+int pdfp_batch_set_kernel_path(pdfp_batch* ctx, int32_t path);
 void pdfp_batch_destroy(pdfp_batch* ctx);
 
 /* ---- pack of independent instances ------------------------------------ */

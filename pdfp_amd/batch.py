@@ -127,6 +127,15 @@ class PDFPBatch:
     def total_count(self, k: int) -> int:
         return int(self.info(k).total_count)
 
+    def set_kernel_path(self, path: str) -> None:
+        """"auto" (the default): when the instance fits in LDS, each block
+        runs as one launch with one workgroup per running trajectory;
+        "general": always the batched passes."""
+        codes = {"auto": 0, "general": 1}
+        if path not in codes:
+            raise ValueError("kernel path must be 'auto' or 'general'")
+        self._check(self._lib.pdfp_batch_set_kernel_path(self._ctx, codes[path]))
+
     def set_block_graph(self, on: bool) -> None:
         """Replay each block of iterations (with its evaluations) as one
         captured HIP graph (the default), or launch every kernel on the

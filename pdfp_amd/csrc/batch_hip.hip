@@ -21,9 +21,23 @@
 #include "engine.h"
 #include "host_util.h"
 #include "pdfp_kernels.h"
+#include "small_kernels.h"
 #include "solver_common.h"
 
 namespace pdfp {
+
+// The one-workgroup block (small_kernels.h) for the trajectories of a batch
+// whose instance fits in LDS: workgroup b runs the trajectory of args[b] on
+// its view (stride 8) of the interleaved vectors.
+using SmallBatchArgs = SmallArgsT<EpiBatchEvalCol, EpiBatchEvalRow>;
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_small_batch(const SmallBatchArgs* __restrict__ args) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
+  const SmallBatchArgs a = args[blockIdx.x];
+  small_block_body<BLOCK, kBatch>(a, smem, red);
+}
 
 struct Trajectory {
   RestartState r;
@@ -72,6 +86,17 @@ struct Batch {
   };
   BlockGraph graphs[2];
   bool graph_on = true;
+  // The instance fits in LDS: each block runs as one launch with one
+  // workgroup per running trajectory, each iterating as the single FP32
+  // projector's one-workgroup path; path 1 forces the batched passes.
+  bool small_ok = false;
+  int small_lanes_a = 1, small_lanes_t = 1;
+  int path = 0;
+  bool small_used = false;
+  DevBuf<StepParams> d_ssp;
+  PinnedBuf<StepParams> h_ssp;
+  DevBuf<SmallBatchArgs> d_sargs;
+  PinnedBuf<SmallBatchArgs> h_sargs;
   PinnedBuf<double> h_sums;
   std::vector<double> h_w, h_b;
   double b_norm = 0.0, b_norm_original = NAN, dual_den = 1.0;
@@ -150,6 +175,15 @@ struct Batch {
     h_sp.alloc(kBatch);
     d_mask.alloc(2);
     h_mask.alloc(2);
+    small_ok = m > 0 && n > 0 && nnz > 0 && small_lds_bytes(m, n, nnz) <= kSmallLdsMax;
+    if (small_ok) {
+      small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
+      small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);
+      d_ssp.alloc(kBatch);
+      h_ssp.alloc(kBatch);
+      d_sargs.alloc(kBatch);
+      h_sargs.alloc(kBatch);
+    }
     for (int q = 0; q < kBatch; ++q) h_sp.p[q] = BatchStep{1.0, 1.0, 1.0, 0.0, 0.0, {0, 0, 0}};
     for (auto& t : tr) t.h_xt.assign((size_t)n, 0.0);
     autotune();
@@ -440,6 +474,66 @@ struct Batch {
                             hipMemcpyHostToDevice, stream));
   }
 
+  // One block of `freq` iterations for every running trajectory: one
+  // launch, one workgroup per running trajectory.
+  void small_block(int freq, double omega, const std::vector<bool>& active) {
+    int na = 0;
+    for (int q = 0; q < K; ++q) {
+      if (!active[(size_t)q]) continue;
+      const Trajectory& t = tr[(size_t)q];
+      h_ssp.p[q] = StepParams{step_size, t.r.primal_weight, omega, (double)t.r.inner_count};
+      SmallBatchArgs& a = h_sargs.p[na++];
+      a.A = SmallCsr{A.rowptr.p, A.col.p, A.val.p};
+      a.At = SmallCsr{At.rowptr.p, At.col.p, At.val.p};
+      a.m = (int)m;
+      a.n = (int)n;
+      a.nnz = (int)nnz;
+      a.lanes_a = small_lanes_a;
+      a.lanes_t = small_lanes_t;
+      a.freq = freq;
+      a.first_major = t.do_restart ? 1 : 0;
+      a.sp = d_ssp.p + q;
+      a.x = x.p + q;
+      a.y = y.p + q;
+      a.z = z.p + q;
+      a.x0 = x0.p + q;
+      a.y0 = y0.p + q;
+      a.z0 = z0.p + q;
+      a.lb = lb.p;
+      a.ub = ub.p;
+      a.xt = xt.p + q;
+      a.w = w.p;
+      a.b = b.p;
+      a.is_eq = is_eq.p;
+      a.rx = rx.p + q;
+      a.px = px.p + q;
+      a.ds = ds.p + q;
+      a.py = py.p + q;
+      a.ry = ry.p + q;
+      a.pz = pz.p + q;
+      a.rz = rz.p + q;
+      a.ec = EpiBatchEvalCol{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, q};
+      a.er = EpiBatchEvalRow{pz.p, rz.p, b.p, z0.p, is_eq.p,
+                             orig_primal ? row_unscale.p : nullptr, q};
+      a.sums = sums.p + (size_t)q * 32;
+    }
+    if (na == 0) return;
+    PDFP_HIP(hipMemcpyAsync(d_ssp.p, h_ssp.p, sizeof(StepParams) * kBatch,
+                            hipMemcpyHostToDevice, stream));
+    PDFP_HIP(hipMemcpyAsync(d_sargs.p, h_sargs.p, sizeof(SmallBatchArgs) * (size_t)na,
+                            hipMemcpyHostToDevice, stream));
+    k_small_batch<kSmallBlock>
+        <<<na, kSmallBlock, (size_t)small_lds_bytes(m, n, nnz), stream>>>(d_sargs.p);
+    PDFP_HIP(hipGetLastError());
+    PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
+                            hipMemcpyDeviceToHost, stream));
+  }
+
+  void set_path(int p) {
+    if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
+    path = p;
+  }
+
   // One block: iteration 1 (major after a restart; the restarted
   // trajectories evaluated into slot 1), iterations 2 .. freq (freq major),
   // the evaluation of the running trajectories into slot 0, the readback.
@@ -484,8 +578,14 @@ struct Batch {
     if (p.termination_evaluation_frequency < 1)
       throw Error(PDFP_ERR_ARGUMENT, "termination_evaluation_frequency must be >= 1");
     if (!(pw > 0.0)) throw Error(PDFP_ERR_ARGUMENT, "primal_weight must be > 0");
-    if (nnz > 0 && std::isnan(sv))
-      sv = power_iteration(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
+    if (nnz > 0 && std::isnan(sv)) {
+      // as the single FP32 projector: in one workgroup when the batch's
+      // blocks run that way
+      if (small_ok && path == 0)
+        sv = small_sigma(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
+      if (std::isnan(sv))
+        sv = power_iteration(A, At, m, n, p.sv_max_iterations, p.sv_tolerance, stream);
+    }
     step_size = nnz == 0 ? 1.0 : 0.998 / std::sqrt(1.0 + sv * sv);
     Trajectory& t = tr[(size_t)k];
     t.p = p;
@@ -553,7 +653,9 @@ struct Batch {
       h_mask.p[1] = restart_mask;
       PDFP_HIP(hipMemcpyAsync(d_mask.p, h_mask.p, sizeof(int32_t) * 2, hipMemcpyHostToDevice,
                               stream));
-      if (graph_on) block_graph(freq, any_restart);
+      small_used = small_ok && path == 0;
+      if (small_used) small_block(freq, omega, active);
+      else if (graph_on) block_graph(freq, any_restart);
       else general_block(freq, any_restart);
       sync();
       const double now = wall_seconds();
@@ -688,7 +790,8 @@ struct Batch {
     o->persist_at = At.persist;
     o->lane_rows_a = A.lane_len;
     o->lane_rows_at = At.lane_len;
-    o->block_graph = graph_on ? 1 : 0;
+    o->block_graph = graph_on && !small_used ? 1 : 0;
+    o->small_path = small_used ? 1 : 0;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched
@@ -923,6 +1026,10 @@ int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int3
   return batch_guarded(ctx, [&] {
     ctx->impl->set_schedule(npb_a, npb_at, persist, lane_rows_a, lane_rows_at);
   });
+}
+
+int pdfp_batch_set_kernel_path(pdfp_batch* ctx, int32_t path) {
+  return batch_guarded(ctx, [&] { ctx->impl->set_path(path); });
 }
 
 int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on) {

@@ -74,7 +74,13 @@ struct SmallCsr {
   const float* __restrict__ val;
 };
 
-struct SmallArgs {
+// The iterate vectors (x, y, z, their initial, reflected and pdhg copies,
+// x̃, the dual slack) are read at index·S: S = 1 for one instance's own
+// vectors, S = 8 for one trajectory's view of the batch's interleaved ones
+// (the pointers then start at the trajectory's slot).  The problem data (ℓ,
+// u, w, b, is_eq) are always dense.
+template <class EC, class ER>
+struct SmallArgsT {
   SmallCsr A, At;
   int m, n, nnz;
   int lanes_a, lanes_t;  // lanes per row of A and of Aᵀ (powers of two ≤ 64)
@@ -86,10 +92,12 @@ struct SmallArgs {
   const float *x0, *y0, *z0, *lb, *ub, *xt, *w, *b;
   const uint8_t* is_eq;
   float *rx, *px, *ds, *py, *ry, *pz, *rz;
-  EpiEvalCol<float> ec;
-  EpiEvalRow<float> er;
+  EC ec;
+  ER er;
   double* sums;  // slot 0 at sums[0..16), slot 1 at sums[16..32)
 };
+using SmallArgs = SmallArgsT<EpiEvalCol<float>, EpiEvalRow<float>>;
+
 
 template <int BLOCK, int NACC>
 __device__ __forceinline__ void small_reduce(double (&acc)[NACC], double* __restrict__ out,
@@ -112,9 +120,9 @@ __device__ __forceinline__ void small_reduce(double (&acc)[NACC], double* __rest
 }
 
 // The evaluation of the last major step from the HBM copies it wrote.
-template <int BLOCK>
-__device__ __forceinline__ void small_evaluate(const SmallArgs& a, double* __restrict__ out,
-                                               double* red) {
+template <int BLOCK, int S, class EC, class ER>
+__device__ __forceinline__ void small_evaluate(const SmallArgsT<EC, ER>& a,
+                                               double* __restrict__ out, double* red) {
   const int tid = threadIdx.x;
   {
     double acc[kEvalColTerms];
@@ -125,9 +133,9 @@ __device__ __forceinline__ void small_evaluate(const SmallArgs& a, double* __res
       for (int k = a.At.ptr[j]; k < a.At.ptr[j + 1]; ++k) {
         const double av = (double)a.At.val[k];
         const int c = a.At.col[k];
-        const double zc = (double)a.pz[c];
+        const double zc = (double)a.pz[(size_t)c * S];
         s[0] += av * zc;
-        s[1] += av * (zc - (double)a.rz[c]);
+        s[1] += av * (zc - (double)a.rz[(size_t)c * S]);
       }
       a.ec(j, a.ec.pre(j), s, acc);
     }
@@ -140,7 +148,7 @@ __device__ __forceinline__ void small_evaluate(const SmallArgs& a, double* __res
     for (int i = tid; i < a.m; i += BLOCK) {
       double s = 0.0;
       for (int k = a.A.ptr[i]; k < a.A.ptr[i + 1]; ++k)
-        s += (double)a.A.val[k] * (double)a.px[a.A.col[k]];
+        s += (double)a.A.val[k] * (double)a.px[(size_t)a.A.col[k] * S];
       a.er(i, a.er.pre(i), &s, acc);
     }
     small_reduce<BLOCK, kEvalRowTerms>(acc, out + 8, red);
@@ -149,9 +157,9 @@ __device__ __forceinline__ void small_evaluate(const SmallArgs& a, double* __res
 
 // One block of a.freq iterations of one instance in the calling workgroup;
 // smem holds small_lds_bytes(m, n, nnz) bytes, red the reduction scratch.
-template <int BLOCK>
-__device__ __forceinline__ void small_block_body(const SmallArgs& a, unsigned char* smem,
-                                                 double* red) {
+template <int BLOCK, int S, class EC, class ER>
+__device__ __forceinline__ void small_block_body(const SmallArgsT<EC, ER>& a,
+                                                 unsigned char* smem, double* red) {
   const int tid = threadIdx.x;
   const int n = a.n, m = a.m, nnz = a.nnz;
   int32_t* t_ptr = reinterpret_cast<int32_t*>(smem);
@@ -183,19 +191,19 @@ __device__ __forceinline__ void small_block_body(const SmallArgs& a, unsigned ch
     a_val[k] = a.A.val[k];
   }
   for (int i = tid; i < m; i += BLOCK) {
-    s_z[i] = a.z[i];
-    s_z0[i] = a.z0[i];
+    s_z[i] = a.z[(size_t)i * S];
+    s_z0[i] = a.z0[(size_t)i * S];
     s_b[i] = a.b[i];
     s_eq[i] = a.is_eq[i];
   }
   for (int j = tid; j < n; j += BLOCK) {
-    s_x[j] = a.x[j];
-    s_y[j] = a.y[j];
-    s_x0[j] = a.x0[j];
-    s_y0[j] = a.y0[j];
+    s_x[j] = a.x[(size_t)j * S];
+    s_y[j] = a.y[(size_t)j * S];
+    s_x0[j] = a.x0[(size_t)j * S];
+    s_y0[j] = a.y0[(size_t)j * S];
     s_lb[j] = a.lb[j];
     s_ub[j] = a.ub[j];
-    s_xt[j] = a.xt[j];
+    s_xt[j] = a.xt[(size_t)j * S];
     s_w[j] = a.w[j];
   }
   __syncthreads();
@@ -225,11 +233,11 @@ __device__ __forceinline__ void small_block_body(const SmallArgs& a, unsigned ch
       s_y[j] = (float)o.y;
       s_rx[j] = (float)o.rx;
       if (major) {
-        a.rx[j] = (float)o.rx;
-        a.px[j] = (float)o.px;
-        a.ds[j] = (float)o.ds;
-        a.py[j] = (float)o.py;
-        a.ry[j] = (float)o.ry;
+        a.rx[(size_t)j * S] = (float)o.rx;
+        a.px[(size_t)j * S] = (float)o.px;
+        a.ds[(size_t)j * S] = (float)o.ds;
+        a.py[(size_t)j * S] = (float)o.py;
+        a.ry[(size_t)j * S] = (float)o.ry;
       }
     }
     __syncthreads();
@@ -249,22 +257,22 @@ __device__ __forceinline__ void small_block_body(const SmallArgs& a, unsigned ch
                                      s_eq[i] != 0);
       s_z[i] = (float)o.z;
       if (major) {
-        a.pz[i] = (float)o.pz;
-        a.rz[i] = (float)o.rz;
+        a.pz[(size_t)i * S] = (float)o.pz;
+        a.rz[(size_t)i * S] = (float)o.rz;
       }
     }
     __syncthreads();
-    if (it == 1 && a.first_major && a.freq > 1) small_evaluate<BLOCK>(a, a.sums + 16, red);
+    if (it == 1 && a.first_major && a.freq > 1) small_evaluate<BLOCK, S>(a, a.sums + 16, red);
   }
-  small_evaluate<BLOCK>(a, a.sums, red);
+  small_evaluate<BLOCK, S>(a, a.sums, red);
   if (a.freq == 1 && a.first_major) {
     // the first step was also the last: both slots hold its evaluation
     if (tid < 16) a.sums[16 + tid] = a.sums[tid];
   }
-  for (int i = tid; i < m; i += BLOCK) a.z[i] = s_z[i];
+  for (int i = tid; i < m; i += BLOCK) a.z[(size_t)i * S] = s_z[i];
   for (int j = tid; j < n; j += BLOCK) {
-    a.x[j] = s_x[j];
-    a.y[j] = s_y[j];
+    a.x[(size_t)j * S] = s_x[j];
+    a.y[(size_t)j * S] = s_y[j];
   }
 }
 
@@ -272,7 +280,7 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
-  small_block_body<BLOCK>(a, smem, red);
+  small_block_body<BLOCK, 1>(a, smem, red);
 }
 
 // Independent small instances, one workgroup each (workgroup b runs the
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_small_pack(const SmallArgs* __restric
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
   const SmallArgs a = args[blockIdx.x];
-  small_block_body<BLOCK>(a, smem, red);
+  small_block_body<BLOCK, 1>(a, smem, red);
 }
 
 // ---- the power iteration in one workgroup ----------------------------------

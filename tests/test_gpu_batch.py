@@ -11,7 +11,10 @@ from oracle import oracle as O
 from pdfp_amd import generators as G
 from pdfp_amd import pdfp as P
 from pdfp_amd.batch import PDFPBatch
+from pdfp_amd.milp import read_mps
 from pdfp_amd.pdfp import SolverParameters, projector_problem
+
+from .conftest import example_path
 
 pytestmark = pytest.mark.gpu
 
@@ -247,3 +250,64 @@ def test_batch_block_graph_replays_equal_stream_launches():
         for a, b in zip(runs[0], run):
             assert a[:4] == b[:4]
             assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
+
+
+def test_batch_small_instance_trajectories_equal_single_projector_bitwise():
+    """When the instance fits in LDS, the batch runs one workgroup per running
+    trajectory on its view of the interleaved vectors, each iterating as the
+    single FP32 projector's one-workgroup path: every trajectory's
+    projections (a cold one, then a warm-started one begun at another
+    block) equal the single projector's bit for bit, and the batched passes
+    (path forced) reach the same ℓ1 distances to the tolerance."""
+    print("batch small", flush=True)
+    m = read_mps(example_path("p0201"))
+    pp = projector_problem(m)
+    K = 8
+    tol = 1e-6
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=100_000)
+    pw = [0.7 + 0.1 * k for k in range(K)]
+    t1 = [G.binary_targets(m, 60 + k) / pp.D_c for k in range(K)]
+    t2 = [G.binary_targets(m, 80 + k) / pp.D_c for k in range(K)]
+    ref = []
+    for k in range(K):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        seq = []
+        for t in (t1[k], t2[k]):
+            P.set_target(s, t)
+            assert P.pdfp_solve(s, p, pw[k]) == "OPTIMAL"
+            assert s.info().small_path == 1
+            seq.append((s.total_count, s.info().l1_distance, s.pdhg_x.copy()))
+        s.close()
+        ref.append(seq)
+    bt = _batch(pp, K)
+    for k in range(K):
+        bt.set_target(k, t1[k])
+        bt.begin(k, p, pw[k])
+    running, second = [True] * K, [False] * K
+    got = [[] for _ in range(K)]
+    while any(running):
+        terms, _ = bt.run(until_any=True)
+        for k in range(K):
+            if not running[k] or terms[k] == "UNSPECIFIED":
+                continue
+            assert terms[k] == "OPTIMAL" and bt.info(k).small_path == 1
+            got[k].append((bt.total_count(k), bt.info(k).l1_distance, bt.pdhg_x(k)))
+            running[k] = False
+            if not second[k]:
+                second[k] = running[k] = True
+                bt.set_target(k, t2[k])
+                bt.begin(k, p, pw[k])
+    for k in range(K):
+        for a, b in zip(ref[k], got[k]):
+            assert a[0] == b[0] and a[1] == b[1], (k, a[:2], b[:2])
+            assert np.array_equal(a[2], b[2]), k
+    bt.set_kernel_path("general")
+    for k in range(K):
+        bt.set_target(k, t1[k])
+    assert bt.solve(p, np.asarray(pw)) == ["OPTIMAL"] * K
+    for k in range(K):
+        assert bt.info(k).small_path == 0
+        l1g, l1s = bt.info(k).l1_distance, ref[k][0][1]
+        assert abs(l1g - l1s) <= 1e-4 * max(1.0, l1s), (k, l1g, l1s)
