@@ -347,7 +347,8 @@ typedef struct pdlp_info {
  * (c, var_lb/ub, con_lb/ub of the ScaledProblem, src/preconditioner.jl:13-24). */
 /* Kernel path of the root LP (no counterpart in the reference): 0 (the
  * default) runs each block of iterations in one workgroup when the instance
- * fits in LDS (24 nnz + 52 n + 36 m + 8 bytes ≤ 62 KiB), else the general
+ * fits in LDS (24 nnz + 52 n + 36 m + 8 bytes ≤ 152 KiB on gfx950: the device's
+ * per-workgroup LDS less 8 KiB), else the general
  * passes; 1 always runs the general passes. */
 int pdlp_set_kernel_path(pdlp_ctx* ctx, int32_t path);
 
@@ -407,7 +408,8 @@ int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
 
 /* Kernel path of the FP32 form (no counterpart in the reference): 0 (the
  * default) runs each block of iterations in one workgroup when the instance
- * fits in LDS (4(m+n+2) + 16 nnz + 36 n + 13 m bytes ≤ 62 KiB), else the
+ * fits in LDS (4(m+n+2) + 16 nnz + 36 n + 13 m bytes ≤ 152 KiB on gfx950: the
+ * device's per-workgroup LDS less 8 KiB), else the
  * general passes; 1 always runs the general passes.  The FP64 form ignores
  * it. */
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);

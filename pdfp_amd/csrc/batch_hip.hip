@@ -175,7 +175,7 @@ struct Batch {
     h_sp.alloc(kBatch);
     d_mask.alloc(2);
     h_mask.alloc(2);
-    small_ok = m > 0 && n > 0 && nnz > 0 && small_lds_bytes(m, n, nnz) <= kSmallLdsMax;
+    small_ok = m > 0 && n > 0 && nnz > 0 && small_lds_bytes(m, n, nnz) <= small_lds_max();
     if (small_ok) {
       small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
       small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);

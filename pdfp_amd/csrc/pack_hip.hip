@@ -150,7 +150,7 @@ struct Pack {
     small_ok = K > 0;
     for (const PackInstance& I : in)
       small_ok = small_ok && I.m > 0 && I.n > 0 && I.nnz > 0 &&
-                 small_lds_bytes(I.m, I.n, I.nnz) <= kSmallLdsMax;
+                 small_lds_bytes(I.m, I.n, I.nnz) <= small_lds_max();
     if (small_ok) upload_small();
     A.upload(ha, stream);
     At.upload(ht, stream);

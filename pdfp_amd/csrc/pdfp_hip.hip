@@ -112,7 +112,7 @@ struct Projector final : ProjectorBase {
     A.upload(ha, stream);
     At.upload(ht, stream);
     small_ok = kFused && m > 0 && n > 0 && nnz > 0 &&
-               small_lds_bytes(m, n, nnz) <= kSmallLdsMax;
+               small_lds_bytes(m, n, nnz) <= small_lds_max();
     if (small_ok) {
       small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
       small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);

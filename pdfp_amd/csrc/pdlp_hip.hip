@@ -91,7 +91,7 @@ struct LpSolver final : LpBase {
     autotune_schedule(A, n, stream);
     autotune_schedule(At, m, stream);
     if constexpr (std::is_same<T, double>::value) {
-      small_ok = m > 0 && n > 0 && nnz > 0 && small_lp_lds_bytes(m, n, nnz) <= kSmallLdsMax;
+      small_ok = m > 0 && n > 0 && nnz > 0 && small_lp_lds_bytes(m, n, nnz) <= small_lds_max();
       if (small_ok) {
         small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
         small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);

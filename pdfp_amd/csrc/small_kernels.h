@@ -41,7 +41,19 @@
 namespace pdfp {
 
 constexpr int kSmallBlock = 1024;
-constexpr int64_t kSmallLdsMax = 62 * 1024;
+// Dynamic LDS one workgroup may take for an instance: the current device's
+// per-workgroup maximum (160 KiB on gfx950) less 8 KiB for the kernels'
+// static LDS, at most PDFP_SMALL_LDS_MAX.
+#ifndef PDFP_SMALL_LDS_MAX
+#define PDFP_SMALL_LDS_MAX (152 * 1024)
+#endif
+inline int64_t small_lds_max() {
+  int dev = 0, bytes = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return 0;  // unknown: no one-workgroup path
+  return std::max<int64_t>(std::min<int64_t>((int64_t)bytes - 8192, PDFP_SMALL_LDS_MAX), 0);
+}
 
 inline int64_t small_lds_bytes(int64_t m, int64_t n, int64_t nnz) {
   return 4 * (n + 1) + 4 * (m + 1) + 16 * nnz + 36 * n + 13 * m;
@@ -422,7 +434,7 @@ double small_power_iteration(SmallCsrT<TV> A, SmallCsrT<TV> At, const std::vecto
                              const double* start, int max_iterations, double tolerance,
                              hipStream_t stream) {
   const int64_t lds = small_power_lds_bytes<TV>(m, n, nnz);
-  if (m <= 0 || n <= 0 || nnz <= 0 || lds > kSmallLdsMax) return NAN;
+  if (m <= 0 || n <= 0 || nnz <= 0 || lds > small_lds_max()) return NAN;
   DevBuf<double> d_start, d_out;
   PinnedBuf<double> h_out;
   d_start.alloc((size_t)m);
@@ -443,7 +455,7 @@ double small_power_iteration(SmallCsrT<TV> A, SmallCsrT<TV> At, const std::vecto
 template <typename TV>
 double small_sigma(const DevCsr<TV>& A, const DevCsr<TV>& At, int64_t m, int64_t n,
                    int max_iterations, double tolerance, hipStream_t stream) {
-  if (m <= 0 || small_power_lds_bytes<TV>(m, n, A.nnz) > kSmallLdsMax) return NAN;
+  if (m <= 0 || small_power_lds_bytes<TV>(m, n, A.nnz) > small_lds_max()) return NAN;
   std::vector<double> start((size_t)m);
   start_vector(m, kSvSeed, start.data());
   return small_power_iteration<TV>(SmallCsrT<TV>{A.rowptr.p, A.col.p, A.val.p},

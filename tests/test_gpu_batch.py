@@ -219,6 +219,7 @@ def test_batch_block_graph_replays_equal_stream_launches():
     runs = []
     for graph in (False, True, True):
         bt = _batch(pp, K)
+        bt.set_kernel_path("general")
         bt.set_schedule(256, 256, 0, 32, 32)
         bt.set_block_graph(graph)
         queue = list(range(12))

@@ -123,6 +123,7 @@ def test_pack_block_graph_replays_equal_stream_launches():
     runs = []
     for graph in (False, True, True):
         pk = PDFPPack(pps)
+        pk.set_kernel_path("general")   # every instance fits in LDS
         pk.set_block_graph(graph)
         rec = []
         restarted = [False] * len(pps)

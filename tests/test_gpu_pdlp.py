@@ -51,7 +51,7 @@ def test_root_lp_p0201_at_pump_tolerance(path):
 
 
 def test_root_lp_small_path_runs_on_small_instances():
-    """p0201 fits in LDS (24·nnz + 52n + 36m + 8 bytes ≤ 62 KiB): the
+    """p0201 fits in LDS (24·nnz + 52n + 36m + 8 bytes ≤ 152 KiB): the
     default path runs whole blocks in one workgroup, and agrees with the
     general passes on the iteration count and, to FP64 summation-order
     rounding, on the point."""

@@ -157,7 +157,8 @@ def test_fp32_l1_matches_highs_and_kkt(case, path):
                                               iteration_limit=200_000), 1.0)
     assert status == "OPTIMAL"
     m_, n_ = pp.A.shape
-    fits = 4 * (n_ + 1) + 4 * (m_ + 1) + 16 * pp.A.nnz + 36 * n_ + 13 * m_ <= 62 * 1024
+    # MI355X: 160 KiB of LDS per workgroup, less 8 KiB for static LDS
+    fits = 4 * (n_ + 1) + 4 * (m_ + 1) + 16 * pp.A.nnz + 36 * n_ + 13 * m_ <= 152 * 1024
     assert s.info().small_path == (1 if path == "auto" and fits and pp.A.nnz > 0 else 0)
     x = pp.D_c * s.pdhg_x
     ref = case["l1_optimum"]
