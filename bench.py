@@ -447,6 +447,10 @@ def main():
     ap.add_argument("--batch-schedule", default=None,
                     help="npb_a,npb_at,persist,lane_rows_a,lane_rows_at: the batched passes' "
                          "schedule instead of the autotuner's (to profile a given form)")
+    ap.add_argument("--stream-launches", action="store_true",
+                    help="launch every block's passes on the stream instead of replaying "
+                         "captured block graphs (same kernels; for profilers that do not "
+                         "trace graph replays)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
@@ -490,6 +494,8 @@ def main():
     for prec in (["fp32"] if args.no_fp64 else ["fp32", "fp64"]):
         s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
                         precision=prec)
+        if args.stream_launches:
+            s.set_block_graph(False)
         s.set_x(np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         t0 = time.perf_counter()
         run_projections(s, pp, milp, targets_w, args.tol)  # warmup (includes ‖A‖₂)
@@ -517,6 +523,8 @@ def main():
         bt = PDFPBatch(K, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
         if args.batch_schedule:
             bt.set_schedule(*[int(v) for v in args.batch_schedule.split(",")])
+        if args.stream_launches:
+            bt.set_block_graph(False)
         for k in range(K):
             bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]

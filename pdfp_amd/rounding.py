@@ -135,10 +135,13 @@ class Rounder:
         ranked = self._int_arr[np.argsort(keys, kind="stable")]
         n = len(ranked)
         if iteration >= self.swap_iteration and n >= 10:
-            for _ in range(max(1, n // 10)):
-                a = int(rng.integers(1, n + 1)) - 1
-                b = int(rng.integers(1, n + 1)) - 1
-                ranked[a], ranked[b] = ranked[b], ranked[a]
+            # The draws a₁, b₁, a₂, b₂, … in one call: a Generator's vector
+            # of bounded integers is the sequence of its scalar draws.
+            ab = (rng.integers(1, n + 1, size=2 * max(1, n // 10)) - 1).tolist()
+            r = ranked.tolist()
+            for a, b in zip(ab[0::2], ab[1::2]):
+                r[a], r[b] = r[b], r[a]
+            ranked = np.array(r, dtype=ranked.dtype)
         return ranked
 
     def _row_violation(self, i: int, activity: float) -> float:

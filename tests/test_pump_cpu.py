@@ -265,3 +265,31 @@ def test_total_integer_row_violation_equals_row_loop(seed):
             d = act[i] - m.b[i]
             total += max(0.0, d) if m.row_sense[i] == "le" else abs(d)
         assert total_integer_row_violation(m, x) == total
+
+
+@pytest.mark.parametrize("n_int,seed", [(10, 1), (11, 2), (977, 3), (70_000, 4)])
+def test_ranking_swaps_follow_scalar_draws(n_int, seed):
+    """The ranking's |I|/10 transpositions (src/rounding.jl:108-112), drawn in
+    one vector call, equal the reference's loop of scalar draws
+    ``a = rand(rng, 1:len); b = rand(rng, 1:len)`` applied one swap at a time."""
+    from pdfp_amd.milp import MILP
+    from pdfp_amd.rounding import Rounder
+
+    rng0 = np.random.default_rng(seed)
+    n = n_int + 3
+    A = sp.random(4, n, density=0.5, random_state=seed, format="csc")
+    milp = MILP(c=np.zeros(n), c0=0.0, A=A, b=np.ones(4), row_sense=np.array(["le"] * 4),
+                l=np.zeros(n), u=np.full(n, 3.0), int_idx=np.arange(n_int))
+    x = rng0.uniform(0.0, 3.0, n)
+    r = Rounder(milp, backend="python")
+    got = r.ranking(x, np.random.Generator(np.random.PCG64(seed)), 10)
+    # the reference's loop, one scalar draw at a time
+    ranked = r.ranking(x, None, 0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    for _ in range(max(1, n_int // 10)):
+        a = int(rng.integers(1, n_int + 1)) - 1
+        b = int(rng.integers(1, n_int + 1)) - 1
+        ranked[a], ranked[b] = ranked[b], ranked[a]
+    assert np.array_equal(got, ranked)
+    assert sorted(got.tolist()) == list(range(n_int))
+    assert not np.array_equal(got, r.ranking(x, None, 0))
