@@ -192,8 +192,10 @@ const char* pdfp_batch_last_error(const pdfp_batch* ctx);
  * trajectory in lockstep blocks of termination_evaluation_frequency
  * iterations (which, with the reflection coefficient, the running
  * trajectories must share), each to its own termination, and returns after
- * the first block in which one ends (until_any = 1) or when none runs
- * (until_any = 0): termination[K] (0 for those still running),
+ * the first block in which one ends (until_any = 1), after one block
+ * (until_any = 2: a caller with trajectories waiting to start polls between
+ * blocks) or when none runs (until_any = 0): termination[K] (0 for those
+ * still running),
  * total_count[K], and the number still running.  A trajectory that has
  * ended can be given a new target and begun again while the others run on. */
 int pdfp_batch_begin(pdfp_batch* ctx, int32_t k, const pdfp_params* params,
@@ -290,8 +292,8 @@ int pdfp_pack_solve(pdfp_pack* ctx, const pdfp_params* params, const double* pri
 /* The continuous form (no counterpart in the reference; a pump over the
  * pack's instances uses it), as pdfp_batch_begin / pdfp_batch_run: instance
  * k starts a solve with its own parameters and primal weight; run iterates
- * every begun, unfinished instance until one ends (until_any = 1) or none
- * runs, and reports termination[K] (0 while running), total_count[K] and
+ * every begun, unfinished instance until one ends (until_any = 1), for one
+ * block (until_any = 2) or until none runs (until_any = 0), and reports termination[K] (0 while running), total_count[K] and
  * the number still running. */
 int pdfp_pack_begin(pdfp_pack* ctx, int32_t k, const pdfp_params* params,
                     double primal_weight);

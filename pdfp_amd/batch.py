@@ -94,14 +94,17 @@ class PDFPBatch:
         self._check(self._lib.pdfp_batch_begin(self._ctx, k, C.byref(cp), float(primal_weight)))
         self.termination_reason[k] = "UNSPECIFIED"
 
-    def run(self, until_any: bool = True) -> tuple[list, int]:
+    def run(self, until_any: bool = True, one_block: bool = False) -> tuple[list, int]:
         """Run the begun trajectories in lockstep blocks until one ends
-        (until_any) or none runs.  Returns (termination reasons, with
-        "UNSPECIFIED" for those still running; the number still running)."""
+        (until_any), for one block (one_block: a caller with trajectories waiting
+        to begin polls between blocks) or until none runs.  Returns
+        (termination reasons, with "UNSPECIFIED" for those still running;
+        the number still running)."""
+        mode = 2 if one_block else int(bool(until_any))
         term = (C.c_int32 * self.K)()
         total = (C.c_int64 * self.K)()
         left = C.c_int32()
-        self._check(self._lib.pdfp_batch_run(self._ctx, int(until_any), term, total,
+        self._check(self._lib.pdfp_batch_run(self._ctx, mode, term, total,
                                              C.byref(left)))
         self.termination_reason = [_lib.TERMINATION[int(t)] for t in term]
         self.last_total_count = [int(t) for t in total]

@@ -615,7 +615,8 @@ struct Batch {
   // trajectory, in lockstep blocks of `freq` iterations, each trajectory with
   // its own termination; returns after the first block in which one ends
   // (until_any) or when none runs.  Returns the number still running.
-  int run(bool until_any) {
+  // mode: 0 until none runs, 1 until one ends, 2 after one block.
+  int run(int mode) {
     int freq = 0;
     double omega = 0.0;
     for (const Trajectory& t : tr) {
@@ -710,7 +711,7 @@ struct Batch {
           ended = true;
         }
       }
-      if (until_any && ended) break;
+      if (mode == 2 || (mode == 1 && ended)) break;
     }
     sync();
     int n_running = 0;
@@ -724,7 +725,7 @@ struct Batch {
     for (Trajectory& t : tr) t.running = false;
     for (int k = 0; k < K; ++k)
       if (!act || act[k]) begin(k, p, pw[k]);
-    run(false);
+    run(0);
   }
 
   void polish(int k) {
@@ -977,7 +978,9 @@ int pdfp_batch_begin(pdfp_batch* ctx, int32_t k, const pdfp_params* params,
 int pdfp_batch_run(pdfp_batch* ctx, int32_t until_any, int32_t* termination,
                    int64_t* total_count, int32_t* n_running) {
   return batch_guarded(ctx, [&] {
-    const int left = ctx->impl->run(until_any != 0);
+    if (until_any < 0 || until_any > 2)
+      throw pdfp::Error(PDFP_ERR_ARGUMENT, "until_any must be 0, 1 or 2");
+    const int left = ctx->impl->run(until_any);
     if (n_running) *n_running = left;
     for (int k = 0; k < ctx->impl->K; ++k) {
       if (termination) termination[k] = ctx->impl->tr[(size_t)k].termination;

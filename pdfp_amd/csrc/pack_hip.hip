@@ -603,7 +603,8 @@ struct Pack {
   // instance, in lockstep blocks of `freq` iterations, each instance with
   // its own termination; returns after the first block in which one ends
   // (until_any) or when none runs.  Returns the number still running.
-  int run(bool until_any) {
+  // mode: 0 until none runs, 1 until one ends, 2 after one block.
+  int run(int mode) {
     int freq = 0;
     double omega = 0.0;
     for (const PackInstance& I : in) {
@@ -694,7 +695,7 @@ struct Pack {
             y0.p, z.p, z0.p);
         sync();  // h_flag is rewritten in the next block
       }
-      if (until_any && ended) break;
+      if (mode == 2 || (mode == 1 && ended)) break;
     }
     sync();
     int n_running = 0;
@@ -706,7 +707,7 @@ struct Pack {
     for (PackInstance& I : in) I.running = false;
     for (int k = 0; k < K; ++k)
       if (!act || act[k]) begin(k, p, pw[k]);
-    run(false);
+    run(0);
   }
 
   // The FP64 polish of instance k's projection and its ℓ1 distance.
@@ -974,7 +975,9 @@ int pdfp_pack_run(pdfp_pack* ctx, int32_t until_any, int32_t* termination,
                   int64_t* total_count, int32_t* n_running) {
   return pack_guarded(ctx, [&] {
     pdfp::Pack& P = *ctx->impl;
-    const int left = P.run(until_any != 0);
+    if (until_any < 0 || until_any > 2)
+      throw pdfp::Error(PDFP_ERR_ARGUMENT, "until_any must be 0, 1 or 2");
+    const int left = P.run(until_any);
     for (int k = 0; k < P.K; ++k) {
       const pdfp::PackInstance& I = P.in[(size_t)k];
       if (termination) termination[k] = I.running ? PDFP_UNSPECIFIED : I.termination;

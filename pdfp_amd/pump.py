@@ -658,7 +658,9 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                 running.add(k)
         pending = {}
         if running:
-            terms, _ = batch.run(until_any=True)
+            # with projections waiting on the host, return after each block
+            # so that those that end their rounding re-enter at the next one
+            terms, _ = batch.run(until_any=True, one_block=bool(futures))
             for k in sorted(running):
                 if terms[k] != "UNSPECIFIED":
                     running.discard(k)
@@ -814,7 +816,9 @@ def feasibility_pump_pack(milps, *, seed: int = 1, time_limit: float = 1800.0,
                 running.add(k)
         pending = {}
         if running:
-            terms, _ = pack.run(until_any=True)
+            # with projections waiting on the host, return after each block
+            # so that those that end their rounding re-enter at the next one
+            terms, _ = pack.run(until_any=True, one_block=bool(futures))
             for k in sorted(running):
                 if terms[k] != "UNSPECIFIED":
                     running.discard(k)

@@ -163,6 +163,43 @@ def test_batch_continuous_begin_and_run():
         bt.run()
 
 
+def test_batch_one_block_runs_match_until_any():
+    """pdfp_batch_run with until_any = 2 returns after every block: stepped
+    block by block, two trajectories reach the same terminations, iteration
+    counts and points, bit for bit, as when each call runs until one ends."""
+    print("batch one-block runs", flush=True)
+    m = G.mixed(400, 800, 32)
+    pp = projector_problem(m)
+    tg = [G.binary_targets(m, 950 + k) / pp.D_c for k in range(2)]
+    p = [SolverParameters(eps_optimal_relative=1e-3, eps_feasible_relative=1e-4),
+         SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                          iteration_limit=200_000)]
+    out = []
+    for one_block in (False, True):
+        bt = _batch(pp, 2)
+        bt.set_kernel_path("general")
+        bt.set_schedule(256, 256, 0, 32, 32)
+        for k in range(2):
+            bt.set_target(k, tg[k])
+            bt.begin(k, p[k], 1.0)
+        left, calls, prev = 2, 0, [0, 0]
+        while left:
+            terms, left = bt.run(until_any=True, one_block=one_block)
+            calls += 1
+            if one_block:   # each call is one block of 200 for every running one
+                counts = bt.last_total_count
+                assert all(counts[k] - prev[k] in (0, 200) for k in range(2))
+                assert max(counts[k] - prev[k] for k in range(2)) == 200
+                prev = counts
+        out.append((terms, [bt.total_count(k) for k in range(2)],
+                    [bt.pdhg_x(k) for k in range(2)], calls))
+        bt.close()
+    (t0, c0, x0, n0), (t1, c1, x1, n1) = out
+    assert t0 == t1 == ["OPTIMAL", "OPTIMAL"] and c0 == c1
+    assert all(np.array_equal(a, b) for a, b in zip(x0, x1))
+    assert n1 == max(c1) // 200 and n0 < n1
+
+
 def test_batch_schedules_agree():
     """Every schedule form of the batched passes (nonzeros per workgroup,
     persistent grid, lane-per-(row, trajectory) reduction of short-row
