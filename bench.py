@@ -204,8 +204,8 @@ def _oracle_pump_backends():
 
 def time_to_first_feasible(runs: int, cpu: bool):
     """The full pump (root LP, projections, rounding) on the 133 × 201 binary
-    set-covering instance (seed 1), on p0201 and on a 2000 × 4000 mixed MILP
-    (seed 1): median PumpResult.time over `runs` runs after one warm-up run,
+    set-covering instance (seed 1), on p0201 and on 2000 × 4000 and
+    10,000 × 20,000 mixed MILPs (seed 1; 40k and 200k nonzeros): median PumpResult.time over `runs` runs after one warm-up run,
     as the reference times it (scripts/solve.jl:88-97); FP32 and FP64 single
     trajectories and 8 concurrent trajectories on the batched projector
     (first feasible wins, as scripts/solve.jl:121-148); and the four example
@@ -219,7 +219,8 @@ def time_to_first_feasible(runs: int, cpu: bool):
     cases = [("set_cover_133x201_seed1", G.set_cover(133, 201, 1)),
              ("p0201", read_mps(os.path.join(ROOT, "tests", "golden", "examples",
                                              "p0201.mps.gz"))),
-             ("mixed_2000x4000_seed1", G.mixed(2000, 4000, 1))]
+             ("mixed_2000x4000_seed1", G.mixed(2000, 4000, 1)),
+             ("mixed_10000x20000_seed1", G.mixed(10_000, 20_000, 1))]
     feasibility_pump(G.set_cover(133, 201, 7), time_limit=60.0)        # warm-up
     feasibility_pump_batch(G.set_cover(133, 201, 7), time_limit=60.0)
     forms = {"fp32": lambda m: feasibility_pump(m, time_limit=60.0, precision="fp32"),
