@@ -1,13 +1,13 @@
 """Benchmark of the PDHG projector on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed-1m|powerlaw-5m|setcover-pack-64|general-int-4m]
 
-Workload (default, "c2"): the mixed MILP with 50,000 rows × 100,000 columns
+Workload (default, "mixed-1m"): the mixed MILP with 50,000 rows × 100,000 columns
 (70 % binary, 30 % continuous in [0, 10]), 1,000,000 nonzeros, row lengths
 uniform in 10-30, values N(0,1), b = A x0 + |N(0,1)| for a random feasible
-x0 (seed 1).  ``--workload c3``: 500,000 × 1,000,000 with exactly 5,000,000
-nonzeros in power-law rows; ``c5``: a general-integer, equality-heavy MILP of
-200,000 × 400,000 with 4M nonzeros; ``c4``: 64 set-covering instances of
+x0 (seed 1).  ``--workload powerlaw-5m``: 500,000 × 1,000,000 with exactly 5,000,000
+nonzeros in power-law rows; ``general-int-4m``: a general-integer, equality-heavy MILP of
+200,000 × 400,000 with 4M nonzeros; ``setcover-pack-64``: 64 set-covering instances of
 uneven sizes in one block-diagonal pack (15.5M nonzeros in all).  A step is one warm-started ℓ1
 projection toward a new random integer target (0/1 on binaries) at relative
 KKT tolerance 1e-4, the way the pump calls the projector
@@ -28,7 +28,8 @@ launch over its mean duration, timed with the kernel's own start/stop events
 on the library's stream; HBM traffic per launch from the committed rocprofv3
 counter summary of the same workload, when there is one), the CPU baseline
 (the FP64 C oracle on one host core, on a bounded number of iterations of the
-same workload) and the time to the first feasible point of two small MILPs.
+same workload) and the time to the first feasible point of four MILPs and of the
+four example instances' pumps on one pack.
 
     python bench.py --copy-calibration
 
@@ -55,15 +56,15 @@ HBM_PEAK_GBS = 8000.0
 def workload(name: str):
     from pdfp_amd import generators as G
 
-    if name == "c2":
+    if name == "mixed-1m":
         m = G.mixed(50_000, 100_000, 1)
         desc = ("mixed MILP 50000 rows x 100000 cols (70% binary, 30% continuous [0,10]), "
                 "1M nnz, rows of 10-30, N(0,1) values, b = A x0 + |N(0,1)|, seed 1")
-    elif name == "c3":
+    elif name == "powerlaw-5m":
         m = G.mixed(500_000, 1_000_000, 1, power_law=True, nnz=5_000_000, max_len=5000)
         desc = ("mixed MILP 500000 rows x 1000000 cols, 5M nnz, power-law row lengths "
                 "(Zipf 1.5, max 5000), N(0,1) values, seed 1")
-    elif name == "c5":
+    elif name == "general-int-4m":
         m = G.general_integer(200_000, 400_000, 1)
         desc = ("general-integer MILP 200000 rows (40% equality) x 400000 integer cols in "
                 "[0, U_j], U_j in 1..20, 4M nnz of U{-9..9}, rows of 10-30, seed 1")
@@ -292,7 +293,7 @@ def copy_calibration(nbytes: int = 1 << 30, reps: int = 10):
 
 
 def bench_pack(args) -> None:
-    """``--workload c4``: 64 set-covering instances of uneven sizes (seeds
+    """``--workload setcover-pack-64``: 64 set-covering instances of uneven sizes (seeds
     1..64; rows e^U[ln 2k, ln 40k], columns e^U[ln 4k, ln 80k]) in one
     block-diagonal pack.  A step is one warm-started ℓ1 projection of every
     instance to a new 0/1 target at the tolerance, all solved together; the
@@ -312,7 +313,7 @@ def bench_pack(args) -> None:
     desc = (f"64 set-covering instances of uneven sizes (seeds 1..64, rows e^U[ln 2k, ln 40k], "
             f"columns e^U[ln 4k, ln 80k], each column covering 3-15 rows): {mm} rows, {nn} "
             f"columns, {nnz} nnz in total, one block-diagonal pack")
-    print(f"[bench] workload c4: {mm}x{nn}, nnz {nnz}, built in {time.perf_counter() - t0:.1f}s",
+    print(f"[bench] workload setcover-pack-64: {mm}x{nn}, nnz {nnz}, built in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     K = len(milps)
     p = P.SolverParameters(eps_optimal_relative=args.tol, eps_feasible_relative=args.tol,
@@ -404,7 +405,7 @@ def bench_pack(args) -> None:
                    "iterations_timed": it},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("c4", dom_key), "algorithmic_bytes": dom_bytes,
+                     "traffic": measured_traffic("setcover-pack-64", dom_key), "algorithmic_bytes": dom_bytes,
                      "kernel_ms": dom_ms},
         "iteration_us": ms_iter * 1e3,
         "kernel_us": {"transpose_pass": ms_at * 1e3, "row_pass": ms_a * 1e3},
@@ -421,7 +422,7 @@ def bench_pack(args) -> None:
 def measured_traffic(workload: str, kernel_key: str):
     """HBM bytes per launch of a kernel from the committed counter summary
     (profiles/pmc_traffic_<workload>.md, its ```json block), or None."""
-    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.md")
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload.replace('-', '_')}.md")
     if not os.path.exists(path):
         return None
     text = open(path).read()
@@ -437,7 +438,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="mixed-1m")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=1000)
@@ -463,7 +464,7 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "c4":
+    if args.workload == "setcover-pack-64":
         bench_pack(args)
         return
 
