@@ -41,61 +41,96 @@ inline double pmax(double a, double b) { return b > a ? b : a; }  // Python's ma
 inline double pmin(double a, double b) { return b < a ? b : a; }  // Python's min(a, b)
 
 // Everything about the propagation that depends on the model only
-// (src/propagation.jl:19-84).
+// (src/propagation.jl:19-84).  The propagation visits rows and variables in
+// an order set by the model's graph, so it is bound by the latency of its
+// scattered accesses: each row's and each variable's fields sit together
+// (one or two cache lines per visit), and each column's and row's nonzeros
+// are one array of (index, value) pairs.
+struct Entry {
+  int32_t idx;  // the row of a column's nonzero, the column of a row's
+  double val;
+};
+
+// A variable's domain, its tightening counts and its integrality.
+struct alignas(32) Var {
+  double l, u;
+  int32_t tlo, thi;
+  int32_t is_int, pad;
+};
+
+// A row's activity bounds, the largest |a_ij|·(u_j - l_j) seen on its last
+// scan, and its right-hand side.
+struct alignas(32) Row {
+  double L_min, L_max, maxroom, b;
+};
+
+enum : uint8_t { kLe = 1 };  // row flags: bit 0 is_le, bits 1-2 the RowKind
+
 struct Data {
   int64_t n = 0, m = 0;
-  std::vector<double> l, u, b;
-  std::vector<uint8_t> is_int, is_le;
+  std::vector<Var> var;             // initial domains, counts zero
+  std::vector<Row> row;             // initial activity bounds
+  std::vector<uint8_t> flags;       // is_le | kind << 1
+  std::vector<double> b, row_maxcoef;
+  std::vector<uint8_t> is_le;
   std::vector<int64_t> col_ptr, row_ptr;
-  std::vector<int32_t> col_rows, row_cols;
-  std::vector<double> col_vals, row_vals;
-  std::vector<double> L_min, L_max, row_maxcoef, row_maxroom;
-  std::vector<uint8_t> row_kind;
+  std::vector<Entry> col_nz, row_nz;
 };
 
 // Domains of one rounding pass and the FIFO propagation queue
-// (src/propagation.jl:87-107).
+// (src/propagation.jl:87-107).  A row is queued at most once at a time
+// (in_queue), so the queue is a ring of m slots.
 struct State {
   const Data& d;
-  std::vector<double> l, u, L_min, L_max, row_maxroom;
+  std::vector<Var> v;
+  std::vector<Row> r;
   std::vector<uint8_t> in_queue;
-  std::deque<int32_t> queue;
-  std::vector<int32_t> tighten_lo, tighten_hi;
+  std::vector<int32_t> ring;
+  size_t head = 0, count = 0;
   int64_t max_pops;
   bool infeasible = false;
 
   State(const Data& data, int64_t pops)
-      : d(data), l(data.l), u(data.u), L_min(data.L_min), L_max(data.L_max),
-        row_maxroom(data.row_maxroom), in_queue((size_t)data.m, 0),
-        tighten_lo((size_t)data.n, 0), tighten_hi((size_t)data.n, 0), max_pops(pops) {}
+      : d(data), v(data.var), r(data.row), in_queue((size_t)data.m, 0),
+        ring((size_t)data.m), max_pops(pops) {}
+
+  double lo(int64_t j) const { return v[(size_t)j].l; }
+  double hi(int64_t j) const { return v[(size_t)j].u; }
 
   void enqueue(int32_t i) {
     if (!in_queue[(size_t)i]) {
-      queue.push_back(i);
+      size_t tail = head + count;
+      if (tail >= ring.size()) tail -= ring.size();
+      ring[tail] = i;
+      ++count;
       in_queue[(size_t)i] = 1;
     }
   }
 
-  bool is_fixed(int64_t j) const { return std::fabs(u[(size_t)j] - l[(size_t)j]) < kBoundTol; }
+  bool is_fixed(int64_t j) const {
+    const Var& x = v[(size_t)j];
+    return std::fabs(x.u - x.l) < kBoundTol;
+  }
 
   // src/propagation.jl:117-136
   bool tighten_upper(int64_t j, double new_u) {
-    const size_t J = (size_t)j;
-    const double u_old = u[J];
+    Var& x = v[(size_t)j];
+    const double u_old = x.u;
     if (!(new_u < u_old - kBoundTol)) return false;
-    double nu = d.is_int[J] ? std::floor(new_u + kBoundTol) : new_u;
-    if (nu < l[J] - kBoundTol) nu = l[J];
-    nu = pmax(nu, l[J]);
+    double nu = x.is_int ? std::floor(new_u + kBoundTol) : new_u;
+    if (nu < x.l - kBoundTol) nu = x.l;
+    nu = pmax(nu, x.l);
     if (nu >= u_old - kBoundTol) return false;
-    u[J] = nu;
-    tighten_hi[J] += 1;
-    const bool wake = tighten_hi[J] <= kMaxTightenings || std::fabs(nu - l[J]) < kBoundTol;
+    x.u = nu;
+    x.thi += 1;
+    const bool wake = x.thi <= kMaxTightenings || std::fabs(nu - x.l) < kBoundTol;
     const double du = nu - u_old;
-    for (int64_t p = d.col_ptr[J]; p < d.col_ptr[J + 1]; ++p) {
-      const int32_t i = d.col_rows[(size_t)p];
-      const double a = d.col_vals[(size_t)p];
-      if (a > 0) L_max[(size_t)i] += a * du;
-      else L_min[(size_t)i] += a * du;
+    const Entry* e = d.col_nz.data();
+    for (int64_t p = d.col_ptr[(size_t)j]; p < d.col_ptr[(size_t)j + 1]; ++p) {
+      const int32_t i = e[p].idx;
+      const double a = e[p].val;
+      if (a > 0) r[(size_t)i].L_max += a * du;
+      else r[(size_t)i].L_min += a * du;
       if (wake) enqueue(i);
     }
     return true;
@@ -103,22 +138,23 @@ struct State {
 
   // src/propagation.jl:138-156
   bool tighten_lower(int64_t j, double new_l) {
-    const size_t J = (size_t)j;
-    const double l_old = l[J];
+    Var& x = v[(size_t)j];
+    const double l_old = x.l;
     if (!(new_l > l_old + kBoundTol)) return false;
-    double nl = d.is_int[J] ? std::ceil(new_l - kBoundTol) : new_l;
-    if (nl > u[J] + kBoundTol) nl = u[J];
-    nl = pmin(nl, u[J]);
+    double nl = x.is_int ? std::ceil(new_l - kBoundTol) : new_l;
+    if (nl > x.u + kBoundTol) nl = x.u;
+    nl = pmin(nl, x.u);
     if (nl <= l_old + kBoundTol) return false;
-    l[J] = nl;
-    tighten_lo[J] += 1;
-    const bool wake = tighten_lo[J] <= kMaxTightenings || std::fabs(u[J] - nl) < kBoundTol;
+    x.l = nl;
+    x.tlo += 1;
+    const bool wake = x.tlo <= kMaxTightenings || std::fabs(x.u - nl) < kBoundTol;
     const double dl = nl - l_old;
-    for (int64_t p = d.col_ptr[J]; p < d.col_ptr[J + 1]; ++p) {
-      const int32_t i = d.col_rows[(size_t)p];
-      const double a = d.col_vals[(size_t)p];
-      if (a > 0) L_min[(size_t)i] += a * dl;
-      else L_max[(size_t)i] += a * dl;
+    const Entry* e = d.col_nz.data();
+    for (int64_t p = d.col_ptr[(size_t)j]; p < d.col_ptr[(size_t)j + 1]; ++p) {
+      const int32_t i = e[p].idx;
+      const double a = e[p].val;
+      if (a > 0) r[(size_t)i].L_min += a * dl;
+      else r[(size_t)i].L_max += a * dl;
       if (wake) enqueue(i);
     }
     return true;
@@ -127,10 +163,10 @@ struct State {
   // src/propagation.jl:160-177
   void propagate_cardinality_row(int32_t i) {
     const size_t I = (size_t)i;
-    const double b = d.b[I];
-    const double slack_up = b - L_min[I];
-    const double slack_dn = L_max[I] - b;
-    const bool is_eq = !d.is_le[I];
+    const double b = r[I].b;
+    const double slack_up = b - r[I].L_min;
+    const double slack_dn = r[I].L_max - b;
+    const bool is_eq = !(d.flags[I] & kLe);
     if (slack_up < -kInfeasTol || (is_eq && slack_dn < -kInfeasTol)) {
       infeasible = true;
       return;
@@ -138,60 +174,65 @@ struct State {
     const bool up = slack_up < 1.0 - kBoundTol;
     const bool dn = is_eq && slack_dn < 1.0 - kBoundTol;
     if (!(up || dn)) return;
+    const Entry* e = d.row_nz.data();
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const int32_t j = d.row_cols[(size_t)k];
-      if (up) tighten_upper(j, l[(size_t)j] + slack_up);
-      if (dn) tighten_lower(j, u[(size_t)j] - slack_dn);
+      const int32_t j = e[k].idx;
+      if (up) tighten_upper(j, v[(size_t)j].l + slack_up);
+      if (dn) tighten_lower(j, v[(size_t)j].u - slack_dn);
     }
   }
 
   // src/propagation.jl:180-192
   void propagate_knapsack_row(int32_t i) {
     const size_t I = (size_t)i;
-    const double slack_up = d.b[I] - L_min[I];
+    const double slack_up = r[I].b - r[I].L_min;
     if (slack_up < -kInfeasTol) {
       infeasible = true;
       return;
     }
     if (d.row_maxcoef[I] <= slack_up + kBoundTol) return;
+    const Entry* e = d.row_nz.data();
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const int32_t j = d.row_cols[(size_t)k];
-      if (d.row_vals[(size_t)k] > slack_up + kBoundTol) tighten_upper(j, l[(size_t)j]);
+      const int32_t j = e[k].idx;
+      if (e[k].val > slack_up + kBoundTol) tighten_upper(j, v[(size_t)j].l);
     }
   }
 
   // src/propagation.jl:194-232
   void propagate_row(int32_t i) {
     const size_t I = (size_t)i;
-    const uint8_t kind = d.row_kind[I];
+    const uint8_t f = d.flags[I];
+    const uint8_t kind = f >> 1;
     if (kind == kCardinality) return propagate_cardinality_row(i);
     if (kind == kKnapsack) return propagate_knapsack_row(i);
-    const bool is_le = d.is_le[I];
-    const double b = d.b[I];
-    const double slack_up = b - L_min[I];
-    const double slack_dn = L_max[I] - b;
+    const bool is_le = f & kLe;
+    Row& R = r[I];
+    const double b = R.b;
+    const double slack_up = b - R.L_min;
+    const double slack_dn = R.L_max - b;
     if (slack_up < -kInfeasTol || (!is_le && slack_dn < -kInfeasTol)) {
       infeasible = true;
       return;
     }
     const double threshold = (is_le ? slack_up : pmin(slack_up, slack_dn)) + kBoundTol;
-    if (row_maxroom[I] <= threshold) return;
+    if (R.maxroom <= threshold) return;
+    const Entry* e = d.row_nz.data();
     double max_room = 0.0;
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const int32_t j = d.row_cols[(size_t)k];
-      const double room = std::fabs(d.row_vals[(size_t)k]) * (u[(size_t)j] - l[(size_t)j]);
+      const Var& x = v[(size_t)e[k].idx];
+      const double room = std::fabs(e[k].val) * (x.u - x.l);
       if (room > max_room) max_room = room;
     }
-    row_maxroom[I] = max_room;
+    R.maxroom = max_room;
     if (is_le && max_room <= slack_up + kBoundTol) return;
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const int32_t j = d.row_cols[(size_t)k];
-      const double a = d.row_vals[(size_t)k];
-      if (a > 0) tighten_upper(j, l[(size_t)j] + slack_up / a);
-      else tighten_lower(j, u[(size_t)j] + slack_up / a);
+      const int32_t j = e[k].idx;
+      const double a = e[k].val;
+      if (a > 0) tighten_upper(j, v[(size_t)j].l + slack_up / a);
+      else tighten_lower(j, v[(size_t)j].u + slack_up / a);
       if (!is_le) {
-        if (a > 0) tighten_lower(j, u[(size_t)j] - slack_dn / a);
-        else tighten_upper(j, l[(size_t)j] - slack_dn / a);
+        if (a > 0) tighten_lower(j, v[(size_t)j].u - slack_dn / a);
+        else tighten_upper(j, v[(size_t)j].l - slack_dn / a);
       }
     }
   }
@@ -199,9 +240,10 @@ struct State {
   // src/propagation.jl:235-244
   int64_t propagate() {
     int64_t pops = 0;
-    while (!queue.empty() && pops < max_pops) {
-      const int32_t i = queue.front();
-      queue.pop_front();
+    while (count > 0 && pops < max_pops) {
+      const int32_t i = ring[head];
+      if (++head == ring.size()) head = 0;
+      --count;
       in_queue[(size_t)i] = 0;
       propagate_row(i);
       ++pops;
@@ -211,29 +253,31 @@ struct State {
 
   // src/propagation.jl:252-274
   bool fix(int64_t j, double value) {
-    const size_t J = (size_t)j;
-    const double l_old = l[J], u_old = u[J];
+    Var& x = v[(size_t)j];
+    const double l_old = x.l, u_old = x.u;
     if (value < l_old - kBoundTol || value > u_old + kBoundTol) {
       infeasible = true;
       return false;
     }
-    const double v = value > u_old ? u_old : (value < l_old ? l_old : value);
-    if (std::fabs(v - l_old) < kBoundTol && std::fabs(v - u_old) < kBoundTol) return true;
-    l[J] = v;
-    u[J] = v;
-    tighten_lo[J] += 1;
-    tighten_hi[J] += 1;
-    const double dl = v - l_old;
-    const double du = v - u_old;
-    for (int64_t p = d.col_ptr[J]; p < d.col_ptr[J + 1]; ++p) {
-      const int32_t i = d.col_rows[(size_t)p];
-      const double a = d.col_vals[(size_t)p];
+    const double val = value > u_old ? u_old : (value < l_old ? l_old : value);
+    if (std::fabs(val - l_old) < kBoundTol && std::fabs(val - u_old) < kBoundTol) return true;
+    x.l = val;
+    x.u = val;
+    x.tlo += 1;
+    x.thi += 1;
+    const double dl = val - l_old;
+    const double du = val - u_old;
+    const Entry* e = d.col_nz.data();
+    for (int64_t p = d.col_ptr[(size_t)j]; p < d.col_ptr[(size_t)j + 1]; ++p) {
+      const int32_t i = e[p].idx;
+      const double a = e[p].val;
+      Row& R = r[(size_t)i];
       if (a > 0) {
-        L_min[(size_t)i] += a * dl;
-        L_max[(size_t)i] += a * du;
+        R.L_min += a * dl;
+        R.L_max += a * du;
       } else {
-        L_min[(size_t)i] += a * du;
-        L_max[(size_t)i] += a * dl;
+        R.L_min += a * du;
+        R.L_max += a * dl;
       }
       enqueue(i);
     }
@@ -252,8 +296,7 @@ Data build_data(int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowv
   d.m = m;
   const int64_t nnz = colptr[n] - base;
   d.col_ptr.assign((size_t)n + 1, 0);
-  d.col_rows.resize((size_t)nnz);
-  d.col_vals.resize((size_t)nnz);
+  d.col_nz.resize((size_t)nnz);
   for (int64_t j = 0; j <= n; ++j) d.col_ptr[(size_t)j] = colptr[j] - base;
   std::vector<int64_t> count((size_t)m + 1, 0);
   for (int64_t j = 0; j < n; ++j) {
@@ -262,58 +305,54 @@ Data build_data(int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowv
     for (int64_t p = d.col_ptr[(size_t)j]; p < d.col_ptr[(size_t)j + 1]; ++p) {
       const int64_t i = rowval[p] - base;
       if (i < 0 || i >= m) throw Error(PDFP_ERR_ARGUMENT, "row index out of range");
-      d.col_rows[(size_t)p] = (int32_t)i;
-      d.col_vals[(size_t)p] = nzval[p];
+      d.col_nz[(size_t)p] = Entry{(int32_t)i, nzval[p]};
       count[(size_t)i + 1]++;
     }
   }
   // rows with ascending columns (counting sort over the columns in order)
   d.row_ptr.assign((size_t)m + 1, 0);
   for (int64_t i = 0; i < m; ++i) d.row_ptr[(size_t)i + 1] = d.row_ptr[(size_t)i] + count[(size_t)i + 1];
-  d.row_cols.resize((size_t)nnz);
-  d.row_vals.resize((size_t)nnz);
+  d.row_nz.resize((size_t)nnz);
   std::vector<int64_t> next(d.row_ptr.begin(), d.row_ptr.end() - 1);
   for (int64_t j = 0; j < n; ++j)
     for (int64_t p = d.col_ptr[(size_t)j]; p < d.col_ptr[(size_t)j + 1]; ++p) {
-      const int64_t q = next[(size_t)d.col_rows[(size_t)p]]++;
-      d.row_cols[(size_t)q] = (int32_t)j;
-      d.row_vals[(size_t)q] = d.col_vals[(size_t)p];
+      const int64_t q = next[(size_t)d.col_nz[(size_t)p].idx]++;
+      d.row_nz[(size_t)q] = Entry{(int32_t)j, d.col_nz[(size_t)p].val};
     }
-  d.l.resize((size_t)n);
-  d.u.resize((size_t)n);
-  d.is_int.assign(is_int, is_int + n);
+  d.var.resize((size_t)n);
   for (int64_t j = 0; j < n; ++j) {
-    d.l[(size_t)j] = std::isfinite(lo[j]) ? lo[j] : -kDomainInfinity;
-    d.u[(size_t)j] = std::isfinite(hi[j]) ? hi[j] : kDomainInfinity;
+    Var& x = d.var[(size_t)j];
+    x.l = std::isfinite(lo[j]) ? lo[j] : -kDomainInfinity;
+    x.u = std::isfinite(hi[j]) ? hi[j] : kDomainInfinity;
+    x.tlo = x.thi = 0;
+    x.is_int = is_int[j] ? 1 : 0;
+    x.pad = 0;
   }
   d.b.assign(b, b + m);
   d.is_le.assign(is_le, is_le + m);
-  d.L_min.assign((size_t)m, 0.0);
-  d.L_max.assign((size_t)m, 0.0);
-  d.row_kind.assign((size_t)m, kGeneral);
+  d.row.resize((size_t)m);
+  d.flags.assign((size_t)m, 0);
   d.row_maxcoef.assign((size_t)m, 0.0);
-  d.row_maxroom.assign((size_t)m, 0.0);
   for (int64_t i = 0; i < m; ++i) {
     const size_t I = (size_t)i;
+    const Entry* e = d.row_nz.data();
     double lmin = 0.0, lmax = 0.0;
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const size_t j = (size_t)d.row_cols[(size_t)k];
-      const double a = d.row_vals[(size_t)k];
+      const Var& x = d.var[(size_t)e[k].idx];
+      const double a = e[k].val;
       if (a > 0) {
-        lmin += a * d.l[j];
-        lmax += a * d.u[j];
+        lmin += a * x.l;
+        lmax += a * x.u;
       } else {
-        lmin += a * d.u[j];
-        lmax += a * d.l[j];
+        lmin += a * x.u;
+        lmax += a * x.l;
       }
     }
-    d.L_min[I] = lmin;
-    d.L_max[I] = lmax;
     bool all_one = true, all_pos = true, all_bin = true;
     double maxcoef = 0.0;
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const int64_t j = d.row_cols[(size_t)k];
-      const double a = d.row_vals[(size_t)k];
+      const int64_t j = e[k].idx;
+      const double a = e[k].val;
       if (std::fabs(a) > maxcoef) maxcoef = std::fabs(a);
       if (a != 1.0) all_one = false;
       if (!(a > 0.0)) all_pos = false;
@@ -321,16 +360,18 @@ Data build_data(int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowv
       if (!(all_bin && (all_one || all_pos))) break;
     }
     d.row_maxcoef[I] = maxcoef;
-    if (all_bin && all_one) d.row_kind[I] = kCardinality;
-    else if (all_bin && all_pos && d.is_le[I]) d.row_kind[I] = kKnapsack;
+    uint8_t kind = kGeneral;
+    if (all_bin && all_one) kind = kCardinality;
+    else if (all_bin && all_pos && is_le[i]) kind = kKnapsack;
+    d.flags[I] = (uint8_t)((is_le[i] ? kLe : 0) | (kind << 1));
     double room = 0.0;
     for (int64_t k = d.row_ptr[I]; k < d.row_ptr[I + 1]; ++k) {
-      const int64_t j = d.row_cols[(size_t)k];
+      const int64_t j = e[k].idx;
       const double width = hi[j] - lo[j];
-      const double r = std::isfinite(width) ? std::fabs(d.row_vals[(size_t)k]) * width : INFINITY;
-      if (r > room) room = r;
+      const double rr = std::isfinite(width) ? std::fabs(e[k].val) * width : INFINITY;
+      if (rr > room) room = rr;
     }
-    d.row_maxroom[I] = room;
+    d.row[I] = Row{lmin, lmax, room, b[i]};
   }
   return d;
 }
@@ -352,15 +393,15 @@ struct Rounder {
   double guided_value(const State& s, int64_t j, double target,
                       const std::vector<double>& activity) const {
     const size_t J = (size_t)j;
-    if (!(s.u[J] - s.l[J] > kBoundTol)) return target;
-    const double lo = std::ceil(s.l[J] - kBoundTol);
-    const double hi = std::floor(s.u[J] + kBoundTol);
+    if (!(s.hi(j) - s.lo(j) > kBoundTol)) return target;
+    const double lo = std::ceil(s.lo(j) - kBoundTol);
+    const double hi = std::floor(s.hi(j) + kBoundTol);
     if (!(hi > lo)) return target;
     auto score = [&](double v) {
       double t = 0.0;
       for (int64_t p = d.col_ptr[J]; p < d.col_ptr[J + 1]; ++p) {
-        const size_t i = (size_t)d.col_rows[(size_t)p];
-        const double a = d.col_vals[(size_t)p];
+        const size_t i = (size_t)d.col_nz[(size_t)p].idx;
+        const double a = d.col_nz[(size_t)p].val;
         t += row_violation(i, activity[i] + a * v) - row_violation(i, activity[i]);
       }
       return t;
@@ -374,7 +415,7 @@ struct Rounder {
 
   void add_activity(std::vector<double>& activity, int64_t j, double v) const {
     for (int64_t p = d.col_ptr[(size_t)j]; p < d.col_ptr[(size_t)j + 1]; ++p)
-      activity[(size_t)d.col_rows[(size_t)p]] += d.col_vals[(size_t)p] * v;
+      activity[(size_t)d.col_nz[(size_t)p].idx] += d.col_nz[(size_t)p].val * v;
   }
 
   // src/rounding.jl:152-174
@@ -384,14 +425,14 @@ struct Rounder {
       const int64_t j = idx[q];
       const size_t J = (size_t)j;
       if (s.is_fixed(j)) {
-        xt[J] = s.l[J];
+        xt[J] = s.lo(j);
         continue;
       }
-      double target = jround(jclamp(x_star[J], s.l[J], s.u[J]));
-      if (target < s.l[J] - kBoundTol || target > s.u[J] + kBoundTol) {
+      double target = jround(jclamp(x_star[J], s.lo(j), s.hi(j)));
+      if (target < s.lo(j) - kBoundTol || target > s.hi(j) + kBoundTol) {
         const double other =
-            jround(jclamp(x_star[J] + (target < s.l[J] ? 1.0 : -1.0), s.l[J], s.u[J]));
-        target = jclamp(other, s.l[J], s.u[J]);
+            jround(jclamp(x_star[J] + (target < s.lo(j) ? 1.0 : -1.0), s.lo(j), s.hi(j)));
+        target = jclamp(other, s.lo(j), s.hi(j));
       }
       if (activity && direction[J]) target = guided_value(s, j, target, *activity);
       s.fix(j, target);
@@ -404,7 +445,7 @@ struct Rounder {
   void read_integers(const State& s, double* xt, const double* x_star) const {
     for (int64_t j : int_idx) {
       const size_t J = (size_t)j;
-      xt[J] = s.is_fixed(j) ? s.l[J] : jround(jclamp(x_star[J], s.l[J], s.u[J]));
+      xt[J] = s.is_fixed(j) ? s.lo(j) : jround(jclamp(x_star[J], s.lo(j), s.hi(j)));
     }
   }
 
@@ -453,10 +494,10 @@ struct Rounder {
     for (int64_t j : batch) {
       const size_t J = (size_t)j;
       if (s.is_fixed(j)) {
-        xt[J] = s.l[J];
+        xt[J] = s.lo(j);
         continue;
       }
-      const double target = jround(jclamp(x_star[J], s.l[J], s.u[J]));
+      const double target = jround(jclamp(x_star[J], s.lo(j), s.hi(j)));
       s.fix(j, target);
       if (activity) add_activity(*activity, j, target);
       xt[J] = target;
