@@ -198,6 +198,10 @@ def test_batch_one_block_runs_match_until_any():
     assert t0 == t1 == ["OPTIMAL", "OPTIMAL"] and c0 == c1
     assert all(np.array_equal(a, b) for a, b in zip(x0, x1))
     assert n1 == max(c1) // 200 and n0 < n1
+    bt = _batch(pp, 2)
+    assert bt._lib.pdfp_batch_run(bt._ctx, 3, None, None, None) != 0   # no such mode
+    assert bt._lib.pdfp_batch_run(bt._ctx, -1, None, None, None) != 0
+    bt.close()
 
 
 def test_batch_schedules_agree():

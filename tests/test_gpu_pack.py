@@ -194,7 +194,8 @@ def test_pack_small_instances_equal_single_projector_bitwise():
     running, second = [True] * len(pps), [False] * len(pps)
     got = [[] for _ in pps]
     while any(running):
-        terms, _ = pk.run(until_any=True)
+        # one block per call: the pump's form while some instances round
+        terms, _ = pk.run(until_any=True, one_block=True)
         for k in range(len(pps)):
             if not running[k] or terms[k] == "UNSPECIFIED":
                 continue
@@ -209,6 +210,7 @@ def test_pack_small_instances_equal_single_projector_bitwise():
         for a, b in zip(ref[k], got[k]):
             assert a[0] == b[0] and a[1] == b[1], (k, a[:2], b[:2])
             assert np.array_equal(a[2], b[2]), k
+    assert pk._lib.pdfp_pack_run(pk._ctx, 3, None, None, None) != 0   # no such mode
     pk.set_kernel_path("general")
     for k in range(len(pps)):
         pk.set_target(k, t1[k])
