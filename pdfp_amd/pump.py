@@ -472,7 +472,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
         stats.rounding_time += time.perf_counter() - t
         candidate = xt.copy()
         action = "round"
-        if _differs(x_round, xt, int_list):
+        if _differs(x_round, xt, int_idx):
             candidate[:] = x_round
         else:
             flips = int(rng.integers(cfg.flip_parameter // 2, 3 * cfg.flip_parameter // 2 + 1))
