@@ -353,3 +353,36 @@ def test_batch_small_instance_trajectories_equal_single_projector_bitwise():
         assert bt.info(k).small_path == 0
         l1g, l1s = bt.info(k).l1_distance, ref[k][0][1]
         assert abs(l1g - l1s) <= 1e-4 * max(1.0, l1s), (k, l1g, l1s)
+
+
+@pytest.mark.parametrize("path", ["auto", "general"])
+def test_batch_on_instance_with_empty_row_and_column(path):
+    """Two trajectories on an instance with an empty row, an equality row
+    and an empty column with an infinite upper bound, on both kernel paths:
+    each ends as the single FP32 projector does on the same target."""
+    import scipy.sparse as sp
+
+    print("batch edges", path, flush=True)
+    A = sp.csc_matrix(np.array([[1.0, 1.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0],
+                                [0.0, 1.0, 1.0, 0.0]]))
+    m = G.MILP(c=np.zeros(4), c0=0.0, A=A, b=np.array([1.0, 1.0, 1.0]),
+               row_sense=np.array(["le", "le", "eq"]), l=np.zeros(4),
+               u=np.array([1.0, 1.0, 1.0, np.inf]), int_idx=np.array([0, 1, 2]))
+    pp = projector_problem(m)
+    p = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                         iteration_limit=200_000)
+    tg = [np.array([1.0, 1.0, 1.0, 5.0]) / pp.D_c, np.array([0.0, 1.0, 1.0, 0.0]) / pp.D_c]
+    bt = _batch(pp, 2)
+    bt.set_kernel_path(path)
+    for k in range(2):
+        bt.set_target(k, tg[k])
+    assert bt.solve(p, np.ones(2)) == ["OPTIMAL"] * 2
+    for k in range(2):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, tg[k])
+        assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
+        l1b, l1s = bt.info(k).l1_distance, s.info().l1_distance
+        assert abs(l1b - l1s) <= 1e-4 * max(1.0, l1s), (k, l1b, l1s)
+        s.close()
+    bt.close()

@@ -219,3 +219,41 @@ def test_pack_small_instances_equal_single_projector_bitwise():
         assert pk.info(k).small_path == 0
         l1g, l1s = pk.info(k).l1_distance, ref[k][0][1]
         assert abs(l1g - l1s) <= 1e-4 * max(1.0, l1s), (k, l1g, l1s)
+
+
+@pytest.mark.parametrize("path", ["auto", "general"])
+def test_pack_instance_with_empty_row_and_column(path):
+    """An instance with an empty row (0 ≤ 1), an equality row and an empty
+    column with an infinite upper bound, packed beside a set cover, on both
+    kernel paths: each instance ends as the single FP32 projector does on
+    it alone (termination, ℓ1 distance), and the oracle's FP64 residuals of
+    the pack's point meet the tolerance."""
+    import scipy.sparse as sp
+
+    print("pack edges", path, flush=True)
+    A = sp.csc_matrix(np.array([[1.0, 1.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0],
+                                [0.0, 1.0, 1.0, 0.0]]))
+    edge = G.MILP(c=np.zeros(4), c0=0.0, A=A, b=np.array([1.0, 1.0, 1.0]),
+                  row_sense=np.array(["le", "le", "eq"]), l=np.zeros(4),
+                  u=np.array([1.0, 1.0, 1.0, np.inf]), int_idx=np.array([0, 1, 2]))
+    milps = [edge, G.set_cover(133, 201, 3)]
+    pps = [projector_problem(m) for m in milps]
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    targets = [np.array([1.0, 1.0, 1.0, 5.0]) / pps[0].D_c,
+               G.binary_targets(milps[1], 7) / pps[1].D_c]
+    pk = PDFPPack(pps)
+    pk.set_kernel_path(path)
+    for k, t in enumerate(targets):
+        pk.set_target(k, t)
+    assert pk.solve(p, np.ones(2)) == ["OPTIMAL"] * 2
+    for k, pp in enumerate(pps):
+        assert pk.info(k).small_path == (1 if path == "auto" else 0)
+        r = _oracle_kkt(pp, targets[k], pk.pdhg_x(k), pk.vector(k, "pdhg_y"),
+                        pk.vector(k, "pdhg_z"), pk.vector(k, "dual_slack"))
+        assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (k, r)
+        l1s = _single(pp, targets[k], p, 1.0)
+        l1 = pk.info(k).l1_distance
+        assert abs(l1 - l1s) <= 1e-4 * max(1.0, l1s), (k, l1, l1s)
+    pk.close()
