@@ -138,10 +138,9 @@ class Rounder:
             # The draws a₁, b₁, a₂, b₂, … in one call: a Generator's vector
             # of bounded integers is the sequence of its scalar draws.
             ab = (rng.integers(1, n + 1, size=2 * max(1, n // 10)) - 1).tolist()
-            r = ranked.tolist()
+            r = memoryview(ranked)          # swaps in place, one at a time
             for a, b in zip(ab[0::2], ab[1::2]):
                 r[a], r[b] = r[b], r[a]
-            ranked = np.array(r, dtype=ranked.dtype)
         return ranked
 
     def _row_violation(self, i: int, activity: float) -> float:
