@@ -143,3 +143,23 @@ def test_uneven_set_covers_follow_their_recipe():
         nn = int(round(float(np.exp(rng.uniform(np.log(40), np.log(120))))))
         assert m.A.shape == (mm, nn)
         assert np.array_equal(m.A.toarray(), G.set_cover(mm, nn, s).A.toarray())
+
+
+@pytest.mark.parametrize("workload,keys", [
+    ("mixed-1m", ("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>",
+                  "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
+    ("powerlaw-5m", ("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>",
+                     "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
+    ("general-int-4m", ("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>",
+                        "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
+    ("setcover-pack-64", ("EpiPackPrimalY<false>", "EpiPackDualZ<false>")),
+])
+def test_bench_finds_the_traffic_of_each_workload(workload, keys):
+    """bench.py's roofline.traffic comes from the committed counter summary
+    of its workload: every workload name reaches its file and the passes it
+    may report as dominant."""
+    import bench
+
+    for key in keys:
+        traffic = bench.measured_traffic(workload, key)
+        assert traffic is not None and traffic > 0, (workload, key)
