@@ -10,8 +10,9 @@ Pins, per oracle function:
   form of a scaled permutation matrix.
 - estimate_max_singular_value (src/preconditioner.jl:108-125): dense SVD.
 - CSC product (SparseArrays mul!): scipy's CSC product.
-- pdlp_solve (src/pdlp.jl:325-372): HiGHS LP optima (tests/golden/root_lp.json)
-  and p0201's published LP value 6875 (examples/p0201.mps.gz header).
+- pdlp_solve (src/pdlp.jl:325-372): HiGHS LP optima (tests/golden/root_lp.json),
+  p0201's published LP value 6875 (examples/p0201.mps.gz header) and the
+  README's 600 root-LP iterations on p0201 (README.md:39), exactly.
 """
 import os
 import re
@@ -207,13 +208,13 @@ def test_oracle_root_lp_matches_highs(case):
     assert abs(obj - ref) <= 1e-3 * (1.0 + abs(ref))
 
 
-def test_p0201_root_lp_iterations_loose_pin():
+def test_p0201_root_lp_iterations_exact():
     """README.md:38-39: the root LP of p0201 at tolerance 1e-2 ends OPTIMAL
-    after a few hundred iterations (600 on the reference's run)."""
+    after 600 PDHG iterations, which the oracle reproduces exactly."""
     milp = read_mps(example_path("p0201"))
     x, ds, status, iters = O.solve_lp_relaxation(milp, tolerance=1e-2)
     assert status == "OPTIMAL"
-    assert 200 <= iters <= 3000
+    assert iters == 600
     assert abs(float(milp.c @ x) - 6875.0) <= 0.05 * 6875.0
 
 

@@ -293,3 +293,78 @@ def test_ranking_swaps_follow_scalar_draws(n_int, seed):
     assert np.array_equal(got, ranked)
     assert sorted(got.tolist()) == list(range(n_int))
     assert not np.array_equal(got, r.ranking(x, None, 0))
+
+
+# ---- the README's p0201 log (README.md:36-48) ------------------------------------
+
+def _iteration_lines(r):
+    return [line.split() for line in r.log if line[:6].strip().isdigit()]
+
+
+def test_p0201_readme_log_through_iteration_one():
+    """README.md:38-42 on the oracle projector and root LP: the root LP ends
+    OPTIMAL after 600 iterations, the first rounding violates the rows by 9,
+    and pump iteration 1 has ℓ1 distance 8.08613, violation 1 and 600 PDHG
+    iterations, all exactly as printed."""
+    m = read_mps(example_path("p0201"))
+    r = pump(m, time_limit=60.0, batch_policy="always")
+    assert "root LP     OPTIMAL after 600 iterations" in r.log[2]
+    assert "(first rounding violates by 9)" in r.log[3]
+    it1 = _iteration_lines(r)[0]
+    assert it1[:5] == ["1", "8.08613", "1", "600", "1.00e-02"] and it1[5] == "round"
+    assert r.found and is_feasible(m, r.x)
+
+
+def _perturbed_module(start_rel, frac_rel, seed):
+    """The oracle projector with the first projection's start point scaled by
+    (1 + start_rel·U[-1,1]) and its result's fractional coordinates (those
+    not exactly integral in original units) by (1 + frac_rel·U[-1,1])."""
+    from types import SimpleNamespace
+
+    from oracle import oracle as O
+    from pdfp_amd.pdfp import projector_problem
+
+    m = read_mps(example_path("p0201"))
+    D_c = projector_problem(m, scaling="host").D_c
+    rng = np.random.default_rng(seed)
+    calls = {"n": 0}
+
+    class State(O.OraclePDFPState):
+        def set_x(self, x):
+            x = np.asarray(x, dtype=np.float64)
+            super().set_x(x * (1.0 + start_rel * rng.uniform(-1.0, 1.0, x.size)))
+
+        @property
+        def pdhg_x(self):
+            v = self.vector("pdhg_x")
+            calls["n"] += 1
+            if calls["n"] == 1:
+                x = D_c * v
+                frac = np.abs(x - np.rint(x)) != 0.0
+                v = np.where(frac, v * (1.0 + frac_rel * rng.uniform(-1.0, 1.0, v.size)), v)
+            return v
+
+    return SimpleNamespace(PDFPState=State, set_target=O.set_target, warm_start=O.warm_start,
+                           pdfp_solve=O.pdfp_solve,
+                           original_primal_residual=O.original_primal_residual)
+
+
+@pytest.mark.parametrize("start_rel,frac_rel", [(1e-6, 0.0), (0.0, 1e-4), (1e-9, 1e-3)])
+def test_p0201_second_target_is_not_an_arithmetic_knife_edge(start_rel, frac_rel):
+    """The README's iteration 2 (1.23237 / 5 / 400) differs from the
+    restatement's (1.15587 / 1 / 600).  The second target is the rounding of
+    projection 1, so this pins what does NOT explain the difference:
+    perturbations far larger than summation-order or FMA effects — the
+    projection's start point by up to 1e-6 relative, the fractional
+    coordinates of projection 1 (whose ranking keys lie 4e-6 apart) by up to
+    1e-3 — leave iteration 1 and iteration 2 unchanged."""
+    m = read_mps(example_path("p0201"))
+    for seed in range(3):
+        r = feasibility_pump(m, projector=_perturbed_module(start_rel, frac_rel, seed),
+                             root_lp=oracle_root_lp, scaling="host", batch_policy="always",
+                             time_limit=60.0)
+        lines = _iteration_lines(r)
+        # the printed distance of iteration 1 is measured on the perturbed point
+        assert lines[0][2:4] == ["1", "600"]
+        assert lines[0][1] == "8.08613" or frac_rel > 0
+        assert lines[1][:4] == ["2", "1.15587", "1", "600"]
