@@ -5,9 +5,10 @@ Pins, per oracle function:
 - projector solve (src/pdfp.jl:252-288): the reference's closed-form toys
   (test/runtests.jl:40-55) and HiGHS ℓ1 optima of 15 instances
   (tests/golden/l1_projections.json, made by tests/golden/make_fixtures.py).
-- scale_problem (src/preconditioner.jl:26-99): bit-identical to the
-  independent numpy implementation; A_s = diag(D_r) A diag(D_c); the closed
-  form of a scaled permutation matrix.
+- scale_problem (src/preconditioner.jl:26-99): bit-identical to the numpy
+  restatement; A_s = diag(D_r) A diag(D_c); closed forms on a scaled
+  permutation matrix and on {0, ±1} matrices (Ruiz is the identity there and
+  Pock–Chambolle divides by the square roots of the row and column counts).
 - estimate_max_singular_value (src/preconditioner.jl:108-125): dense SVD.
 - CSC product (SparseArrays mul!): scipy's CSC product.
 - pdlp_solve (src/pdlp.jl:325-372): HiGHS LP optima (tests/golden/root_lp.json),
@@ -166,6 +167,34 @@ def test_scaling_permutation_closed_form():
                       bound_objective_rescaling=False, backend="host")
     np.testing.assert_allclose(np.abs(a.A.data), 1.0, rtol=1e-15)
     np.testing.assert_allclose(a.D_r * a.D_c[perm], 1.0 / np.abs(vals), rtol=1e-15)
+
+
+@pytest.mark.parametrize("signs", [False, True])
+def test_scaling_unit_entries_closed_form(signs):
+    """Entries in {0, ±1}: every row and column ∞-norm is 1, so Ruiz leaves
+    the matrix alone, and Pock–Chambolle with α = 1 divides row i by
+    √(nnz in row i) and column j by √(nnz in column j); rows and columns
+    without nonzeros keep scale 1 (SCALING_EPSILON, src/preconditioner.jl:38-60)."""
+    m = G.set_cover(133, 201, 4)
+    A = m.A.tocsc().astype(np.float64)
+    if signs:
+        A.data = np.where(np.random.default_rng(1).random(A.nnz) < 0.5, -1.0, 1.0)
+    # one empty row and one empty column
+    A = sp.vstack([A, sp.csr_matrix((1, A.shape[1]))]).tocsc()
+    A = sp.hstack([A, sp.csc_matrix((A.shape[0], 1))]).tocsc()
+    mm, nn = A.shape
+    b = O.scale_problem(A, np.zeros(nn), np.zeros(nn), np.ones(nn), np.full(mm, -np.inf),
+                        np.ones(mm), bound_objective_rescaling=False)
+    rc = np.diff(A.tocsr().indptr).astype(float)
+    cc = np.diff(A.indptr).astype(float)
+    D_r = np.where(rc > 0, 1.0 / np.sqrt(np.maximum(rc, 1.0)), 1.0)
+    D_c = np.where(cc > 0, 1.0 / np.sqrt(np.maximum(cc, 1.0)), 1.0)
+    np.testing.assert_allclose(b["D_r"], D_r, rtol=1e-15)
+    np.testing.assert_allclose(b["D_c"], D_c, rtol=1e-15)
+    assert b["D_r"][-1] == 1.0 and b["D_c"][-1] == 1.0
+    ref = (sp.diags(D_r) @ A @ sp.diags(D_c)).tocsc()
+    ref.sort_indices()
+    np.testing.assert_allclose(b["A"].toarray(), ref.toarray(), rtol=1e-15, atol=0)
 
 
 # ---- power iteration, SpMV ------------------------------------------------------------
