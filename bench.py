@@ -1,13 +1,16 @@
 """Benchmark of the PDHG projector on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mixed-1m|powerlaw-5m|setcover-pack-64|general-int-4m]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw-5m|mixed-1m|setcover-pack-64|general-int-4m]
 
-Workload (default, "mixed-1m"): the mixed MILP with 50,000 rows × 100,000 columns
-(70 % binary, 30 % continuous in [0, 10]), 1,000,000 nonzeros, row lengths
-uniform in 10-30, values N(0,1), b = A x0 + |N(0,1)| for a random feasible
-x0 (seed 1).  ``--workload powerlaw-5m``: 500,000 × 1,000,000 with exactly 5,000,000
-nonzeros in power-law rows; ``general-int-4m``: a general-integer, equality-heavy MILP of
-200,000 × 400,000 with 4M nonzeros; ``setcover-pack-64``: 64 set-covering instances of
+Workload (default, "powerlaw-5m"): the mixed MILP with 500,000 rows × 1,000,000
+columns (70 % binary, 30 % continuous in [0, 10]) and exactly 5,000,000
+nonzeros in power-law row lengths (Zipf 1.5, longest rows 5,000), values
+N(0,1), b = A x0 + |N(0,1)| for a random feasible x0 (seed 1): the FP32-bulk /
+FP64-residual configuration on which the targets (batched FP32 at 3x the plain
+FP64 form, the SpMV against the HBM roofline) are stated.  ``--workload
+mixed-1m``: the same generator at 50,000 × 100,000 with 1M nonzeros in rows of
+10-30; ``general-int-4m``: a general-integer, equality-heavy MILP of 200,000 ×
+400,000 with 4M nonzeros; ``setcover-pack-64``: 64 set-covering instances of
 uneven sizes in one block-diagonal pack (15.5M nonzeros in all).  A step is one warm-started ℓ1
 projection toward a new random integer target (0/1 on binaries) at relative
 KKT tolerance 1e-4, the way the pump calls the projector
@@ -90,6 +93,25 @@ def kernel_bytes_fp64(m: int, n: int, nnz: int) -> tuple[float, float]:
     at = 12 * nnz + 4 * (n + 1) + 8 * m + 8 * n
     a = 12 * nnz + 4 * (m + 1) + 8 * n + 8 * m
     return float(at), float(a)
+
+
+def spmv_figures(m: int, n: int, nnz: int, spmv_ms):
+    """The standalone FP32 SpMVs y = A x and y = Aᵀ x on the resident CSR
+    operators (FP64 accumulation, FP32 output): algorithmic bytes
+    8·nnz + 4(rows+1) + 4·cols + 4·rows over the mean launch duration, beside
+    the random-gather floor of the same shape (nnz 4-byte gathers from a table
+    of the gathered vector's size)."""
+    from pdfp_amd._lib import gather_ceiling_us
+
+    out = {}
+    for name, ms, rows, cols in (("Ax", spmv_ms[0], m, n), ("ATz", spmv_ms[1], n, m)):
+        byt = 8.0 * nnz + 4.0 * (rows + 1) + 4.0 * cols + 4.0 * rows
+        gbs = byt / (ms * 1e-3) / 1e9
+        ceil = gather_ceiling_us(max(cols, 1) * 4, 4, nnz)
+        out[name] = {"us": ms * 1e3, "algorithmic_bytes": byt, "achieved_gbs": gbs,
+                     "frac": gbs / HBM_PEAK_GBS, "roofline_us": byt / HBM_PEAK_GBS / 1e3,
+                     "gather_ceiling_us": ceil, "frac_of_gather_ceiling": ceil / (ms * 1e3)}
+    return out
 
 
 def run_projections(s, pp, milp, targets, tol):
@@ -203,7 +225,7 @@ def _oracle_pump_backends():
     return proj, root_lp
 
 
-def time_to_first_feasible(runs: int, cpu: bool):
+def time_to_first_feasible(runs: int, cpu: bool, large: bool = True):
     """The full pump (root LP, projections, rounding) on the 133 × 201 binary
     set-covering instance (seed 1), on p0201 and on 2000 × 4000 and
     10,000 × 20,000 mixed MILPs (seed 1; 40k and 200k nonzeros): median PumpResult.time over `runs` runs after one warm-up run,
@@ -249,6 +271,21 @@ def time_to_first_feasible(runs: int, cpu: bool):
                                  "kind": "port"}
         out[name] = rec
     out["p0201"]["reference_h100_seconds"] = 0.12   # README.md:47-48
+    if large:
+        # The 1M-nonzero mixed MILP of the mixed-1m workload (the pump's host
+        # side dominates here): one run per form; the oracle pump would take
+        # many minutes on one core and is left out.
+        milp = G.mixed(50_000, 100_000, 1)
+        rec = {}
+        for form in ("fp32", "fp32_batch8"):
+            r0 = forms[form](milp)
+            rec[form] = {"seconds": r0.time, "found": bool(r0.found and is_feasible(milp, r0.x)),
+                         "iterations": r0.iterations, "pdhg_iterations": r0.pdhg_iterations,
+                         "root_lp_s": r0.root_lp_time, "projection_s": r0.projection_time,
+                         "rounding_s": r0.rounding_time, "runs": 1}
+            if form == "fp32_batch8":
+                rec[form]["winning_trajectory"] = r0.trajectory
+        out["mixed_50000x100000_seed1"] = rec
     # The four example instances at once: their pumps on one block-diagonal
     # pack, against the same pumps run one after another (single FP32).
     from pdfp_amd.pump import feasibility_pump_pack
@@ -438,17 +475,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed-1m")
+    ap.add_argument("--workload", default="powerlaw-5m")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-iterations", type=int, default=1000)
+    ap.add_argument("--cpu-iterations", type=int, default=None,
+                    help="PDHG iterations of the CPU baseline (default: 400 above 2M "
+                         "nonzeros, else 1000: about 10 s on one core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttff", action="store_true")
+    ap.add_argument("--no-ttff-large", action="store_true",
+                    help="skip the time to first feasible of the 1M-nonzero mixed MILP")
     ap.add_argument("--batch", type=int, default=8,
                     help="trajectories per step of the headline measurement (1: single)")
     ap.add_argument("--batch-schedule", default=None,
                     help="npb_a,npb_at,persist,lane_rows_a,lane_rows_at: the batched passes' "
                          "schedule instead of the autotuner's (to profile a given form)")
+    ap.add_argument("--schedule", default=None,
+                    help="npb_a,npb_at,persist_a,persist_at: the single-trajectory FP32 "
+                         "passes' schedule instead of the autotuner's (to profile a given form)")
     ap.add_argument("--stream-launches", action="store_true",
                     help="launch every block's passes on the stream instead of replaying "
                          "captured block graphs (same kernels; for profilers that do not "
@@ -498,6 +542,8 @@ def main():
                         precision=prec)
         if args.stream_launches:
             s.set_block_graph(False)
+        if args.schedule and prec == "fp32":
+            s.set_schedule(*[int(v) for v in args.schedule.split(",")])
         s.set_x(np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         t0 = time.perf_counter()
         run_projections(s, pp, milp, targets_w, args.tol)  # warmup (includes ‖A‖₂)
@@ -508,8 +554,10 @@ def main():
         ms_at, ms_a = s.time_kernels(200)
         ms_iter = s.time_iterations(400)
         info = s.info()
+        spmv_ms = ((s.time_spmv(100), s.time_spmv(100, transpose=True)) if prec == "fp32"
+                   else None)
         results[prec] = dict(iters=it, seconds=dt, statuses=st, ms_at=ms_at, ms_a=ms_a,
-                             ms_iter=ms_iter, info=info, quality=quality,
+                             ms_iter=ms_iter, info=info, quality=quality, spmv_ms=spmv_ms,
                              npb={"A": info.npb_a, "At": info.npb_at,
                                   "A_persistent_wg_per_cu": info.persist_a,
                                   "At_persistent_wg_per_cu": info.persist_at})
@@ -614,6 +662,7 @@ def main():
                                   "row_pass": r32["ms_a"] * 1e3},
         "schedule_nnz_per_block": r32["npb"],
         "scaling_seconds": scaling_s,
+        "spmv": spmv_figures(mm, nn, nnz, r32["spmv_ms"]),
     }
     if K > 1:
         out["batch_schedule_nnz_per_block"] = head["npb"]
@@ -635,10 +684,12 @@ def main():
             for a, b in zip(r32["quality"], r64["quality"])]
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(pp, milp, targets[0], args.cpu_iterations)
+        cpu_it = args.cpu_iterations or (400 if nnz > 2_000_000 else 1000)
+        out["cpu_baseline"] = cpu_baseline(pp, milp, targets[0], cpu_it)
     if not args.no_ttff:
         print("[bench] time to first feasible", file=sys.stderr, flush=True)
-        out["time_to_first_feasible"] = time_to_first_feasible(3, cpu=not args.no_cpu)
+        out["time_to_first_feasible"] = time_to_first_feasible(3, cpu=not args.no_cpu,
+                                                               large=not args.no_ttff_large)
     print(json.dumps(out), flush=True)
 
 

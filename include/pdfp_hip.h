@@ -405,6 +405,13 @@ int pdfp_time_iterations(pdfp_ctx* ctx, int32_t iterations, double* ms_out);
 int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at,
                       double* ms_a);
 
+/* No counterpart in the reference (measurement): mean duration in ms of a
+ * plain SpMV y = A x (transpose = 1: y = Aᵀ x) over the context's resident
+ * CSR operator, in its precision, each launch timed by its own events — the
+ * sparse product that cuSPARSE performs at src/pdfp_kernels.jl:26 and 81,
+ * without the fused epilogue. */
+int pdfp_time_spmv(pdfp_ctx* ctx, int32_t transpose, int32_t iterations, double* ms_out);
+
 /* Device memory held by the context, in bytes. */
 int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
 

@@ -120,6 +120,7 @@ SIGNATURES = {
     "pdfp_spmv": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_time_iterations": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "pdfp_time_spmv": (C.c_int, [_P, C.c_int32, C.c_int32, _D]),
     "pdfp_device_bytes": (C.c_int64, [_P]),
     "pdfp_set_kernel_path": (C.c_int, [_P, C.c_int32]),
     "pdfp_set_block_graph": (C.c_int, [_P, C.c_int32]),

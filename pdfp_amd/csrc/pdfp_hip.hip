@@ -43,6 +43,7 @@ struct ProjectorBase {
   virtual void spmv(int transpose, const double* x, double* y) = 0;
   virtual double time_iterations(int iters) = 0;
   virtual void time_kernels(int iters, double* ms_at, double* ms_a) = 0;
+  virtual double time_spmv(int transpose, int iters) = 0;
   virtual int64_t device_bytes() const = 0;
   virtual void set_path(int path) = 0;
   virtual void set_block_graph(int on) = 0;
@@ -438,18 +439,20 @@ struct Projector final : ProjectorBase {
 
   // One block of the general passes: iteration 1 (major, and evaluated into
   // slot 1 after a restart), iterations 2 .. freq (freq major), the
-  // evaluation into slot 0, and the readback of both slots.
-  void general_block(int freq, bool restart_eval) {
+  // evaluation into slot 0, and (readback) the copy of both slots to the host.
+  void general_block(int freq, bool restart_eval, bool readback = true) {
     step(1, restart_eval || freq == 1);
     if (restart_eval) evaluate(1);
     for (int i = 2; i <= freq; ++i) step(i, i == freq);
     evaluate(0);
-    read_sums();
+    if (readback) read_sums();
   }
 
   // general_block as a captured graph.  Its launches read the block's
   // scalars from d_sp, which the host writes before each replay, so one
-  // capture serves every block of its shape.  The block has no memset node.
+  // capture serves every block of its shape.  The graph holds kernel nodes
+  // only (no memset, no copy): the readback follows each replay on the
+  // stream.
   void block_graph(int freq, bool restart_eval) {
     BlockGraph& g = graphs[restart_eval ? 1 : 0];
     if (!g.exec || g.freq != freq) {
@@ -458,7 +461,7 @@ struct Projector final : ProjectorBase {
       hipGraph_t graph = nullptr;
       PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
-        general_block(freq, restart_eval);
+        general_block(freq, restart_eval, false);
       } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -471,6 +474,7 @@ struct Projector final : ProjectorBase {
       g.freq = freq;
     }
     PDFP_HIP(hipGraphLaunch(g.exec, stream));
+    read_sums();
   }
 
   void set_path(int p) override {
@@ -780,6 +784,35 @@ struct Projector final : ProjectorBase {
     *ms_at = sa / std::max(iters, 1);
     *ms_a = sb / std::max(iters, 1);
   }
+
+  // Mean duration of a plain SpMV y = A x (transpose: y = Aᵀ x) over the
+  // resident operator with its schedule, in the context's precision (FP64
+  // accumulation, y stored in T), each launch bracketed by its own events.
+  double time_spmv(int transpose, int iters) override {
+    const DevCsr<T>& M = transpose ? At : A;
+    const int64_t nin = transpose ? m : n, nout = transpose ? n : m;
+    DevBuf<T> xd, yd;
+    xd.alloc((size_t)std::max<int64_t>(nin, 1));
+    yd.alloc((size_t)std::max<int64_t>(nout, 1));
+    std::vector<T> ones((size_t)std::max<int64_t>(nin, 1), (T)1);
+    xd.upload(ones.data(), ones.size(), stream);
+    for (int w = 0; w < 3; ++w)
+      launch_pass<0>(M, Gather1<T>{xd.p}, EpiStore<T>{yd.p}, nullptr, stream);
+    std::vector<hipEvent_t> ev((size_t)iters * 2);
+    for (auto& e : ev) PDFP_HIP(hipEventCreate(&e));
+    for (int i = 0; i < iters; ++i)
+      launch_pass<0>(M, Gather1<T>{xd.p}, EpiStore<T>{yd.p}, nullptr, stream,
+                     ev[(size_t)i * 2], ev[(size_t)i * 2 + 1]);
+    sync();
+    double sum = 0.0;
+    for (int i = 0; i < iters; ++i) {
+      float t = 0.f;
+      PDFP_HIP(hipEventElapsedTime(&t, ev[(size_t)i * 2], ev[(size_t)i * 2 + 1]));
+      sum += t;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return sum / std::max(iters, 1);
+  }
 };
 
 }  // namespace pdfp
@@ -976,6 +1009,13 @@ int pdfp_time_kernels(pdfp_ctx* ctx, int32_t iterations, double* ms_at, double* 
   return guarded(ctx, [&] {
     if (!ms_at || !ms_a || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
     ctx->impl->time_kernels(iterations, ms_at, ms_a);
+  });
+}
+
+int pdfp_time_spmv(pdfp_ctx* ctx, int32_t transpose, int32_t iterations, double* ms_out) {
+  return guarded(ctx, [&] {
+    if (!ms_out || iterations < 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "bad arguments");
+    *ms_out = ctx->impl->time_spmv(transpose, iterations);
   });
 }
 

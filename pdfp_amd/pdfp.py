@@ -182,6 +182,13 @@ class PDFPState:
         self._check(self._lib.pdfp_time_kernels(self._ctx, iterations, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def time_spmv(self, iterations: int, transpose: bool = False) -> float:
+        """Mean ms of a plain SpMV y = A x (or Aᵀ x) over the resident operator."""
+        ms = C.c_double()
+        self._check(self._lib.pdfp_time_spmv(self._ctx, int(transpose), iterations,
+                                             C.byref(ms)))
+        return ms.value
+
     @property
     def device_bytes(self) -> int:
         return int(self._lib.pdfp_device_bytes(self._ctx))
