@@ -537,17 +537,22 @@ struct Batch {
   // One block: iteration 1 (major after a restart; the restarted
   // trajectories evaluated into slot 1), iterations 2 .. freq (freq major),
   // the evaluation of the running trajectories into slot 0, the readback.
-  void general_block(int freq, bool any_restart) {
+  void general_block(int freq, bool any_restart, bool readback = true) {
     step(1, any_restart || freq == 1);
     if (any_restart) evaluate_all(d_mask.p + 1, 1);
     for (int i = 2; i <= freq; ++i) step(i, i == freq);
     evaluate_all(d_mask.p, 0);
+    if (readback) read_block_sums();
+  }
+
+  void read_block_sums() {
     PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
                             hipMemcpyDeviceToHost, stream));
   }
 
   // general_block as a captured graph, captured on first use of its shape
-  // and replayed after; it holds no memset node.
+  // and replayed after; it holds kernel nodes only (no memset, no copy): the
+  // readback follows each replay on the stream.
   void block_graph(int freq, bool any_restart) {
     BlockGraph& g = graphs[any_restart ? 1 : 0];
     if (!g.exec || g.freq != freq) {
@@ -556,7 +561,7 @@ struct Batch {
       hipGraph_t graph = nullptr;
       PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
-        general_block(freq, any_restart);
+        general_block(freq, any_restart, false);
       } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -569,6 +574,7 @@ struct Batch {
       g.freq = freq;
     }
     PDFP_HIP(hipGraphLaunch(g.exec, stream));
+    read_block_sums();
   }
 
   // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for

@@ -565,17 +565,22 @@ struct Pack {
   // One block: iteration 1 (major after a restart, and every instance
   // evaluated into slot 1), iterations 2 .. freq (freq major), the
   // evaluation into slot 0, the readback of the sums.
-  void general_block(int freq, bool any_restart) {
+  void general_block(int freq, bool any_restart, bool readback = true) {
     step(1, any_restart || freq == 1);
     if (any_restart) evaluate(1);
     for (int i = 2; i <= freq; ++i) step(i, i == freq);
     evaluate(0);
+    if (readback) read_block_sums();
+  }
+
+  void read_block_sums() {
     PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)K * 32,
                             hipMemcpyDeviceToHost, stream));
   }
 
   // general_block as a captured graph, captured on first use of its shape
-  // and replayed after; it holds no memset node.
+  // and replayed after; it holds kernel nodes only (no memset, no copy): the
+  // readback follows each replay on the stream.
   void block_graph(int freq, bool any_restart) {
     BlockGraph& g = graphs[any_restart ? 1 : 0];
     if (!g.exec || g.freq != freq) {
@@ -584,7 +589,7 @@ struct Pack {
       hipGraph_t graph = nullptr;
       PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
-        general_block(freq, any_restart);
+        general_block(freq, any_restart, false);
       } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -597,6 +602,7 @@ struct Pack {
       g.freq = freq;
     }
     PDFP_HIP(hipGraphLaunch(g.exec, stream));
+    read_block_sums();
   }
 
   // The block loop of pdfp_solve! (src/pdfp.jl:252-288) for every running

@@ -357,19 +357,20 @@ struct LpSolver final : LpBase {
   }
 
   // One block of the general passes: iterations 1 and freq major, the
-  // restart evaluation after iteration 1, the block's evaluation, the
-  // readback of the sums.
-  void general_block(int freq, bool restart_eval) {
+  // restart evaluation after iteration 1, the block's evaluation and
+  // (readback) the copy of the sums to the host.
+  void general_block(int freq, bool restart_eval, bool readback = true) {
     step(1, true);
     if (restart_eval) evaluate(1);
     for (int i = 2; i <= freq - 1; ++i) step(i, false);
     step(freq, true);
     evaluate(0);
-    read_sums();
+    if (readback) read_sums();
   }
 
   // general_block as a captured graph, captured on first use of its shape
-  // and replayed after; it holds no memset node.
+  // and replayed after; it holds kernel nodes only (no memset, no copy):
+  // the readback follows each replay on the stream.
   void block_graph(int freq, bool restart_eval) {
     BlockGraph& g = graphs[restart_eval ? 1 : 0];
     if (!g.exec || g.freq != freq) {
@@ -378,7 +379,7 @@ struct LpSolver final : LpBase {
       hipGraph_t graph = nullptr;
       PDFP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
-        general_block(freq, restart_eval);
+        general_block(freq, restart_eval, false);
       } catch (...) {
         (void)hipStreamEndCapture(stream, &graph);
         if (graph) (void)hipGraphDestroy(graph);
@@ -391,6 +392,7 @@ struct LpSolver final : LpBase {
       g.freq = freq;
     }
     PDFP_HIP(hipGraphLaunch(g.exec, stream));
+    read_sums();
   }
 
   // pdlp_solve!  src/pdlp.jl:325-372
