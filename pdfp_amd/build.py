@@ -36,17 +36,45 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
+    """Compiles every source to an object in parallel (one hipcc per file,
+    under pdfp_amd/_build/), then links the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = out + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-I",
-           os.path.join(ROOT, "include"), *srcs, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
-    os.replace(tmp, out)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    common = [*flags, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include")]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.basename(src) + (".%d.o" % os.getpid()))
+        cmd = [hipcc(), *common, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1, 8)) as ex:
+        results = list(ex.map(compile_one, srcs))
+    failed = [(o, r) for o, r in results if r.returncode != 0]
+    objs = [o for o, _ in results]
+    try:
+        if failed:
+            for _, r in failed:
+                sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {len(failed)} source(s)")
+        tmp = out + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
+        os.replace(tmp, out)
+    finally:
+        for o in objs:
+            if os.path.exists(o):
+                os.remove(o)
     return out
 
 

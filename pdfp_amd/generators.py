@@ -14,7 +14,7 @@ seed ("seed s" means this generator seeded with s).  The families:
 - ``general_integer``   integer columns in [0, U_j], U_j uniform in 1-20,
                         integer coefficients in [-9, 9]∖{0}, 40 % equality rows
                         A_eq x = A_eq x0, the rest ≤ with |N(0,1)| slack.
-- ``batched_set_cover`` K set-cover instances of log-uniform sizes.
+- ``uneven_set_covers`` K set-cover instances of log-uniform sizes.
 - ``binary_targets``    the pump-like targets x̃: uniform 0/1 on the binary
                         columns (rounded uniform integers on general ones).
 """
@@ -175,20 +175,6 @@ def general_integer(m: int = 200_000, n: int = 400_000, seed: int = 1, *,
     b = np.where(is_eq, Ax0, Ax0 + np.abs(rng.standard_normal(m)))
     return MILP(c=np.zeros(n), c0=0.0, A=A, b=b, row_sense=np.where(is_eq, "eq", "le"),
                 l=np.zeros(n), u=U, int_idx=np.arange(n), name=f"genint_{m}x{n}_s{seed}")
-
-
-def batched_set_cover(K: int = 64, *, m_range=(2_000, 40_000), n_range=(4_000, 80_000),
-                      seed0: int = 1) -> list[MILP]:
-    """K set-cover instances, seeds seed0..seed0+K-1, with m and n drawn
-    log-uniform (independently) from the given ranges."""
-    out = []
-    for k in range(K):
-        seed = seed0 + k
-        rng = _rng(10_000 + seed)
-        m = int(round(np.exp(rng.uniform(np.log(m_range[0]), np.log(m_range[1])))))
-        n = int(round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))))
-        out.append(set_cover(m, n, seed))
-    return out
 
 
 def binary_targets(milp: MILP, seed: int) -> np.ndarray:

@@ -89,8 +89,8 @@ def _is_infinite(v: float) -> bool:
 
 
 def max_violation(milp: MILP, x) -> float:
-    """Largest violation of A x ≤ b / A x = b at x  (src/milp.jl:102-114).
-    Bounds and integrality are not checked."""
+    """Largest violation of A x ≤ b / A x = b at x  (src/milp.jl:102-114);
+    only the rows count, not the column bounds or integrality."""
     x = np.asarray(x, dtype=np.float64)
     Ax = _csc_matvec(milp.A, x)
     d = Ax - milp.b

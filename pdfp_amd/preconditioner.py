@@ -2,12 +2,10 @@
 
 ``scale_problem`` restates src/preconditioner.jl:26-99: 10 Ruiz rounds, one
 Pock–Chambolle pass with α = 1 and, optionally, the bound/objective
-rescaling.  With D_r, D_c the row and column scalings the scaled problem is
-
-    min (D_c c)' x̂   s.t.  D_r con_lb ≤ (D_r A D_c) x̂ ≤ D_r con_ub,
-                           var_lb ./ D_c ≤ x̂ ≤ var_ub ./ D_c,
-
-and x = D_c .* x̂.  The arithmetic is max, sqrt, divide and multiply, in the
+rescaling.  The scalings D_r (rows) and D_c (columns) change variables to
+x = D_c ∘ x̂: the objective becomes (D_c ∘ c)ᵀ x̂, the rows read
+D_r ∘ con_lb ≤ diag(D_r) A diag(D_c) x̂ ≤ D_r ∘ con_ub and the column bounds
+become var_lb / D_c ≤ x̂ ≤ var_ub / D_c.  The arithmetic is max, sqrt, divide and multiply, in the
 reference's order, so the result is bit-identical to the reference's CPU
 code; the Pock–Chambolle sums accumulate in the CSC's column-major order.
 

@@ -1,12 +1,14 @@
-"""Bound propagation over the linear constraints: the engine behind the
-constraint-propagation rounding (host, FP64, the reference's arithmetic).
+"""Domain propagation on the rows of A, as the rounders use it (host, FP64,
+src/propagation.jl's arithmetic).
 
-Restates src/propagation.jl: each row keeps its minimum and maximum activity
-under the current domains; a domain change updates them incrementally and
-queues (FIFO) the rows that may tighten further bounds.  Cardinality rows
-(all coefficients 1 over binaries) and knapsack rows (≤, positive
-coefficients over binaries) have specialised propagators that give the same
-tightenings.  Plain Python lists: the loops touch one row or column at a time.
+For every row the state tracks the smallest and largest value its left-hand
+side can take over the current variable domains.  Tightening a domain
+adjusts those two numbers in place and appends the affected rows, first in
+first out, to the queue of rows to re-examine.  Rows whose coefficients are
+all 1 on binary columns, and ≤ rows with positive coefficients on binary
+columns, are examined by shortcut routines that derive exactly the bounds
+the general routine would.  The data are plain Python lists because every
+loop visits a single row or column.
 """
 from __future__ import annotations
 
@@ -20,8 +22,8 @@ from .milp import MILP
 
 BOUND_TOLERANCE = 1e-6
 INFEASIBILITY_TOLERANCE = 1e-6
-MAX_TIGHTENINGS = 10       # wake-up cap per variable and direction
-DOMAIN_INFINITY = 1e12     # stands in for infinite bounds in activity arithmetic
+MAX_TIGHTENINGS = 10       # after this many tightenings of one bound, only a fixing re-queues its rows
+DOMAIN_INFINITY = 1e12     # finite proxy for ±inf bounds inside activity sums
 
 ROW_GENERAL = 0
 ROW_CARDINALITY = 1
@@ -29,7 +31,8 @@ ROW_KNAPSACK = 2
 
 
 class PropagationData:
-    """Everything about the propagation that depends on the model only
+    """The model-derived, call-invariant half of the propagation: row and
+    column nonzero lists, row kinds and the starting domains
     (src/propagation.jl:19-84)."""
 
     def __init__(self, milp: MILP):
@@ -107,7 +110,7 @@ class PropagationData:
 
 
 class PropagationState:
-    """Domains of one rounding pass and the propagation queue
+    """Per-call state: the current lower/upper bounds, the row activity range and the FIFO of rows to revisit
     (src/propagation.jl:87-107)."""
 
     def __init__(self, d: PropagationData, max_pops: int):
@@ -254,7 +257,7 @@ class PropagationState:
                     self._tighten_upper(j, l[j] - slack_dn / a)
 
     def propagate(self) -> int:
-        """Propagate the queued rows until the queue is empty or the budget is
+        """Revisit queued rows, first in first out, until none is left or the pop budget is
         spent (src/propagation.jl:235-244)."""
         pops = 0
         q = self.queue

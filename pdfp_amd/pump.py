@@ -80,8 +80,9 @@ def _distance(x, xt, idx) -> float:
 
 
 def round_to_nearest(x, milp: MILP) -> np.ndarray:
-    """Nearest integer on I, kept inside the bounds (src/pump.jl:53-62);
-    element-wise, so vectorised exactly."""
+    """x with every integer coordinate replaced by its nearest integer,
+    clipped to the column's bounds (src/pump.jl:53-62); each coordinate is
+    independent, so the numpy form is exact."""
     xt = np.array(x, dtype=np.float64)
     I = milp.int_idx
     v = np.rint(xt[I])
@@ -116,8 +117,8 @@ def batch_schedule(milp: MILP, x, dual_slack, push_threshold: float) -> str:
 
 
 def total_integer_row_violation(milp: MILP, x) -> float:
-    """Total violation of the rows without continuous variables
-    (src/pump.jl:98-118)."""
+    """Sum, over rows that touch no continuous column, of how far x breaks
+    the row (src/pump.jl:98-118)."""
     A = milp.A
     m = num_constraints(milp)
     cols = np.repeat(np.arange(A.shape[1]), np.diff(A.indptr))
@@ -134,8 +135,9 @@ def total_integer_row_violation(milp: MILP, x) -> float:
 
 
 def flip_farthest(xt: np.ndarray, x, milp: MILP, TT: int) -> None:
-    """FP's stalling move: flip the TT integer variables farthest from x
-    (src/pump.jl:121-135)."""
+    """When rounding returns the previous target, move the TT integer
+    coordinates of x̃ that lie farthest from x one unit towards x, inside the
+    bounds (src/pump.jl:121-135)."""
     l, u = milp.l, milp.u
     I = milp.int_idx
     dist = np.abs(np.asarray(x)[I] - xt[I])
@@ -214,8 +216,8 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                      propagation_budget: int = 100_000, feasibility_tolerance: float = 1e-6,
                      precision: str = "fp32", device: int = 0, scaling: str = "device",
                      projector=None, root_lp=None) -> PumpResult:
-    """Search for a feasible solution of ``milp`` with the feasibility pump
-    (src/pump.jl:231-410).  The keyword arguments and defaults are the
+    """Run the feasibility pump on ``milp`` until a feasible point turns up or
+    the time limit passes (src/pump.jl:231-410).  The keyword arguments and defaults are the
     reference's (README.md:89-108); ``precision`` picks the projector's form
     ("fp32": FP32 data and iterates with FP64 reductions and polish; "fp64");
     ``scaling`` where the projector's matrix is equilibrated ("device" or
@@ -331,6 +333,10 @@ class _Config:
     params_cls: type
     t0: float
     violation: object = None   # x -> max_violation(milp, x)
+    stop: object = None        # a threading.Event: set, the trajectories end at their next step
+
+    def stopped(self) -> bool:
+        return self.stop is not None and self.stop.is_set()
 
     def elapsed(self) -> float:
         return time.perf_counter() - self.t0
@@ -397,7 +403,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
                       policy=BatchPolicy() if cfg.batch_policy == "cost" else None)
     params_cls = cfg.params_cls
 
-    # First rounding, then the decision on the violation-guided rule.
+    # Round the root LP point once; a large row violation of that rounding switches on the guided rule.
     rounder.schedule = batch_schedule(milp, x, dual_slack, cfg.push_threshold)
     t = time.perf_counter()
     xt = rounder.round_guarded(x, rng, 0)
@@ -424,7 +430,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
 
     emit(f"{'iter':>6s} {'distance':>12s} {'violation':>10s} {'pdhg':>10s} {'gap tol':>9s} "
          f"{'action':>8s} {'time':>9s}\n")
-    while cfg.elapsed() < cfg.time_limit:
+    while cfg.elapsed() < cfg.time_limit and not cfg.stopped():
         violation = cfg.violation(xt)
         if violation <= tol and is_integral(xt, int_idx, tol):
             return True, xt.copy()
@@ -432,7 +438,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
         iteration = stats.iterations
         since_restart += 1
 
-        # Project x̃ onto the LP relaxation.
+        # ℓ1 projection of the current integer target onto the LP polytope.
         t = time.perf_counter()
         gap_used = gap
         params = params_cls(iteration_limit=cfg.projection_iteration_limit,
@@ -440,8 +446,11 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
                             eps_feasible_relative=gap * cfg.residual_ratio,
                             time_limit=cfg.remaining())
         x, inner = yield ("target", xt, params)
-        # Mixed models: re-solve an integral projection whose rounding is
-        # infeasible only through its continuous part (src/pump.jl:340-356).
+        if cfg.stopped():
+            return False, xt
+        # Mixed models: when the projection is integral but rounding it still
+        # breaks a row (so the continuous part is off), solve again at a
+        # tenfold tighter gap, at most five times a run (src/pump.jl:340-356).
         while (mixed and tightening_budget > 1e-6 and cfg.elapsed() < cfg.time_limit
                and is_integral(x, int_idx, tol)
                and cfg.violation(round_to_nearest(x, milp)) > tol):
@@ -453,6 +462,8 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
                                 time_limit=cfg.remaining())
             x, count = yield ("tighten", None, params)
             inner += count
+            if cfg.stopped():
+                return False, xt
         stats.pdhg_iterations += inner
         if gap > cfg.gap_floor:
             gap = max(gap * cfg.gap_decay, cfg.gap_floor)
@@ -466,7 +477,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
                                  "integral", cfg.elapsed())
                 return True, candidate
 
-        # Round x; if the rounding did not change, flip (FP's stalling move).
+        # New target from the projection; if rounding reproduces the old target, flip coordinates instead.
         t = time.perf_counter()
         x_round = rounder.round_guarded(x, rng, iteration)
         stats.rounding_time += time.perf_counter() - t
@@ -479,7 +490,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
             flip_farthest(candidate, x, milp, flips)
             action = "flip"
 
-        # Restart on a cycle (a point seen before) and periodically.
+        # Perturb when the target repeats an earlier one, and every restart_period iterations.
         h_candidate = integer_key(candidate, int_idx)
         cycled = h_candidate in seen
         periodic = since_restart >= cfg.restart_period
@@ -487,7 +498,7 @@ def _trajectory(milp, cfg: _Config, seed: int, x, dual_slack, stats, emit):
             perturb(candidate, x, milp, rng, general, is_general, ladder, iteration)
             stats.perturbations += 1
             since_restart = 0
-            for _ in range(10):              # until the point is new
+            for _ in range(10):              # at most 10 retries for an unseen target
                 h = integer_key(candidate, int_idx)
                 if not (h in seen or h == h_current):
                     break
@@ -550,6 +561,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             return checker(v)
 
     cfg.violation = violation
+    cfg.stop = threading.Event()
     mixed = num_continuous(milp) > 0
     root_stats = _Counters()
     root_log: list = []
@@ -592,9 +604,9 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
     fp64: list = [None] * K        # a trajectory's own FP64 projector, once it needs one
     target: list = [None] * K      # its last target (scaled)
 
-    def finish(k, found, xt):
-        r = _finish(milp, found, xt, stats[k], cfg.elapsed(), _emitter(logs[k], False),
-                    logs[k])
+    def finish(k, found, xt, t=None):
+        r = _finish(milp, found, xt, stats[k], cfg.elapsed() if t is None else t,
+                    _emitter(logs[k], False), logs[k])
         r.trajectory = k
         r.seed = seeds[k]
         return r
@@ -680,8 +692,13 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                 else:
                     last = (k, end[1])
         if winner is not None:
-            pool.shutdown(wait=False, cancel_futures=True)
-            return finish(winner[0], True, winner[1])
+            # the time to the first feasible point is taken now; the other
+            # trajectories' host work in flight ends at its next step, and is
+            # waited for, so that none of it outlives this call
+            t_win = cfg.elapsed()
+            cfg.stop.set()
+            pool.shutdown(wait=True, cancel_futures=True)
+            return finish(winner[0], True, winner[1], t_win)
     pool.shutdown()
     k, xt = last if last is not None else (0, x)
     return finish(k, False, xt)

@@ -1,12 +1,13 @@
-"""Rounding: FP 2.0's constraint-propagation rounding (CPR) and the guarded
-two-phase CPR (host; the reference's arithmetic, unchanged).
+"""The pump's rounders, on the host, with src/rounding.jl's arithmetic.
 
-Restates src/rounding.jl.  CPR ranks the integer variables by fractionality
-(binaries first), then fixes them one at a time to their rounded LP value,
-propagating each fixing.  The guarded rounder first fixes, in one
-propagation, every variable whose LP value is already integral (the batch),
-then rounds the rest sequentially; if the batch makes a row infeasible its
-work is discarded and the call falls back to sequential CPR.
+Sequential rounding orders the integer columns (binary columns ahead of the
+others, then by distance of the LP value from its nearest integer) and
+settles them in that order: each column is set to its rounded LP value and
+the bound propagation runs before the next one is settled.  The guarded
+variant settles, in a single propagation, all columns whose LP value is an
+exact integer, and only then walks the remaining columns one by one; should
+that first propagation prove some row infeasible, everything it did is
+thrown away and the sequential rounder runs from scratch instead.
 
 Random numbers: the reference draws from Julia's MersenneTwister; here the
 caller's ``numpy.random.Generator`` is used at the same points
@@ -40,8 +41,9 @@ def jclamp(x: float, lo: float, hi: float) -> float:
 
 
 class BatchPolicy:
-    """Online rule that decides, per call, whether to try the batch
-    (src/rounding.jl:22-63)."""
+    """Chooses at each call between trying the all-at-once step of the
+    guarded rounder and going straight to sequential rounding, from running
+    averages of what each choice cost before (src/rounding.jl:22-63)."""
 
     def __init__(self, xi: float = 0.3, nu: float = 0.9, reprobe: int = 20):
         self.xi, self.nu, self.reprobe = xi, nu, reprobe
@@ -82,8 +84,9 @@ class BatchPolicy:
 
 
 class Rounder:
-    """Rounding state that persists across the calls of one pump run
-    (src/rounding.jl:78-98)."""
+    """One pump run's rounder: the model's propagation data, the schedule,
+    the optional violation-guided columns and the counters live here from
+    call to call (src/rounding.jl:78-98)."""
 
     def __init__(self, milp: MILP, *, schedule: str = "all", max_pops: int = 2**63 - 1,
                  swap_iteration: int = 10, policy: BatchPolicy | None | str = "default",
@@ -227,7 +230,7 @@ class Rounder:
         if self._native is not None:
             xt_native, infeasible = self._native.round_guarded(x_star, ranked_arr,
                                                                self.schedule == "regroup")
-            if infeasible:               # the guard: discard the batch
+            if infeasible:               # a row went infeasible: drop the all-at-once step
                 self.n_fallbacks += 1
                 wasted = 0.0 if P is None else (time.perf_counter_ns() - t0) / 1e9
                 out = self.round_sequential(x_star, rng, iteration)
@@ -272,7 +275,7 @@ class Rounder:
             xt[j] = target
         s.propagate()
 
-        if s.infeasible:                 # the guard: discard the batch
+        if s.infeasible:                 # a row went infeasible: drop the all-at-once step
             self.n_fallbacks += 1
             wasted = 0.0 if P is None else (time.perf_counter_ns() - t0) / 1e9
             out = self.round_sequential(x_star, rng, iteration)

@@ -1,9 +1,9 @@
-"""Short text summaries of rocprofv3 output for the committed profiles.
+"""Short text summaries of rocprofv3 output for the committed profiles (measurement tooling beside bench.py; not part of the library).
 
-    python -m pdfp_amd.rocprof_summary stats KERNEL_STATS_CSV [--top N]
-    python -m pdfp_amd.rocprof_summary window KERNEL_TRACE_CSV --last N \
+    python rocprof_summary.py stats KERNEL_STATS_CSV [--top N]
+    python rocprof_summary.py window KERNEL_TRACE_CSV --last N \
         EPILOGUE [EPILOGUE ...]
-    python -m pdfp_amd.rocprof_summary traffic --fetch CSV --write CSV \\
+    python rocprof_summary.py traffic --fetch CSV --write CSV \\
         --copy-fetch CSV --copy-write CSV
 
 ``stats`` turns a ``--kernel-trace --stats`` kernel_stats.csv into a
@@ -114,24 +114,26 @@ def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str,
     fw = copy_bytes / (_copy_kib(copy_write) * 1024.0)
     F, W = _per_kernel(fetch), _per_kernel(write)
 
-    def tuned(counts: dict) -> dict:
-        # per epilogue, the instantiation with the most dispatches (the tuned
-        # schedule; the two counter runs may tune to different block sizes)
-        best: dict = {}
-        for name, vals in counts.items():
-            key = pass_key(name)
-            if key is not None and (key not in best or len(vals) > len(counts[best[key]])):
-                best[key] = name
-        return best
-
-    bf, bw = tuned(F), tuned(W)
+    # Per epilogue, the one instantiation present in both counter runs with
+    # the most dispatches: with the schedules pinned (bench.py --schedule,
+    # --batch-schedule) that is the benchmarked kernel, and the fetch and
+    # write figures then come from the same kernel.
+    best: dict = {}
+    for name in set(F) & set(W):
+        key = pass_key(name)
+        if key is None:
+            continue
+        n = len(F[name]) + len(W[name])
+        if key not in best or n > len(F[best[key]]) + len(W[best[key]]):
+            best[key] = name
+    bf = bw = best
     recs = {}
     lines = [f"copy calibration: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f} "
              f"(1 GiB copy / counter reading)", "",
              "| epilogue | dispatches (fetch / write run) | FETCH KiB | WRITE KiB | "
              "HBM MB per launch |",
              "|---|---:|---:|---:|---:|"]
-    for key in sorted(set(bf) & set(bw), key=lambda k: -len(F[bf[k]])):
+    for key in sorted(best, key=lambda k: -len(F[bf[k]])):
         nf, nw = bf[key], bw[key]
         fk = sum(F[nf]) / len(F[nf])
         wk = sum(W[nw]) / len(W[nw])
