@@ -483,6 +483,24 @@ int pdfp_checker_max_violation(pdfp_checker* c, const double* x, double* max_vio
 const char* pdfp_checker_last_error(const pdfp_checker* c);
 void pdfp_checker_destroy(pdfp_checker* c);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): max_violation (src/milp.jl:102-114) of a device-resident point
+ * on a device-resident CSR matrix, with the kernels of
+ * pdfp_checker_max_violation, allocating nothing.  All pointers are device
+ * pointers: rowptr (m + 1, int32, 0-based), col (int32) and val (FP64) of A
+ * in CSR with ascending columns per row, b and is_le (1 = ≤ row, 0 = equality)
+ * of length m, x of length n, and out (one double) that receives the result.
+ * The launches go on `stream` (a hipStream_t; NULL = the null stream) and the
+ * call returns without waiting for them.  The per-workgroup partial maxima
+ * live in a static device array of 65,536 doubles (512 KiB) on each GPU, so
+ * m may not exceed 16,777,216 rows (PDFP_ERR_ARGUMENT beyond; the message is
+ * at pdfp_checker_last_error(NULL)).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+int pdfp_max_violation_noalloc(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* col,
+                               const double* val, const double* b, const uint8_t* is_le,
+                               const double* x, double* out, void* stream);
+
 /* ---- preconditioning ------------------------------------------------------
  * scale_problem (src/preconditioner.jl:26-63), the matrix part, on the
  * device: `ruiz_iterations` Ruiz rounds and one Pock–Chambolle pass

@@ -187,6 +187,9 @@ SIGNATURES = {
     "pdfp_checker_max_violation": (C.c_int, [_P, _D, _D]),
     "pdfp_checker_last_error": (C.c_char_p, [_P]),
     "pdfp_checker_destroy": (None, [_P]),
+    # device pointers throughout; the last argument is a hipStream_t
+    "pdfp_max_violation_noalloc": (C.c_int, [C.c_int64, C.c_int64, _P, _P, _P, _P, _P, _P,
+                                             _P, _P]),
     "pdfp_probe_last_error": (C.c_char_p, []),
 }
 

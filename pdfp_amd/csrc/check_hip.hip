@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -127,6 +128,36 @@ struct Checker {
 
 std::string g_checker_error = "no error";
 
+// Scratch of the allocation-free check: one partial maximum per workgroup of
+// k_violation, so at most kNoAllocMaxBlocks · 256 rows.  Every call
+// overwrites the first ceil(m / 256) entries before k_max_partials reads
+// them, so nothing needs clearing between calls.  Each GPU holds its own copy
+// for the life of the process; host code must use the per-device address
+// from hipGetSymbolAddress, never the array's name.
+constexpr int64_t kNoAllocMaxBlocks = 65536;
+constexpr int64_t kNoAllocMaxRows = kNoAllocMaxBlocks * 256;
+__device__ __align__(256) double scratch[kNoAllocMaxBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+double* scratch_on_current_device() {
+  int dev = 0;
+  PDFP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(PDFP_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    PDFP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(PDFP_ERR_DEVICE, "no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace pdfp
 
@@ -176,5 +207,35 @@ int pdfp_checker_max_violation(pdfp_checker* c, const double* x, double* out) {
 }
 
 void pdfp_checker_destroy(pdfp_checker* c) { delete c; }
+
+int pdfp_max_violation_noalloc(int64_t m, int64_t n, const int32_t* rowptr, const int32_t* col,
+                               const double* val, const double* b, const uint8_t* is_le,
+                               const double* x, double* out, void* stream) {
+  try {
+    if (m < 0 || n < 0) throw pdfp::Error(PDFP_ERR_ARGUMENT, "negative size");
+    if (m > pdfp::kNoAllocMaxRows)
+      throw pdfp::Error(PDFP_ERR_ARGUMENT,
+                        "pdfp_max_violation_noalloc: m = " + std::to_string(m) +
+                            " rows exceeds the maximum of " +
+                            std::to_string(pdfp::kNoAllocMaxRows) + " (the scratch array)");
+    if (n > INT32_MAX) throw pdfp::Error(PDFP_ERR_ARGUMENT, "n exceeds the int32 range");
+    if (!out || (m > 0 && (!rowptr || !b || !is_le)))
+      throw pdfp::Error(PDFP_ERR_ARGUMENT, "null argument");
+    double* part = pdfp::scratch_on_current_device();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nb = (int)((m + 255) / 256);
+    if (nb > 0)
+      pdfp::k_violation<<<nb, 256, 0, s>>>((int)m, rowptr, col, val, x, b, is_le, part);
+    pdfp::k_max_partials<<<1, 256, 0, s>>>(nb, part, out);
+    PDFP_HIP(hipGetLastError());
+    return PDFP_OK;
+  } catch (const pdfp::Error& e) {
+    pdfp::g_checker_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    pdfp::g_checker_error = e.what();
+    return PDFP_ERR_DEVICE;
+  }
+}
 
 }  // extern "C"
