@@ -386,3 +386,36 @@ def test_batch_on_instance_with_empty_row_and_column(path):
         assert abs(l1b - l1s) <= 1e-4 * max(1.0, l1s), (k, l1b, l1s)
         s.close()
     bt.close()
+
+
+def test_batch_general_integer_equality_heavy():
+    """Eight trajectories on a general-integer, equality-heavy instance (40 %
+    equality rows, integer columns in [0, U_j] with U_j up to 20, integer
+    coefficients): each trajectory's point meets the tolerance under the
+    oracle's FP64 residuals, and its ℓ1 distance is within the gap tolerance
+    of the oracle's own FP64 projection of the same target."""
+    print("batch general integer", flush=True)
+    m = G.general_integer(2000, 4000, 5)
+    assert (m.row_sense == "eq").mean() > 0.3 and m.u.max() > 1
+    pp = projector_problem(m)
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    pw = float(np.sqrt(1.0 + len(m.int_idx) / len(m.b)))
+    bt = _batch(pp, 8)
+    targets = [G.binary_targets(m, 700 + k) / pp.D_c for k in range(8)]
+    for k in range(8):
+        bt.set_target(k, targets[k])
+    assert bt.solve(p, pw) == ["OPTIMAL"] * 8
+    ii = np.asarray(pp.is_int, dtype=bool)
+    for k in range(8):
+        r = _oracle_kkt(pp, targets[k], bt.pdhg_x(k), bt.vector(k, "pdhg_y"),
+                        bt.vector(k, "pdhg_z"), bt.vector(k, "dual_slack"))
+        assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (k, r)
+        if k < 3:
+            o = O.OraclePDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+            O.set_target(o, targets[k])
+            assert O.pdfp_solve(o, p, pw) == "OPTIMAL"
+            l1o = o.info().l1_distance
+            den = 1.0 + float(np.sum(np.asarray(pp.w)[ii] * np.abs(targets[k][ii])))
+            assert abs(r[3] - l1o) <= 2 * 10 * tol * den, (k, r[3], l1o)

@@ -1,5 +1,7 @@
 """GPU tests at the benchmark's full sizes: the 1M-nonzero mixed MILP
-(50k × 100k) and the 5M-nonzero power-law one (500k × 1M).
+(50k × 100k), the 5M-nonzero power-law one (500k × 1M), the 4M-nonzero
+general-integer one with 8 batched trajectories (200k × 400k) and the pack of
+64 uneven set covers (15.5M nonzeros).
 
 The oracle is too slow to run these projections, so the checks are the
 size-independent ones: every FP32 projection (single and batched) is
@@ -100,3 +102,80 @@ def test_full_size_projections_fp32_single_and_batch_against_fp64(name):
             assert abs(r[3] - l1_64[k]) <= 2 * TOL * _denominator(pp, targets[k]), \
                 ("batch", k, r[3], l1_64[k])
     bt.close()
+
+
+def test_full_size_general_integer_batch_of_8():
+    """The general-integer, equality-heavy configuration at full size:
+    200,000 rows (40 % equalities) × 400,000 integer columns in [0, U_j],
+    4M coefficients in [-9, 9], 8 trajectories on one batch.  Every
+    trajectory's point meets the tolerance under the oracle's FP64 residuals,
+    and the first two trajectories' ℓ1 distances agree with the FP64 form's
+    projections of the same targets within the gap tolerance."""
+    print("full size general integer batch", flush=True)
+    milp = G.general_integer(200_000, 400_000, 1)
+    pp = projector_problem(milp)
+    n = milp.A.shape[1]
+    x_start = np.clip(np.zeros(n), milp.l, milp.u) / pp.D_c
+    pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
+    p = SolverParameters(eps_optimal_relative=TOL, eps_feasible_relative=TOL,
+                         iteration_limit=200_000)
+    kkt = _Kkt(pp)
+    targets = [G.binary_targets(milp, 90 + k) / pp.D_c for k in range(8)]
+    s64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                      precision="fp64")
+    l1_64 = []
+    for xt in targets[:2]:
+        s64.set_x(x_start)
+        P.set_target(s64, xt)
+        assert P.pdfp_solve(s64, p, pw) == "OPTIMAL"
+        l1_64.append(s64.info().l1_distance)
+    s64.close()
+    bt = PDFPBatch(8, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    for k in range(8):
+        bt.set_x(k, x_start)
+        bt.set_target(k, targets[k])
+    assert bt.solve(p, pw) == ["OPTIMAL"] * 8
+    for k in range(8):
+        r = kkt(targets[k], bt.pdhg_x(k), bt.vector(k, "pdhg_y"), bt.vector(k, "pdhg_z"),
+                bt.vector(k, "dual_slack"))
+        assert r[0] <= 10 * TOL and r[1] <= 10 * TOL and r[2] <= 10 * TOL, ("batch", k, r)
+        if k < 2:
+            assert abs(r[3] - l1_64[k]) <= 2 * TOL * _denominator(pp, targets[k]), \
+                ("batch", k, r[3], l1_64[k])
+    bt.close()
+
+
+def test_full_size_pack_of_64_uneven_set_covers():
+    """The 64-instance configuration at full size: set covers of uneven sizes
+    (seeds 1..64, rows e^U[ln 2k, ln 40k], columns e^U[ln 4k, ln 80k]; 15.5M
+    nonzeros in all) in one block-diagonal pack.  Every instance's point
+    meets the tolerance under the oracle's FP64 residuals, and its ℓ1
+    distance agrees with the single FP64 form's projection of the same
+    target within the gap tolerance."""
+    from pdfp_amd.pack import PDFPPack
+
+    print("full size pack of 64", flush=True)
+    milps = G.uneven_set_covers(64)
+    pps = [projector_problem(m) for m in milps]
+    assert sum(m.A.nnz for m in milps) > 15_000_000
+    p = SolverParameters(eps_optimal_relative=TOL, eps_feasible_relative=TOL,
+                         iteration_limit=200_000)
+    pws = [float(np.sqrt(1.0 + len(m.int_idx) / max(len(m.b), 1))) for m in milps]
+    targets = [G.binary_targets(m, 60 + k) / pp.D_c for k, (m, pp) in enumerate(zip(milps, pps))]
+    pk = PDFPPack(pps)
+    for k, t in enumerate(targets):
+        pk.set_target(k, t)
+    assert pk.solve(p, pws) == ["OPTIMAL"] * 64
+    for k, pp in enumerate(pps):
+        r = _Kkt(pp)(targets[k], pk.pdhg_x(k), pk.vector(k, "pdhg_y"), pk.vector(k, "pdhg_z"),
+                     pk.vector(k, "dual_slack"))
+        assert r[0] <= 10 * TOL and r[1] <= 10 * TOL and r[2] <= 10 * TOL, ("pack", k, r)
+        s64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                          precision="fp64")
+        P.set_target(s64, targets[k])
+        assert P.pdfp_solve(s64, p, pws[k]) == "OPTIMAL"
+        l1_64 = s64.info().l1_distance
+        s64.close()
+        assert abs(r[3] - l1_64) <= 2 * TOL * _denominator(pp, targets[k]), \
+            ("pack", k, r[3], l1_64)
+    pk.close()
