@@ -50,7 +50,7 @@ struct Trajectory {
   bool running = false;  // begun and not yet terminated
   pdfp_params p{};       // the parameters of its current solve
   double t_start = 0.0;
-  std::vector<double> h_xt;
+  bool l1_den_pending = false;  // summed on the host while the next block runs
 };
 
 struct Batch {
@@ -63,7 +63,12 @@ struct Batch {
   DevBuf<float> z, z0, pz, rz;                         // m × 8
   DevBuf<uint8_t> is_eq;
   DevBuf<double> row_unscale;
-  DevBuf<double> lb64, ub64, w64, xt64, x64;
+  DevBuf<double> lb64, ub64, w64, x64;
+  // each trajectory's target x̃ (FP64, scaled) as the caller gave it: in
+  // pinned host memory (the upload's source and the ℓ1-gap denominator's
+  // input) and on the device (for the polish), n values per trajectory
+  PinnedBuf<double> h_xt;
+  DevBuf<double> xt64;
   DevBuf<float> tmpf;
   DevBuf<double> part_t, part_a, part_v, sums;
   DevBuf<BatchStep> d_sp;
@@ -164,8 +169,15 @@ struct Batch {
     vec64(lb64, var_lb);
     vec64(ub64, var_ub);
     vec64(w64, h_w.data());
-    vec64(xt64, nullptr);
     vec64(x64, nullptr);
+    xt64.alloc((size_t)std::max<int64_t>(n, 1) * (size_t)K);
+    h_xt.alloc((size_t)std::max<int64_t>(n, 1) * (size_t)K);
+    std::memset(h_xt.p, 0, sizeof(double) * h_xt.n);
+    {
+      std::vector<double> zero(h_xt.n, 0.0);
+      xt64.upload(zero.data(), zero.size(), stream);
+      PDFP_HIP(hipStreamSynchronize(stream));
+    }
     part_v.alloc((size_t)std::max(grid_for(std::max(m, n)), 1) + 8);
     sums.alloc((size_t)kBatch * 32 + 16);
     h_sums.alloc((size_t)kBatch * 32 + 16);
@@ -185,7 +197,6 @@ struct Batch {
       h_sargs.alloc(kBatch);
     }
     for (int q = 0; q < kBatch; ++q) h_sp.p[q] = BatchStep{1.0, 1.0, 1.0, 0.0, 0.0, {0, 0, 0}};
-    for (auto& t : tr) t.h_xt.assign((size_t)n, 0.0);
     autotune();
     part_t.alloc((size_t)std::max<int64_t>((n + kEvalRows - 1) / kEvalRows, 1) * kBatch *
                  kEvalColTerms);
@@ -260,7 +271,6 @@ struct Batch {
     const int64_t len = std::max(n, m);
     if (len) k_batch_warm<<<grid_for(len), 256, 0, stream>>>((int)n, (int)m, k, x.p, y.p, z.p,
                                                               x0.p, y0.p, z0.p);
-    sync();
     Trajectory& t = tr[(size_t)k];
     t.r.inner_count = 0;
     t.r.total_count = 0;
@@ -270,10 +280,32 @@ struct Batch {
     x64_owner = -1;
   }
 
+  // 1 + Σ_I w|x̃| of trajectory k in the reference's order (src/pdfp.jl:230-240).
+  void finish_l1_den(int k) {
+    Trajectory& t = tr[(size_t)k];
+    if (!t.l1_den_pending) return;
+    const double* h = h_xt.p + (size_t)k * (size_t)n;
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; ++j)
+      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h[(size_t)j]) : 0.0;
+    t.l1_den = 1.0 + acc;
+    t.l1_den_pending = false;
+  }
+
+  // set_target!  src/pdfp.jl:98-101: one copy into pinned memory, then an
+  // asynchronous upload and the narrowing and interleaving on the device.
   void set_target(int k, const double* xts) {
     check_k(k);
-    scatter(xt, n, k, xts);
-    std::copy(xts, xts + n, tr[(size_t)k].h_xt.begin());
+    sync();  // no copy of an earlier target may still read h_xt
+    if (n) {
+      double* h = h_xt.p + (size_t)k * (size_t)n;
+      double* d = xt64.p + (size_t)k * (size_t)n;
+      std::memcpy(h, xts, sizeof(double) * (size_t)n);
+      PDFP_HIP(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, stream));
+      k_batch_scatter64<<<grid_for(n), 256, 0, stream>>>((int)n, k, d, xt.p);
+      PDFP_HIP(hipGetLastError());
+    }
+    if (x64_owner == k) x64_owner = -1;
     warm_start(k);
   }
 
@@ -600,10 +632,7 @@ struct Batch {
     t.r.best_primal_weight = pw;
     t.r.primal_weight_error_sum = 0.0;
     t.r.primal_weight_last_error = 0.0;
-    double acc = 0.0;
-    for (int64_t j = 0; j < n; ++j)
-      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(t.h_xt[(size_t)j]) : 0.0;
-    t.l1_den = 1.0 + acc;
+    t.l1_den_pending = true;  // finish_l1_den, before its first evaluation is read
     t.do_restart = false;
     t.n_restarts = 0;
     t.n_evals = 0;
@@ -664,6 +693,8 @@ struct Batch {
       if (small_used) small_block(freq, omega, active);
       else if (graph_on) block_graph(freq, any_restart);
       else general_block(freq, any_restart);
+      for (int k = 0; k < K; ++k)
+        if (active[(size_t)k]) finish_l1_den(k);  // on the host while the block runs
       sync();
       const double now = wall_seconds();
       bool ended = false;
@@ -737,10 +768,10 @@ struct Batch {
   void polish(int k) {
     Trajectory& t = tr[(size_t)k];
     if (x64_owner == k || n == 0) return;
-    xt64.upload(t.h_xt.data(), (size_t)n, stream);
     const int g = grid_for(n);
-    k_batch_polish<<<g, 256, 0, stream>>>((int)n, k, px.p, lb.p, ub.p, lb64.p, ub64.p, xt64.p,
-                                          w64.p, x64.p, part_v.p);
+    k_batch_polish<<<g, 256, 0, stream>>>((int)n, k, px.p, lb.p, ub.p, lb64.p, ub64.p,
+                                          xt64.p + (size_t)k * (size_t)n, w64.p, x64.p,
+                                          part_v.p);
     finalize_partials<256><<<1, 256, 0, stream>>>(part_v.p, g, 1, 0, sums.p + kBatch * 32);
     PDFP_HIP(hipMemcpyAsync(h_sums.p + kBatch * 32, sums.p + kBatch * 32, sizeof(double),
                             hipMemcpyDeviceToHost, stream));

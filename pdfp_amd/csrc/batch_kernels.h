@@ -378,6 +378,15 @@ static __global__ __launch_bounds__(256) void k_batch_scatter(int len, int k,
   if (i < len) out[(size_t)i * kBatch + k] = in[i];
 }
 
+// Writes trajectory k's FP64 target, narrowed to FP32, into the interleaved
+// vector (the same rounding as a host conversion).
+static __global__ __launch_bounds__(256) void k_batch_scatter64(int len, int k,
+                                                               const double* __restrict__ in,
+                                                               float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) out[(size_t)i * kBatch + k] = (float)in[i];
+}
+
 // Reads column k of an interleaved vector into a dense FP32 vector.
 static __global__ __launch_bounds__(256) void k_batch_gather(int len, int k,
                                                             const float* __restrict__ in,

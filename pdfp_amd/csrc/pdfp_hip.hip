@@ -70,8 +70,13 @@ struct Projector final : ProjectorBase {
   DevBuf<StepParams> d_sp;
   PinnedBuf<StepParams> h_sp;
   PinnedBuf<double> h_sums;
+  // x̃ (FP64, scaled) as the caller gave it: the source of its upload and of
+  // the ℓ1-gap denominator, which is summed on the host while the first
+  // block of a solve runs on the device
+  PinnedBuf<double> h_xt;
+  bool l1_den_pending = false;
 
-  std::vector<double> h_w, h_xt, h_b;  // w (0 off I), x̃ and b in FP64
+  std::vector<double> h_w, h_b;  // w (0 off I) and b in FP64
   bool orig_primal = false;
   double b_norm = 0.0, b_norm_original = NAN;
 
@@ -131,8 +136,14 @@ struct Projector final : ProjectorBase {
     for (int64_t j = 0; j < n; ++j)
       if (is_int_[j] && !(w_[j] > 0.0))
         throw Error(PDFP_ERR_ARGUMENT, "weights of integer columns must be positive");
-    h_xt.assign((size_t)n, 0.0);
+    h_xt.alloc((size_t)std::max<int64_t>(n, 1));
+    std::memset(h_xt.p, 0, sizeof(double) * (size_t)std::max<int64_t>(n, 1));
     h_b.assign(b_, b_ + m);
+    {
+      double a2 = 0.0;
+      for (int64_t j = 0; j < n; ++j) a2 += h_w[(size_t)j] * h_w[(size_t)j];
+      dual_den = 1.0 + std::sqrt(a2);  // 1 + ‖w_I‖₂: w never changes
+    }
     double acc = 0.0;
     for (int64_t i = 0; i < m; ++i) acc += b_[i] * b_[i];
     b_norm = std::sqrt(acc);
@@ -327,12 +338,21 @@ struct Projector final : ProjectorBase {
     upload_T(x0, xs, n);
   }
 
+  // set_target!  src/pdfp.jl:98-101: one copy into pinned memory, then an
+  // asynchronous upload (and, in the FP32 form, the narrowing on the device).
   void set_target(const double* xts) override {
-    upload_T(xt, xts, n);
-    std::copy(xts, xts + n, h_xt.begin());
-    if (kFused) {
-      xt64.upload(xts, (size_t)n, stream);
-      sync();
+    sync();  // no copy of an earlier target may still read h_xt
+    if (n) {
+      std::memcpy(h_xt.p, xts, sizeof(double) * (size_t)n);
+      if constexpr (kFused) {
+        PDFP_HIP(hipMemcpyAsync(xt64.p, h_xt.p, sizeof(double) * (size_t)n,
+                                hipMemcpyHostToDevice, stream));
+        k_narrow<T><<<grid_for(n), 256, 0, stream>>>((int)n, xt64.p, xt.p);
+        PDFP_HIP(hipGetLastError());
+      } else {
+        PDFP_HIP(hipMemcpyAsync(xt.p, h_xt.p, sizeof(double) * (size_t)n,
+                                hipMemcpyHostToDevice, stream));
+      }
     }
     warm_start();
   }
@@ -344,7 +364,6 @@ struct Projector final : ProjectorBase {
       PDFP_HIP(hipMemcpyAsync(y0.p, y.p, sizeof(T) * n, hipMemcpyDeviceToDevice, stream));
     }
     if (m) PDFP_HIP(hipMemcpyAsync(z0.p, z.p, sizeof(T) * m, hipMemcpyDeviceToDevice, stream));
-    sync();
     r.inner_count = 0;
     r.total_count = 0;
     r.fixed_point_error = INFINITY;
@@ -535,13 +554,17 @@ struct Projector final : ProjectorBase {
     r.best_primal_weight = pw;
     r.primal_weight_error_sum = 0.0;
     r.primal_weight_last_error = 0.0;
+    l1_den_pending = true;  // summed by finish_l1_den, before the first evaluation is read
+  }
+
+  // 1 + Σ_I w|x̃| in the reference's order (src/pdfp.jl:230-240).
+  void finish_l1_den() {
+    if (!l1_den_pending) return;
     double acc = 0.0;
-    for (int64_t j = 0; j < n; ++j) acc += h_w[(size_t)j] * h_w[(size_t)j];
-    dual_den = 1.0 + std::sqrt(acc);
-    acc = 0.0;
     for (int64_t j = 0; j < n; ++j)
-      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h_xt[(size_t)j]) : 0.0;
+      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h_xt.p[(size_t)j]) : 0.0;
     l1_den = 1.0 + acc;
+    l1_den_pending = false;
   }
 
   // Scalars of one evaluation slot (see evaluate()).
@@ -590,6 +613,7 @@ struct Projector final : ProjectorBase {
       } else {
         general_block(freq, restart_eval);
       }
+      finish_l1_den();  // on the host while the block runs
       sync();
       ++n_evals;
       if (restart_eval) {

@@ -347,6 +347,15 @@ __global__ __launch_bounds__(256) void k_convert(int n, TO* __restrict__ out,
   if (i < n) out[i] = (TO)in[i];
 }
 
+// out[i] = in[i] in the iterate precision (round to nearest, as the host's
+// conversion): the target arrives in FP64 and is narrowed on the device.
+template <typename T>
+__global__ __launch_bounds__(256) void k_narrow(int n, const double* __restrict__ in,
+                                                T* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)in[i];
+}
+
 // FP64 polish of an FP32 projection: a coordinate that the primal prox
 // clamped to a bound gets that bound's FP64 value (x = D_c·clamp(x̂, l/D_c,
 // u/D_c) then holds exactly as in FP64); interior coordinates are widened.
