@@ -163,3 +163,25 @@ def test_bench_finds_the_traffic_of_each_workload(workload, keys):
     for key in keys:
         traffic = bench.measured_traffic(workload, key)
         assert traffic is not None and traffic > 0, (workload, key)
+
+
+def _julia_struct_fields(name: str):
+    """(field, type) pairs of a `struct name ... end` block in INTEGRATION.md's shim."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = text.split(f"struct {name}", 1)[1].split("\nend", 1)[0]
+    body = body.split("\n", 1)[1]                      # drop the comment on the header line
+    return re.findall(r"(\w+)::(Float64|Int64|Int32)", body)
+
+
+def _c_type_name(ctype):
+    return {C.c_double: "Float64", C.c_int64: "Int64", C.c_int32: "Int32"}[ctype]
+
+
+@pytest.mark.parametrize("julia,ctype", [("SolverParametersC", _lib.PdfpParams),
+                                         ("PDFPInfoC", _lib.PdfpInfo),
+                                         ("PDLPInfoC", _lib.PdlpInfo)])
+def test_julia_shim_structs_mirror_the_c_structs(julia, ctype):
+    """The Julia structs of the integration shim have the C structs' fields in
+    the same order with the same types, so `Ref{…}` passes them by layout."""
+    want = [(f, _c_type_name(t)) for f, t in ctype._fields_]
+    assert _julia_struct_fields(julia) == want
