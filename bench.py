@@ -139,13 +139,16 @@ def run_projections(s, pp, milp, targets, tol):
 
 
 def kernel_bytes_batch(m: int, n: int, nnz: int, K: int = 8) -> tuple[float, float]:
-    """Algorithmic bytes of the two batched launches (8 trajectory slots,
-    FP32).  Transpose pass: 8·nnz + 4(n+1), z of the 8 slots read once 32m,
-    per column x, y, x⁰, y⁰, x̃ (32 B each) and ℓ, u, w (4 B each) 172n,
-    write x, y, x̄ 96n.  A pass: 8·nnz + 4(m+1), x̄ read once 32n, per row
-    z, z⁰ (32 B each), b, is_eq 69m, write z 32m."""
-    at = 8 * nnz + 4 * (n + 1) + 32 * m + 172 * n + 96 * n
-    a = 8 * nnz + 4 * (m + 1) + 32 * n + 69 * m + 32 * m
+    """Algorithmic bytes of the two batched launches with W = 8 trajectory
+    slots per column (K ≤ 8) or W = 16 (K ≤ 16), FP32, r = 4W bytes per
+    interleaved record.  Transpose pass: 8·nnz + 4(n+1), z of the W slots
+    read once r·m, per column x, y, x⁰, y⁰, x̃ (r bytes each) and ℓ, u, w
+    (4 B each) (5r + 12)·n, write x, y, x̄ 3r·n.  A pass: 8·nnz + 4(m+1),
+    x̄ read once r·n, per row z, z⁰ (r bytes each), b, is_eq (2r + 5)·m,
+    write z r·m."""
+    r = 4 * (8 if K <= 8 else 16)
+    at = 8 * nnz + 4 * (n + 1) + r * m + (5 * r + 12) * n + 3 * r * n
+    a = 8 * nnz + 4 * (m + 1) + r * n + (2 * r + 5) * m + r * m
     return float(at), float(a)
 
 
@@ -466,7 +469,12 @@ def measured_traffic(workload: str, kernel_key: str):
     if "```json" not in text:
         return None
     block = text.split("```json", 1)[1].split("```", 1)[0]
-    rec = json.loads(block).get(kernel_key)
+    recs = json.loads(block)
+    rec = recs.get(kernel_key)
+    if rec is None and "<8, " in kernel_key:
+        # summaries taken before the batch width became a template argument
+        # name the same width-8 kernels without it
+        rec = recs.get(kernel_key.replace("<8, ", "<"))
     return None if rec is None else rec["traffic_bytes"]
 
 
@@ -601,7 +609,7 @@ def main():
               f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)
         bt.close()
         head = results["batch"]
-        bat, ba = kernel_bytes_batch(mm, nn, nnz)
+        bat, ba = kernel_bytes_batch(mm, nn, nnz, K)
         steps_desc = (f"{K} warm-started l1 projections to new integer targets at relative KKT "
                       f"tolerance {args.tol:g}, served by the {K} trajectory slots of one batch "
                       f"sharing A (a slot that finishes takes the next target)")
@@ -612,7 +620,8 @@ def main():
                       f"tolerance {args.tol:g}")
     r32 = results["fp32"]
     value = head["iters"] / head["seconds"]
-    keys = (("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>") if K > 1 else
+    width = 8 if K <= 8 else 16
+    keys = ((f"EpiBatchPrimalY<{width}, false>", f"EpiBatchDualZ<{width}, false>") if K > 1 else
             ("EpiPrimalY<float, false>", "EpiDualZ<float, false>"))
     dom_bytes, dom_ms, dom_name, dom_key = max(
         (bat, head["ms_at"], "transpose pass (A^T z + x/y update)", keys[0]),
@@ -623,7 +632,7 @@ def main():
     # column (Aᵀz gathers z: m records; A x̄ gathers x̄: n records).
     from pdfp_amd._lib import gather_ceiling_us
 
-    rec = 32 if K > 1 else 4
+    rec = (32 if K <= 8 else 64) if K > 1 else 4
     ceil = {"transpose_pass": gather_ceiling_us(max(mm, 1) * rec, rec, nnz),
             "row_pass": gather_ceiling_us(max(nn, 1) * rec, rec, nnz)}
     dom_ceil = ceil["transpose_pass" if dom_key == keys[0] else "row_pass"]

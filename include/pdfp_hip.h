@@ -175,10 +175,12 @@ void pdfp_destroy(pdfp_ctx* ctx);
 /* ---- batch of trajectories on one instance ----------------------------- */
 /* The reference runs K concurrent pump trajectories as K OS processes, each
  * with its own PDFPState on the same GPU (scripts/solve.jl:121-148).  A
- * pdfp_batch holds one instance's scaled matrix once and up to 8
+ * pdfp_batch holds one instance's scaled matrix once and up to 16
  * trajectories (own target, iterates, primal weight, restart state) that
  * iterate together in the same fused launches, in the FP32 form; each
- * trajectory follows pdfp_solve! (src/pdfp.jl:252-288) on its own. */
+ * trajectory follows pdfp_solve! (src/pdfp.jl:252-288) on its own.  Up to 8
+ * trajectories are stored 8 to a column (32 bytes), 9 to 16 sixteen to a
+ * column (64 bytes). */
 
 typedef struct pdfp_batch pdfp_batch;
 
@@ -203,7 +205,7 @@ int pdfp_batch_begin(pdfp_batch* ctx, int32_t k, const pdfp_params* params,
 int pdfp_batch_run(pdfp_batch* ctx, int32_t until_any, int32_t* termination,
                    int64_t* total_count, int32_t* n_running);
 
-/* As pdfp_create, for K ∈ 1..8 trajectories (FP32 form). */
+/* As pdfp_create, for K ∈ 1..16 trajectories (FP32 form). */
 int pdfp_batch_create(pdfp_batch** out, int32_t K, int64_t m, int64_t n,
                       int64_t nnz, const int64_t* colptr, const int64_t* rowval,
                       const double* nzval, int32_t index_base, const double* b,

@@ -1,10 +1,12 @@
-"""Batch of up to 8 projector trajectories of one instance on the device.
+"""Batch of up to 16 projector trajectories of one instance on the device.
 
 The reference runs concurrent trajectories as separate OS processes, each
 with its own projector state on the GPU (scripts/solve.jl:121-148).
 ``PDFPBatch`` keeps the instance's scaled matrix in HBM once and runs the K
 trajectories' PDHG iterations in the same fused launches (FP32 iterates,
-FP64 reductions and polish).  Each trajectory has its own target, iterates,
+FP64 reductions and polish).  Up to 8 trajectories share a batch of width 8
+(a column's 8 values are 32 bytes), 9 to 16 one of width 16 (64 bytes, half
+of the line a random gather fills).  Each trajectory has its own target, iterates,
 primal weight, restart state and termination, and follows ``pdfp_solve!``
 (src/pdfp.jl:252-288) on its own.
 """
@@ -21,9 +23,9 @@ from .pdfp import SolverParameters, _csc_arrays
 
 
 class PDFPBatch:
-    """K ∈ 1..8 trajectories on the scaled data of one instance."""
+    """K ∈ 1..16 trajectories on the scaled data of one instance."""
 
-    MAX = 8
+    MAX = 16
 
     def __init__(self, K: int, A: sp.csc_matrix, b, is_eq, is_int, var_lb, var_ub, w, *,
                  device: int = 0):

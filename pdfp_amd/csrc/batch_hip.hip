@@ -1,4 +1,4 @@
-// libpdfp_hip: a batch of up to 8 projector trajectories sharing one matrix.
+// libpdfp_hip: a batch of up to 16 projector trajectories sharing one matrix.
 //
 // The reference runs concurrent trajectories as separate OS processes, each
 // with its own PDFPState on the same GPU (scripts/solve.jl:121-148).  Here
@@ -28,15 +28,16 @@ namespace pdfp {
 
 // The one-workgroup block (small_kernels.h) for the trajectories of a batch
 // whose instance fits in LDS: workgroup b runs the trajectory of args[b] on
-// its view (stride 8) of the interleaved vectors.
-using SmallBatchArgs = SmallArgsT<EpiBatchEvalCol, EpiBatchEvalRow>;
+// its view (stride KB) of the interleaved vectors.
+template <int KB>
+using SmallBatchArgs = SmallArgsT<EpiBatchEvalCol<KB>, EpiBatchEvalRow<KB>>;
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_small_batch(const SmallBatchArgs* __restrict__ args) {
+template <int BLOCK, int KB>
+__global__ __launch_bounds__(BLOCK) void k_small_batch(const SmallBatchArgs<KB>* __restrict__ args) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
-  const SmallBatchArgs a = args[blockIdx.x];
-  small_block_body<BLOCK, kBatch>(a, smem, red);
+  const SmallBatchArgs<KB> a = args[blockIdx.x];
+  small_block_body<BLOCK, KB>(a, smem, red);
 }
 
 struct Trajectory {
@@ -53,8 +54,34 @@ struct Trajectory {
   bool l1_den_pending = false;  // summed on the host while the next block runs
 };
 
-struct Batch {
-  int K;
+// What the C ABI sees of a batch; Batch<KB> implements it for a batch width
+// KB of 8 or 16 (pdfp_batch_create picks the narrowest that holds K).
+struct BatchBase {
+  int K = 0;
+  std::vector<Trajectory> tr;
+  bool graph_on = true;
+  virtual ~BatchBase() = default;
+  virtual void set_original_primal_residual(const double* D_r) = 0;
+  virtual void set_x(int k, const double* xs) = 0;
+  virtual void set_target(int k, const double* xts) = 0;
+  virtual void warm_start(int k) = 0;
+  virtual void begin(int k, const pdfp_params& p, double pw) = 0;
+  virtual int run(int mode) = 0;
+  virtual void solve(const pdfp_params& p, const double* pw, const uint8_t* act) = 0;
+  virtual void get_x(int k, double* out) = 0;
+  virtual void get_vector(int k, int which, double* out) = 0;
+  virtual void info(int k, pdfp_info* o) = 0;
+  virtual void time_kernels(int iters, double* ms_at, double* ms_a) = 0;
+  virtual double time_iterations(int iters) = 0;
+  virtual void set_schedule(int npb_a, int npb_at, int persist, int lane_a, int lane_at) = 0;
+  virtual void set_path(int p) = 0;
+  virtual void drop_graphs() = 0;
+};
+
+template <int KB>
+struct Batch final : BatchBase {
+  static constexpr int kBatch = KB;
+  static constexpr int kEvalRows = 256 / KB;
   int64_t m, n, nnz;
   hipStream_t stream = nullptr;
   DevCsr<float> A, At;
@@ -90,7 +117,6 @@ struct Batch {
     int freq = 0;
   };
   BlockGraph graphs[2];
-  bool graph_on = true;
   // The instance fits in LDS: each block runs as one launch with one
   // workgroup per running trajectory, each iterating as the single FP32
   // projector's one-workgroup path; path 1 forces the batched passes.
@@ -100,14 +126,13 @@ struct Batch {
   bool small_used = false;
   DevBuf<StepParams> d_ssp;
   PinnedBuf<StepParams> h_ssp;
-  DevBuf<SmallBatchArgs> d_sargs;
-  PinnedBuf<SmallBatchArgs> h_sargs;
+  DevBuf<SmallBatchArgs<KB>> d_sargs;
+  PinnedBuf<SmallBatchArgs<KB>> h_sargs;
   PinnedBuf<double> h_sums;
   std::vector<double> h_w, h_b;
   double b_norm = 0.0, b_norm_original = NAN, dual_den = 1.0;
   bool orig_primal = false;
   double sv = NAN, step_size = 0.0;
-  std::vector<Trajectory> tr;
   int x64_owner = -1;  // the trajectory whose polished point x64 holds, or -1
   std::string err = "no error";
 
@@ -115,7 +140,8 @@ struct Batch {
         const double* nzval, int32_t base, const double* b_, const uint8_t* is_eq_,
         const uint8_t* is_int_, const double* var_lb, const double* var_ub, const double* w_,
         int device)
-      : K(K_), m(m_), n(n_) {
+      : m(m_), n(n_) {
+    K = K_;
     PDFP_HIP(hipSetDevice(device));
     PDFP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HostCsr ha = csc_to_csr(m, n, colptr, rowval, nzval, base);
@@ -216,7 +242,7 @@ struct Batch {
 
   // The captured blocks hold kernel arguments (buffers, schedules, flags):
   // dropped whenever one of them changes.
-  void drop_graphs() {
+  void drop_graphs() override {
     for (BlockGraph& g : graphs) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
       g = BlockGraph{};
@@ -233,18 +259,18 @@ struct Batch {
     std::vector<float> tmp((size_t)std::max<int64_t>(len, 1));
     for (int64_t i = 0; i < len; ++i) tmp[(size_t)i] = (float)src[i];
     tmpf.upload(tmp.data(), (size_t)len, stream);
-    if (len) k_batch_scatter<<<grid_for(len), 256, 0, stream>>>((int)len, k, tmpf.p, dst.p);
+    if (len) k_batch_scatter<KB><<<grid_for(len), 256, 0, stream>>>((int)len, k, tmpf.p, dst.p);
     sync();
   }
   void gather(const DevBuf<float>& src, int64_t len, int k, double* out) {
-    if (len) k_batch_gather<<<grid_for(len), 256, 0, stream>>>((int)len, k, src.p, tmpf.p);
+    if (len) k_batch_gather<KB><<<grid_for(len), 256, 0, stream>>>((int)len, k, src.p, tmpf.p);
     std::vector<float> tmp((size_t)std::max<int64_t>(len, 1));
     tmpf.download(tmp.data(), (size_t)len, stream);
     sync();
     for (int64_t i = 0; i < len; ++i) out[i] = (double)tmp[(size_t)i];
   }
 
-  void set_original_primal_residual(const double* D_r) {
+  void set_original_primal_residual(const double* D_r) override {
     std::vector<double> inv((size_t)m);
     double acc = 0.0;
     for (int64_t i = 0; i < m; ++i) {
@@ -260,16 +286,16 @@ struct Batch {
     orig_primal = true;
   }
 
-  void set_x(int k, const double* xs) {
+  void set_x(int k, const double* xs) override {
     check_k(k);
     scatter(x, n, k, xs);
     scatter(x0, n, k, xs);
   }
 
-  void warm_start(int k) {
+  void warm_start(int k) override {
     check_k(k);
     const int64_t len = std::max(n, m);
-    if (len) k_batch_warm<<<grid_for(len), 256, 0, stream>>>((int)n, (int)m, k, x.p, y.p, z.p,
+    if (len) k_batch_warm<KB><<<grid_for(len), 256, 0, stream>>>((int)n, (int)m, k, x.p, y.p, z.p,
                                                               x0.p, y0.p, z0.p);
     Trajectory& t = tr[(size_t)k];
     t.r.inner_count = 0;
@@ -294,7 +320,7 @@ struct Batch {
 
   // set_target!  src/pdfp.jl:98-101: one copy into pinned memory, then an
   // asynchronous upload and the narrowing and interleaving on the device.
-  void set_target(int k, const double* xts) {
+  void set_target(int k, const double* xts) override {
     check_k(k);
     sync();  // no copy of an earlier target may still read h_xt
     if (n) {
@@ -302,7 +328,7 @@ struct Batch {
       double* d = xt64.p + (size_t)k * (size_t)n;
       std::memcpy(h, xts, sizeof(double) * (size_t)n);
       PDFP_HIP(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, stream));
-      k_batch_scatter64<<<grid_for(n), 256, 0, stream>>>((int)n, k, d, xt.p);
+      k_batch_scatter64<KB><<<grid_for(n), 256, 0, stream>>>((int)n, k, d, xt.p);
       PDFP_HIP(hipGetLastError());
     }
     if (x64_owner == k) x64_owner = -1;
@@ -330,11 +356,11 @@ struct Batch {
 
   template <bool MAJOR>
   void step_t(int k_offset) {
-    EpiBatchPrimalY<MAJOR> ex{d_sp.p, d_wt.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+    EpiBatchPrimalY<KB, MAJOR> ex{d_sp.p, d_wt.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                               rx.p, px.p, ds.p, py.p, ry.p};
-    launch_batch(At, GatherBatch{z.p}, ex);
-    EpiBatchDualZ<MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-    launch_batch(A, GatherBatch{rx.p}, ez);
+    launch_batch(At, GatherBatch<KB>{z.p}, ex);
+    EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    launch_batch(A, GatherBatch<KB>{rx.p}, ez);
   }
 
   // Schedule autotuner of the batched passes: time whole iterations with
@@ -403,7 +429,7 @@ struct Batch {
     (void)hipEventDestroy(e1);
   }
 
-  void set_schedule(int npb_a, int npb_at, int persist, int lane_a, int lane_at) {
+  void set_schedule(int npb_a, int npb_at, int persist, int lane_a, int lane_at) override {
     constexpr int kRows = kBlock / kBatch;
     if ((npb_a != 256 && npb_a != 512) || (npb_at != 256 && npb_at != 512) || persist < 0 ||
         persist > 16 || lane_a < 0 || lane_at < 0)
@@ -427,10 +453,10 @@ struct Batch {
   void evaluate_all(const int32_t* mask, int slot) {
     const int nbt = (int)((n + kEvalRows - 1) / kEvalRows);
     const int nba = (int)((m + kEvalRows - 1) / kEvalRows);
-    EpiBatchEvalCol ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, 0};
-    EpiBatchEvalRow er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr, 0};
+    EpiBatchEvalCol<KB> ec{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, 0};
+    EpiBatchEvalRow<KB> er{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr, 0};
     if (nbt) {
-      k_batch_eval_col<<<nbt, 256, 0, stream>>>((int)n, At.rowptr.p, At.col.p, At.val.p, pz.p,
+      k_batch_eval_col<KB><<<nbt, 256, 0, stream>>>((int)n, At.rowptr.p, At.col.p, At.val.p, pz.p,
                                                  rz.p, ec, mask, part_t.p);
       k_batch_eval_finalize<<<kBatch * kEvalColTerms, 256, 0, stream>>>(
           part_t.p, nbt, kEvalColTerms, mask, slot * 16, sums.p);
@@ -438,13 +464,13 @@ struct Batch {
     if (nba && A.nblocks > 0) {
       if (part_a.n < (size_t)A.nblocks * kBatch * kEvalRowTerms)
         throw Error(PDFP_ERR_STATE, "row-term partials smaller than the schedule");
-      EpiBatchEvalRowLanes el{er, mask};
+      EpiBatchEvalRowLanes<KB> el{er, mask};
       if (A.max_rows > kBlock / kBatch)
         throw Error(PDFP_ERR_STATE, "batch schedule holds too many rows per block");
       if (A.npb == 512)
-        launch_pass_npb<512, kEvalRowTerms, 0>(A, GatherBatch{px.p}, el, part_a.p, stream);
+        launch_pass_npb<512, kEvalRowTerms, 0>(A, GatherBatch<KB>{px.p}, el, part_a.p, stream);
       else
-        launch_pass_npb<256, kEvalRowTerms, 0>(A, GatherBatch{px.p}, el, part_a.p, stream);
+        launch_pass_npb<256, kEvalRowTerms, 0>(A, GatherBatch<KB>{px.p}, el, part_a.p, stream);
       k_batch_eval_finalize<<<kBatch * kEvalRowTerms, 256, 0, stream>>>(
           part_a.p, A.nblocks, kEvalRowTerms, mask, slot * 16 + 8, sums.p);
     }
@@ -514,7 +540,7 @@ struct Batch {
       if (!active[(size_t)q]) continue;
       const Trajectory& t = tr[(size_t)q];
       h_ssp.p[q] = StepParams{step_size, t.r.primal_weight, omega, (double)t.r.inner_count};
-      SmallBatchArgs& a = h_sargs.p[na++];
+      SmallBatchArgs<KB>& a = h_sargs.p[na++];
       a.A = SmallCsr{A.rowptr.p, A.col.p, A.val.p};
       a.At = SmallCsr{At.rowptr.p, At.col.p, At.val.p};
       a.m = (int)m;
@@ -544,24 +570,24 @@ struct Batch {
       a.ry = ry.p + q;
       a.pz = pz.p + q;
       a.rz = rz.p + q;
-      a.ec = EpiBatchEvalCol{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, q};
-      a.er = EpiBatchEvalRow{pz.p, rz.p, b.p, z0.p, is_eq.p,
+      a.ec = EpiBatchEvalCol<KB>{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p, q};
+      a.er = EpiBatchEvalRow<KB>{pz.p, rz.p, b.p, z0.p, is_eq.p,
                              orig_primal ? row_unscale.p : nullptr, q};
       a.sums = sums.p + (size_t)q * 32;
     }
     if (na == 0) return;
     PDFP_HIP(hipMemcpyAsync(d_ssp.p, h_ssp.p, sizeof(StepParams) * kBatch,
                             hipMemcpyHostToDevice, stream));
-    PDFP_HIP(hipMemcpyAsync(d_sargs.p, h_sargs.p, sizeof(SmallBatchArgs) * (size_t)na,
+    PDFP_HIP(hipMemcpyAsync(d_sargs.p, h_sargs.p, sizeof(SmallBatchArgs<KB>) * (size_t)na,
                             hipMemcpyHostToDevice, stream));
-    k_small_batch<kSmallBlock>
+    k_small_batch<kSmallBlock, KB>
         <<<na, kSmallBlock, (size_t)small_lds_bytes(m, n, nnz), stream>>>(d_sargs.p);
     PDFP_HIP(hipGetLastError());
     PDFP_HIP(hipMemcpyAsync(h_sums.p, sums.p, sizeof(double) * (size_t)kBatch * 32,
                             hipMemcpyDeviceToHost, stream));
   }
 
-  void set_path(int p) {
+  void set_path(int p) override {
     if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
     path = p;
   }
@@ -611,7 +637,7 @@ struct Batch {
 
   // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for
   // trajectory k: it starts a solve with its own parameters.
-  void begin(int k, const pdfp_params& p, double pw) {
+  void begin(int k, const pdfp_params& p, double pw) override {
     check_k(k);
     if (p.termination_evaluation_frequency < 1)
       throw Error(PDFP_ERR_ARGUMENT, "termination_evaluation_frequency must be >= 1");
@@ -651,7 +677,7 @@ struct Batch {
   // its own termination; returns after the first block in which one ends
   // (until_any) or when none runs.  Returns the number still running.
   // mode: 0 until none runs, 1 until one ends, 2 after one block.
-  int run(int mode) {
+  int run(int mode) override {
     int freq = 0;
     double omega = 0.0;
     for (const Trajectory& t : tr) {
@@ -731,7 +757,7 @@ struct Batch {
             update_primal_weight(t.r, p, e.primal_dist, e.dual_dist);
             const int64_t len = std::max(n, m);
             if (len)
-              k_batch_restart<<<grid_for(len), 256, 0, stream>>>(
+              k_batch_restart<KB><<<grid_for(len), 256, 0, stream>>>(
                   (int)n, (int)m, k, px.p, py.p, pz.p, x.p, x0.p, y.p, y0.p, z.p, z0.p);
             t.r.inner_count = 0;
             t.r.last_trial_fixed_point_error = INFINITY;
@@ -758,7 +784,7 @@ struct Batch {
 
   // pdfp_solve! (src/pdfp.jl:252-288) for every trajectory with active[k]:
   // each begins with primal weight pw[k], then they run to their ends.
-  void solve(const pdfp_params& p, const double* pw, const uint8_t* act) {
+  void solve(const pdfp_params& p, const double* pw, const uint8_t* act) override {
     for (Trajectory& t : tr) t.running = false;
     for (int k = 0; k < K; ++k)
       if (!act || act[k]) begin(k, p, pw[k]);
@@ -769,7 +795,7 @@ struct Batch {
     Trajectory& t = tr[(size_t)k];
     if (x64_owner == k || n == 0) return;
     const int g = grid_for(n);
-    k_batch_polish<<<g, 256, 0, stream>>>((int)n, k, px.p, lb.p, ub.p, lb64.p, ub64.p,
+    k_batch_polish<KB><<<g, 256, 0, stream>>>((int)n, k, px.p, lb.p, ub.p, lb64.p, ub64.p,
                                           xt64.p + (size_t)k * (size_t)n, w64.p, x64.p,
                                           part_v.p);
     finalize_partials<256><<<1, 256, 0, stream>>>(part_v.p, g, 1, 0, sums.p + kBatch * 32);
@@ -780,14 +806,14 @@ struct Batch {
     x64_owner = k;
   }
 
-  void get_x(int k, double* out) {
+  void get_x(int k, double* out) override {
     check_k(k);
     polish(k);
     if (n) x64.download(out, (size_t)n, stream);
     sync();
   }
 
-  void get_vector(int k, int which, double* out) {
+  void get_vector(int k, int which, double* out) override {
     check_k(k);
     switch (which) {
       case 0: get_x(k, out); break;
@@ -802,7 +828,7 @@ struct Batch {
     }
   }
 
-  void info(int k, pdfp_info* o) {
+  void info(int k, pdfp_info* o) override {
     check_k(k);
     Trajectory& t = tr[(size_t)k];
     if (t.termination != PDFP_UNSPECIFIED) polish(k);
@@ -834,18 +860,18 @@ struct Batch {
 
   // Mean duration of the transpose pass and of the A pass of a batched
   // iteration, each bracketed by its own events on the batch's stream.
-  void time_kernels(int iters, double* ms_at, double* ms_a) {
+  void time_kernels(int iters, double* ms_at, double* ms_a) override {
     time_iterations(1);
     std::vector<hipEvent_t> ev((size_t)iters * 4);
     for (auto& e : ev) PDFP_HIP(hipEventCreate(&e));
     for (int i = 0; i < iters; ++i) {
       const int k = 2 + (i % 198);
-      EpiBatchPrimalY<false> ex{d_sp.p, d_wt.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+      EpiBatchPrimalY<KB, false> ex{d_sp.p, d_wt.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                                 rx.p, px.p, ds.p, py.p, ry.p};
-      launch_batch(At, GatherBatch{z.p}, ex,
+      launch_batch(At, GatherBatch<KB>{z.p}, ex,
           ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
-      EpiBatchDualZ<false> ez{d_sp.p, d_wt.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-      launch_batch(A, GatherBatch{rx.p}, ez,
+      EpiBatchDualZ<KB, false> ez{d_sp.p, d_wt.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+      launch_batch(A, GatherBatch<KB>{rx.p}, ez,
           ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
     }
     sync();
@@ -862,7 +888,7 @@ struct Batch {
     *ms_a = sb / std::max(iters, 1);
   }
 
-  double time_iterations(int iters) {
+  double time_iterations(int iters) override {
     if (std::isnan(sv) && nnz > 0) {
       pdfp_params p;
       pdfp_default_params(&p);
@@ -891,7 +917,7 @@ struct Batch {
 }  // namespace pdfp
 
 struct pdfp_batch {
-  std::unique_ptr<pdfp::Batch> impl;
+  std::unique_ptr<pdfp::BatchBase> impl;
   std::string err = "no error";
 };
 
@@ -935,7 +961,7 @@ int pdfp_batch_create(pdfp_batch** out, int32_t K, int64_t m, int64_t n, int64_t
     pdfp::g_create_error = msg;
     return code;
   };
-  if (K < 1 || K > pdfp::kBatch) return fail(PDFP_ERR_ARGUMENT, "K must be in 1..8");
+  if (K < 1 || K > 16) return fail(PDFP_ERR_ARGUMENT, "K must be in 1..16");
   if (m < 0 || n < 0 || nnz < 0) return fail(PDFP_ERR_ARGUMENT, "negative size");
   if (!colptr || (nnz > 0 && (!rowval || !nzval)) || (m > 0 && (!b || !is_eq)) ||
       (n > 0 && (!is_int || !var_lb || !var_ub || !w)))
@@ -949,8 +975,13 @@ int pdfp_batch_create(pdfp_batch** out, int32_t K, int64_t m, int64_t n, int64_t
   auto* ctx = new (std::nothrow) pdfp_batch();
   if (!ctx) return fail(PDFP_ERR_NOMEM, "host allocation failed");
   try {
-    ctx->impl.reset(new pdfp::Batch(K, m, n, colptr, rowval, nzval, index_base, b, is_eq, is_int,
-                                    var_lb, var_ub, w, device));
+    // the narrowest batch width that holds K trajectories
+    if (K <= 8)
+      ctx->impl.reset(new pdfp::Batch<8>(K, m, n, colptr, rowval, nzval, index_base, b, is_eq,
+                                         is_int, var_lb, var_ub, w, device));
+    else
+      ctx->impl.reset(new pdfp::Batch<16>(K, m, n, colptr, rowval, nzval, index_base, b, is_eq,
+                                          is_int, var_lb, var_ub, w, device));
   } catch (const pdfp::Error& e) {
     delete ctx;
     return fail(e.code, e.what());

@@ -1,11 +1,13 @@
-// Shared-matrix batch of up to 8 projector trajectories (FP32 iterates).
+// Shared-matrix batch of up to KB projector trajectories (FP32 iterates);
+// the batch width KB is 8 or 16.
 //
-// K trajectories of one instance — the same A, b, bounds and weights, each
-// with its own target x̃_k, iterates, primal weight and restart state — run
-// their PDHG iterations together: every vector is stored interleaved as
-// v[j·8 + k], so a nonzero's gather fetches the 8 trajectories' entries of
-// column j in one 32-byte request and the matrix is streamed once for all
-// of them.  The per-trajectory scalars (step, primal weight, inner count,
+// K ≤ KB trajectories of one instance — the same A, b, bounds and weights,
+// each with its own target x̃_k, iterates, primal weight and restart state —
+// run their PDHG iterations together: every vector is stored interleaved as
+// v[j·KB + k], so a nonzero's gather fetches the KB trajectories' entries of
+// column j in one request of 4·KB bytes (32 or 64 of the 128-byte line a
+// random gather fills anyway) and the matrix is streamed once for all of
+// them.  The per-trajectory scalars (step, primal weight, inner count,
 // active flag) live in device memory; an inactive trajectory's iterates are
 // left untouched, so it keeps the result of its own last solve.
 #pragma once
@@ -15,12 +17,10 @@
 
 namespace pdfp {
 
-constexpr int kBatch = 8;
-
 // Per-trajectory scalars of the current evaluation block (FP64, formed on
 // the host with the same operations as step_scalars: τ = step/pw,
 // σ = step·pw, 1 - ω), and a table of the Halpern weights of the block's
-// iterations: wt[k_offset·8 + q] = {w_k, 1 - w_k}, w_k = k/(k+1),
+// iterations: wt[k_offset·KB + q] = {w_k, 1 - w_k}, w_k = k/(k+1),
 // k = inner_count_q + k_offset (src/pdfp.jl:120-122).  No division is left
 // for the epilogues.
 struct BatchStep {
@@ -32,6 +32,7 @@ struct BatchStep {
   double pad[3];
 };
 
+template <int KB>
 __device__ __forceinline__ StepScalars batch_scalars(const BatchStep& sp,
                                                      const double2* __restrict__ wt,
                                                      int k_offset, int q) {
@@ -40,37 +41,30 @@ __device__ __forceinline__ StepScalars batch_scalars(const BatchStep& sp,
   s.sigma = sp.sigma;
   s.omega = sp.omega;
   s.omw = sp.omw;
-  const double2 w = wt[(size_t)k_offset * 8 + q];
+  const double2 w = wt[(size_t)k_offset * KB + q];
   s.weight = w.x;
   s.omwt = w.y;
   return s;
 }
 
-struct F8 {
-  float v[kBatch];
-};
-
-__device__ __forceinline__ F8 load8(const float* p) {
-  const float4 a = reinterpret_cast<const float4*>(p)[0];
-  const float4 b = reinterpret_cast<const float4*>(p)[1];
-  return F8{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
-}
-
-__device__ __forceinline__ void store8(float* p, const F8& f) {
-  reinterpret_cast<float4*>(p)[0] = make_float4(f.v[0], f.v[1], f.v[2], f.v[3]);
-  reinterpret_cast<float4*>(p)[1] = make_float4(f.v[4], f.v[5], f.v[6], f.v[7]);
-}
-
-// p_k = a · x[c·8 + k]  for the 8 trajectories (one 32-byte gather); the
-// pass stages the 8 FP32 values (raw) rather than 8 FP64 products.
+// p_k = a · x[c·KB + k]  for the KB trajectories (one gather of 4·KB
+// bytes); the pass stages the FP32 values (raw) rather than FP64 products.
+template <int KB>
 struct GatherBatch {
-  static constexpr int NV = kBatch;
+  static constexpr int kBatch = KB;
+  static constexpr int NV = KB;
   static constexpr bool kRawStage = true;
   const float* __restrict__ x;
   __device__ __forceinline__ void operator()(int c, double a, double* p) const {
-    const F8 f = load8(x + (size_t)c * kBatch);
+    const float4* v = reinterpret_cast<const float4*>(x + (size_t)c * kBatch);
 #pragma unroll
-    for (int q = 0; q < kBatch; ++q) p[q] = a * (double)f.v[q];
+    for (int h = 0; h < KB / 4; ++h) {
+      const float4 f = v[h];
+      p[4 * h + 0] = a * (double)f.x;
+      p[4 * h + 1] = a * (double)f.y;
+      p[4 * h + 2] = a * (double)f.z;
+      p[4 * h + 3] = a * (double)f.w;
+    }
   }
   __device__ __forceinline__ float4 raw4(int c, int h) const {
     return reinterpret_cast<const float4*>(x + (size_t)c * kBatch)[h];
@@ -85,20 +79,22 @@ struct BatchScalars {
   double active;  // 1: iterate this trajectory, 0: leave it unchanged
 };
 
+template <int KB>
 __device__ __forceinline__ BatchScalars load_batch_scalars(const BatchStep* __restrict__ sp,
                                                            const double2* __restrict__ wt,
                                                            int k_offset, int q) {
   const BatchStep& st = sp[q];
-  return {batch_scalars(st, wt, k_offset, q), st.active};
+  return {batch_scalars<KB>(st, wt, k_offset, q), st.active};
 }
 
-// Column j, trajectory q (lane q of the row's 8): the fused primal and dual-y
+// Column j, trajectory q (lane q of the row's KB): the fused primal and dual-y
 // step (src/pdfp.jl:119-143) with s = (Aᵀ z_q)_j.  An inactive trajectory
 // is left untouched.  MAJOR steps also store pdhg x, the dual slack, pdhg y
 // and reflected y.
-template <bool MAJOR>
+template <int KB, bool MAJOR>
 struct EpiBatchPrimalY {
-  static constexpr int kLanes = kBatch;
+  static constexpr int kBatch = KB;
+  static constexpr int kLanes = KB;
   struct Pre {
     float x, y, x0, y0, xt, lb, ub, w;
   };
@@ -124,7 +120,7 @@ struct EpiBatchPrimalY {
   }
   __device__ __forceinline__ void operator()(int j, int q, const Pre& p, double s,
                                              double*) const {
-    const BatchScalars sh = load_batch_scalars(sp, wt, k_offset, q);
+    const BatchScalars sh = load_batch_scalars<KB>(sp, wt, k_offset, q);
     const size_t o = (size_t)j * kBatch + q;
     const PrimalYOut r = primal_y_math(sh.c, s, (double)p.x, (double)p.y, (double)p.x0,
                                        (double)p.y0, (double)p.lb, (double)p.ub, (double)p.xt,
@@ -146,9 +142,10 @@ struct EpiBatchPrimalY {
 
 // Row i, trajectory q: the dual-z step (src/pdfp.jl:146-156) with
 // s = (A x̄_q)_i.
-template <bool MAJOR>
+template <int KB, bool MAJOR>
 struct EpiBatchDualZ {
-  static constexpr int kLanes = kBatch;
+  static constexpr int kBatch = KB;
+  static constexpr int kLanes = KB;
   struct Pre {
     float z, z0, b;
     uint32_t eq;
@@ -168,7 +165,7 @@ struct EpiBatchDualZ {
   }
   __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
                                              double*) const {
-    const BatchScalars sh = load_batch_scalars(sp, wt, k_offset, q);
+    const BatchScalars sh = load_batch_scalars<KB>(sp, wt, k_offset, q);
     const size_t o = (size_t)i * kBatch + q;
     const DualZOut r = dual_z_math(sh.c, s, (double)p.z, (double)p.z0, (double)p.b, p.eq != 0);
     if (sh.active == 0.0) return;
@@ -183,7 +180,9 @@ struct EpiBatchDualZ {
 // ---- the evaluation terms of trajectory k (views of the interleaved vectors) --
 
 // The terms of EpiEvalCol (pdfp_kernels.h) for trajectory k.
+template <int KB>
 struct EpiBatchEvalCol {
+  static constexpr int kBatch = KB;
   struct Pre {
     float px, rx, py, ry, ds, xt, w, x0, y0;
   };
@@ -217,7 +216,9 @@ struct EpiBatchEvalCol {
 };
 
 // The terms of EpiEvalRow (pdfp_kernels.h) for trajectory k.
+template <int KB>
 struct EpiBatchEvalRow {
+  static constexpr int kBatch = KB;
   struct Pre {
     float pz, rz, b, z0;
     uint8_t eq;
@@ -248,45 +249,47 @@ struct EpiBatchEvalRow {
 
 // The row terms of every trajectory in the mask (*mask, a device word the
 // host writes before the launch, so a captured launch serves any mask) as a
-// lane epilogue of the CSR pass over A (gathering the 8 trajectories' pdhg
+// lane epilogue of the CSR pass over A (gathering the KB trajectories' pdhg
 // x): lane (row, q) adds trajectory q's terms; the pass writes
 // per-trajectory partials.
+template <int KB>
 struct EpiBatchEvalRowLanes {
-  static constexpr int kLanes = kBatch;
-  using Pre = EpiBatchEvalRow::Pre;
-  EpiBatchEvalRow er;
+  static constexpr int kLanes = KB;
+  using Pre = typename EpiBatchEvalRow<KB>::Pre;
+  EpiBatchEvalRow<KB> er;
   const int32_t* mask;
   __device__ __forceinline__ Pre pre(int i, int q) const {
-    EpiBatchEvalRow e = er;
+    EpiBatchEvalRow<KB> e = er;
     e.k = q;
     return e.pre(i);
   }
   __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
                                              double* acc) const {
     if (!((*mask >> q) & 1)) return;
-    EpiBatchEvalRow e = er;
+    EpiBatchEvalRow<KB> e = er;
     e.k = q;
     e(i, p, &s, acc);
   }
 };
 
 // ---- evaluation of several trajectories in one pass ---------------------------
-// The column terms: one workgroup serves 32 consecutive columns (rows of Aᵀ);
-// lane (column, q) = (tid / 8, tid % 8) forms trajectory q's two sums over
+// The column terms: one workgroup serves 256/KB consecutive columns (rows of
+// Aᵀ); lane (column, q) = (tid / KB, tid % KB) forms trajectory q's two sums over
 // the column in FP64, in ascending row order (the second over the difference
 // pz - rz, formed first as the reference does), and adds q's terms
 // (EpiBatchEvalCol) when bit q of *mask is set.  Each workgroup writes
-// per-trajectory partials partial[(q·NT + t)·nblocks + block]; the 8
-// trajectories share every 32-byte gather.  (The row terms run as a lane
+// per-trajectory partials partial[(q·NT + t)·nblocks + block]; the KB
+// trajectories share every gather.  (The row terms run as a lane
 // epilogue of the CSR pass over A, EpiBatchEvalRowLanes, whose schedule
 // spreads long rows over the workgroup; columns are short in the pump's
 // instances, and a dense column costs one lane loop here.)
-constexpr int kEvalRows = 256 / kBatch;
-
-static __global__ __launch_bounds__(256) void k_batch_eval_col(
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_eval_col(
     int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const float* __restrict__ pz, const float* __restrict__ rz,
-    EpiBatchEvalCol ec, const int32_t* __restrict__ maskp, double* __restrict__ partial) {
+    EpiBatchEvalCol<KB> ec, const int32_t* __restrict__ maskp, double* __restrict__ partial) {
+  constexpr int kBatch = KB;
+  constexpr int kEvalRows = 256 / KB;
   const int mask = *maskp;
   const int q = threadIdx.x % kBatch;
   const int j = blockIdx.x * kEvalRows + threadIdx.x / kBatch;
@@ -325,7 +328,8 @@ static __global__ __launch_bounds__(256) void k_batch_eval_finalize(
 }
 
 // perform_restart! for trajectory k: initial ← pdhg, current ← pdhg.
-static __global__ __launch_bounds__(256) void k_batch_restart(int n, int m, int k,
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_restart(int n, int m, int k,
                                                               const float* __restrict__ px,
                                                               const float* __restrict__ py,
                                                               const float* __restrict__ pz,
@@ -335,6 +339,7 @@ static __global__ __launch_bounds__(256) void k_batch_restart(int n, int m, int 
                                                               float* __restrict__ y0,
                                                               float* __restrict__ z,
                                                               float* __restrict__ z0) {
+  constexpr int kBatch = KB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const size_t o = (size_t)i * kBatch + k;
@@ -351,13 +356,15 @@ static __global__ __launch_bounds__(256) void k_batch_restart(int n, int m, int 
 }
 
 // warm_start! for trajectory k: initial ← current.
-static __global__ __launch_bounds__(256) void k_batch_warm(int n, int m, int k,
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_warm(int n, int m, int k,
                                                            const float* __restrict__ x,
                                                            const float* __restrict__ y,
                                                            const float* __restrict__ z,
                                                            float* __restrict__ x0,
                                                            float* __restrict__ y0,
                                                            float* __restrict__ z0) {
+  constexpr int kBatch = KB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const size_t o = (size_t)i * kBatch + k;
@@ -371,32 +378,39 @@ static __global__ __launch_bounds__(256) void k_batch_warm(int n, int m, int k,
 }
 
 // Writes column k of an interleaved vector from a dense FP32 vector.
-static __global__ __launch_bounds__(256) void k_batch_scatter(int len, int k,
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_scatter(int len, int k,
                                                              const float* __restrict__ in,
                                                              float* __restrict__ out) {
+  constexpr int kBatch = KB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) out[(size_t)i * kBatch + k] = in[i];
 }
 
 // Writes trajectory k's FP64 target, narrowed to FP32, into the interleaved
 // vector (the same rounding as a host conversion).
-static __global__ __launch_bounds__(256) void k_batch_scatter64(int len, int k,
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_scatter64(int len, int k,
                                                                const double* __restrict__ in,
                                                                float* __restrict__ out) {
+  constexpr int kBatch = KB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) out[(size_t)i * kBatch + k] = (float)in[i];
 }
 
 // Reads column k of an interleaved vector into a dense FP32 vector.
-static __global__ __launch_bounds__(256) void k_batch_gather(int len, int k,
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_gather(int len, int k,
                                                             const float* __restrict__ in,
                                                             float* __restrict__ out) {
+  constexpr int kBatch = KB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < len) out[i] = in[(size_t)i * kBatch + k];
 }
 
 // FP64 polish of trajectory k's projection (see k_polish).
-static __global__ __launch_bounds__(256) void k_batch_polish(int n, int k,
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_polish(int n, int k,
                                                              const float* __restrict__ px,
                                                              const float* __restrict__ lb,
                                                              const float* __restrict__ ub,
@@ -406,6 +420,7 @@ static __global__ __launch_bounds__(256) void k_batch_polish(int n, int k,
                                                              const double* __restrict__ w64,
                                                              double* __restrict__ x64,
                                                              double* __restrict__ partials) {
+  constexpr int kBatch = KB;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   double acc[1] = {0.0};
   if (j < n) {

@@ -253,9 +253,14 @@ template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, in
 __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, const Epi& epi,
                                            double* __restrict__ partials, const int b) {
   constexpr int NV = Gather::NV;
-  constexpr int U = (NPB / BLOCK) < 4 ? (NPB / BLOCK > 0 ? NPB / BLOCK : 1) : 4;
-  constexpr int NA = NACC > 0 ? NACC : 1;
   constexpr bool kRaw = raw_stage<Gather>::value;
+  // sweeps of the slice in flight per lane; a gather of more than 8 values
+  // holds twice the registers per sweep, so it keeps half as many in flight
+  // (more waves per SIMD instead)
+  constexpr int U0 = (NPB / BLOCK) < 4 ? (NPB / BLOCK > 0 ? NPB / BLOCK : 1) : 4;
+  constexpr int U = (kRaw && NV > 8) ? (U0 > 1 ? U0 / 2 : 1) : U0;
+  constexpr int UV = NV > 8 ? 2 : 4;  // the same for the long-row loop
+  constexpr int NA = NACC > 0 ? NACC : 1;
   static_assert(!kRaw || (std::is_same<TV, float>::value && NV % 4 == 0),
                 "raw staging: FP32 matrix values, a multiple of 4 gathered values");
   // products: double[NV][NPB + 1]; raw: float[NPB + 1][NV] values then
@@ -455,11 +460,11 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
     double s[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) s[q] = 0.0;
-    for (int k0 = 0; k0 < cnt; k0 += BLOCK * 4) {
-      int cc[4];
-      TV vv[4];
+    for (int k0 = 0; k0 < cnt; k0 += BLOCK * UV) {
+      int cc[UV];
+      TV vv[UV];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < UV; ++u) {
         const int k = k0 + u * BLOCK + tid;
         const int kc = k < cnt ? k : cnt - 1;
         cc[u] = ld_stream(A.col + base + kc);
@@ -469,13 +474,13 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
       // sum from +0.0 never becomes -0.0)
       if constexpr (kRaw) {
         // the gathered values stay FP32 in registers until their products
-        float4 gv[4][NV / 4];
+        float4 gv[UV][NV / 4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < UV; ++u)
 #pragma unroll
           for (int h = 0; h < NV / 4; ++h) gv[u][h] = g.raw4(cc[u], h);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < UV; ++u) {
           const bool in = k0 + u * BLOCK + tid < cnt;
           const double a = (double)vv[u];
 #pragma unroll
@@ -487,11 +492,11 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
           }
         }
       } else {
-        double p[4][NV];
+        double p[UV][NV];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) g(cc[u], (double)vv[u], p[u]);
+        for (int u = 0; u < UV; ++u) g(cc[u], (double)vv[u], p[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < UV; ++u) {
           const bool in = k0 + u * BLOCK + tid < cnt;
 #pragma unroll
           for (int q = 0; q < NV; ++q) s[q] += in ? p[u][q] : 0.0;

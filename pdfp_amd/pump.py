@@ -539,8 +539,8 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
     feasibility_pump.  The result is the winning trajectory's, with
     ``trajectory`` and ``seed`` set; its ``time`` is the time to the first
     feasible point of the whole run."""
-    if not 1 <= trajectories <= 8:
-        raise ValueError("trajectories must be in 1..8")
+    if not 1 <= trajectories <= 16:
+        raise ValueError("trajectories must be in 1..16")
     if batch_policy not in ("cost", "always"):
         raise ValueError("batch_policy must be 'cost' or 'always'")
     from . import pdfp as _pdfp_mod

@@ -30,7 +30,7 @@ def _oracle_kkt(pp, xt_scaled, x, y, z, ds):
     return o.residual_of(x, y, z, ds)
 
 
-@pytest.mark.parametrize("K", [8, 3])
+@pytest.mark.parametrize("K", [8, 3, 16, 11])
 def test_batch_trajectories_are_independent_projections(K):
     print("batch", K, flush=True)
     m = G.mixed(400, 800, 31)
@@ -92,14 +92,15 @@ def _with_dense_column(m, value=0.1):
                 name=m.name + "_dense_col")
 
 
-def test_batch_long_rows_and_columns():
+@pytest.mark.parametrize("K", [5, 13])
+def test_batch_long_rows_and_columns(K):
     """Power-law rows (up to 3000 nonzeros) and a dense column: the
-    long-row path of both passes, with 5 trajectories."""
-    print("batch long rows", flush=True)
+    long-row path of both passes, with 5 trajectories (width 8) and 13
+    (width 16)."""
+    print("batch long rows", K, flush=True)
     m = _with_dense_column(G.mixed(2000, 5000, 3, power_law=True, nnz=60_000, max_len=3000))
     pp = projector_problem(m)
     assert np.diff(pp.A.tocsr().indptr).max() > 2048 and np.diff(pp.A.indptr).max() >= 2000
-    K = 5
     tol = 1e-5
     p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
                          iteration_limit=200_000)
@@ -204,15 +205,15 @@ def test_batch_one_block_runs_match_until_any():
     bt.close()
 
 
-def test_batch_schedules_agree():
+@pytest.mark.parametrize("K", [4, 10])
+def test_batch_schedules_agree(K):
     """Every schedule form of the batched passes (nonzeros per workgroup,
     persistent grid, lane-per-(row, trajectory) reduction of short-row
     blocks, up to every stream-mode block) gives the same projections:
     oracle KKT within 10×tol and the single FP32 projector's ℓ1 distance to
-    1e-4, on power-law rows with a dense column."""
+    1e-4, on power-law rows with a dense column; batch widths 8 and 16."""
     m = _with_dense_column(G.mixed(2000, 5000, 5, power_law=True, nnz=60_000, max_len=3000))
     pp = projector_problem(m)
-    K = 4
     tol = 1e-5
     p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
                          iteration_limit=200_000)
@@ -243,17 +244,17 @@ def test_batch_schedules_agree():
             assert abs(l1b - ref[k]) <= 1e-4 * max(1.0, ref[k]), (sched, k, l1b, ref[k])
 
 
-def test_batch_block_graph_replays_equal_stream_launches():
+@pytest.mark.parametrize("K", [8, 16])
+def test_batch_block_graph_replays_equal_stream_launches(K):
     """Blocks replayed as captured graphs give the same trajectories, bit for
     bit, as the same launches on the stream: trajectories that start, end,
     restart and start again at different blocks (so the device masks and
     the step tables change under the replayed graph), each run on a fresh
     batch with the same schedule."""
-    print("batch block graph", flush=True)
+    print("batch block graph", K, flush=True)
     m = G.mixed(2000, 4000, 1)
     pp = projector_problem(m)
-    K = 8
-    tg = [G.binary_targets(m, 700 + k) / pp.D_c for k in range(12)]
+    tg = [G.binary_targets(m, 700 + k) / pp.D_c for k in range(K + 4)]
     loose = SolverParameters(eps_optimal_relative=1e-3, eps_feasible_relative=1e-4)
     tight = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
                              iteration_limit=200_000)
@@ -263,7 +264,7 @@ def test_batch_block_graph_replays_equal_stream_launches():
         bt.set_kernel_path("general")
         bt.set_schedule(256, 256, 0, 32, 32)
         bt.set_block_graph(graph)
-        queue = list(range(12))
+        queue = list(range(K + 4))
         ended = []
         running = [True] * K
         for k in range(K):
@@ -286,7 +287,7 @@ def test_batch_block_graph_replays_equal_stream_launches():
                     running[k] = True
         bt.close()
         runs.append(ended)
-    assert len(runs[0]) == 12
+    assert len(runs[0]) == K + 4
     for run in runs[1:]:
         assert len(run) == len(runs[0])
         for a, b in zip(runs[0], run):
@@ -294,17 +295,17 @@ def test_batch_block_graph_replays_equal_stream_launches():
             assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
 
 
-def test_batch_small_instance_trajectories_equal_single_projector_bitwise():
+@pytest.mark.parametrize("K", [8, 16])
+def test_batch_small_instance_trajectories_equal_single_projector_bitwise(K):
     """When the instance fits in LDS, the batch runs one workgroup per running
     trajectory on its view of the interleaved vectors, each iterating as the
     single FP32 projector's one-workgroup path: every trajectory's
     projections (a cold one, then a warm-started one begun at another
     block) equal the single projector's bit for bit, and the batched passes
     (path forced) reach the same ℓ1 distances to the tolerance."""
-    print("batch small", flush=True)
+    print("batch small", K, flush=True)
     m = read_mps(example_path("p0201"))
     pp = projector_problem(m)
-    K = 8
     tol = 1e-6
     p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
                          iteration_limit=100_000)

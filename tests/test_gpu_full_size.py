@@ -179,3 +179,42 @@ def test_full_size_pack_of_64_uneven_set_covers():
         assert abs(r[3] - l1_64) <= 2 * TOL * _denominator(pp, targets[k]), \
             ("pack", k, r[3], l1_64)
     pk.close()
+
+
+def test_full_size_powerlaw_batch_of_16():
+    """The 5M-nonzero power-law instance with 16 trajectories on one batch
+    (batch width 16): every trajectory's point meets the tolerance under the
+    oracle's FP64 residuals, and two of them agree in ℓ1 with the FP64 form's
+    projections of the same targets within the gap tolerance."""
+    print("full size powerlaw batch of 16", flush=True)
+    milp = WORKLOADS["powerlaw_500000x1000000_5M"]()
+    pp = projector_problem(milp)
+    n = milp.A.shape[1]
+    x_start = np.clip(np.zeros(n), milp.l, milp.u) / pp.D_c
+    pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
+    p = SolverParameters(eps_optimal_relative=TOL, eps_feasible_relative=TOL,
+                         iteration_limit=200_000)
+    kkt = _Kkt(pp)
+    targets = [G.binary_targets(milp, 170 + k) / pp.D_c for k in range(16)]
+    s64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                      precision="fp64")
+    l1_64 = {}
+    for k in (0, 15):
+        s64.set_x(x_start)
+        P.set_target(s64, targets[k])
+        assert P.pdfp_solve(s64, p, pw) == "OPTIMAL"
+        l1_64[k] = s64.info().l1_distance
+    s64.close()
+    bt = PDFPBatch(16, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    for k in range(16):
+        bt.set_x(k, x_start)
+        bt.set_target(k, targets[k])
+    assert bt.solve(p, pw) == ["OPTIMAL"] * 16
+    for k in range(16):
+        r = kkt(targets[k], bt.pdhg_x(k), bt.vector(k, "pdhg_y"), bt.vector(k, "pdhg_z"),
+                bt.vector(k, "dual_slack"))
+        assert r[0] <= 10 * TOL and r[1] <= 10 * TOL and r[2] <= 10 * TOL, ("batch16", k, r)
+        if k in l1_64:
+            assert abs(r[3] - l1_64[k]) <= 2 * TOL * _denominator(pp, targets[k]), \
+                ("batch16", k, r[3], l1_64[k])
+    bt.close()

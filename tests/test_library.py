@@ -146,11 +146,11 @@ def test_uneven_set_covers_follow_their_recipe():
 
 
 @pytest.mark.parametrize("workload,keys", [
-    ("mixed-1m", ("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>",
+    ("mixed-1m", ("EpiBatchPrimalY<8, false>", "EpiBatchDualZ<8, false>",
                   "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
-    ("powerlaw-5m", ("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>",
+    ("powerlaw-5m", ("EpiBatchPrimalY<8, false>", "EpiBatchDualZ<8, false>",
                      "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
-    ("general-int-4m", ("EpiBatchPrimalY<false>", "EpiBatchDualZ<false>",
+    ("general-int-4m", ("EpiBatchPrimalY<8, false>", "EpiBatchDualZ<8, false>",
                         "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
     ("setcover-pack-64", ("EpiPackPrimalY<false>", "EpiPackDualZ<false>")),
 ])
