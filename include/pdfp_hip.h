@@ -145,6 +145,11 @@ int pdfp_set_original_primal_residual(pdfp_ctx* ctx, const double* D_r);
 /* copyto!(current_x, x); copyto!(initial_x, x)  src/pump.jl:305-306. */
 int pdfp_set_x(pdfp_ctx* ctx, const double* x_scaled);
 
+/* The same for the duals (current and initial y, length n, and z, length
+ * m): the reference's pump never sets them (no counterpart); the FP64 polish
+ * of the FP32 form starts an FP64 context at an FP32 context's point. */
+int pdfp_set_dual(pdfp_ctx* ctx, const double* y, const double* z);
+
 /* set_target!(s, x̃)  src/pdfp.jl:98-101 (copies x̃, then warm_start!). */
 int pdfp_set_target(pdfp_ctx* ctx, const double* x_tilde_scaled);
 

@@ -110,6 +110,7 @@ def load() -> C.CDLL:
         "orc_pdfp_destroy": (None, [P]),
         "orc_pdfp_original_primal_residual": (None, [P, _D]),
         "orc_pdfp_set_x": (None, [P, _D]),
+        "orc_pdfp_set_dual": (None, [P, _D, _D]),
         "orc_pdfp_warm_start": (None, [P]),
         "orc_pdfp_set_target": (None, [P, _D]),
         "orc_pdfp_solve": (C.c_int32, [P, C.POINTER(OrcParams), C.c_double]),
@@ -244,6 +245,9 @@ class OraclePDFPState:
 
     def set_x(self, x):
         self._lib.orc_pdfp_set_x(self._ctx, _d(_f64(x)))
+
+    def set_dual(self, y, z):
+        self._lib.orc_pdfp_set_dual(self._ctx, _d(_f64(y)), _d(_f64(z)))
 
     def info(self) -> OrcInfo:
         out = OrcInfo()

@@ -512,6 +512,15 @@ EXPORT void orc_pdfp_set_x(orc_pdfp* s, const double* x) {
   memcpy(s->initial_x, x, sizeof(double) * (size_t)s->n);
 }
 
+/* The same for y and z (no counterpart in the reference; the host tests of
+ * the FP64 polish start a state at another state's point). */
+EXPORT void orc_pdfp_set_dual(orc_pdfp* s, const double* y, const double* z) {
+  memcpy(s->current_y, y, sizeof(double) * (size_t)s->n);
+  memcpy(s->initial_y, y, sizeof(double) * (size_t)s->n);
+  memcpy(s->current_z, z, sizeof(double) * (size_t)s->m);
+  memcpy(s->initial_z, z, sizeof(double) * (size_t)s->m);
+}
+
 /* warm_start!(s)  src/pdfp.jl:104-114. */
 EXPORT void orc_pdfp_warm_start(orc_pdfp* s) {
   memcpy(s->initial_x, s->current_x, sizeof(double) * (size_t)s->n);

@@ -98,6 +98,7 @@ SIGNATURES = {
                               C.c_int32, _D, _U8, _U8, _D, _D, _D, C.c_int32, C.c_int32]),
     "pdfp_set_original_primal_residual": (C.c_int, [_P, _D]),
     "pdfp_set_x": (C.c_int, [_P, _D]),
+    "pdfp_set_dual": (C.c_int, [_P, _D, _D]),
     "pdfp_set_target": (C.c_int, [_P, _D]),
     "pdfp_warm_start": (C.c_int, [_P]),
     "pdfp_solve": (C.c_int, [_P, C.POINTER(PdfpParams), C.c_double, C.POINTER(C.c_int32), _I64]),

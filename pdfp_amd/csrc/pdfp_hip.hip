@@ -33,6 +33,7 @@ struct ProjectorBase {
   virtual ~ProjectorBase() = default;
   virtual void set_original_primal_residual(const double* D_r) = 0;
   virtual void set_x(const double* x) = 0;
+  virtual void set_dual(const double* y, const double* z) = 0;
   virtual void set_target(const double* xt) = 0;
   virtual void warm_start() = 0;
   virtual int solve(const pdfp_params& p, double pw) = 0;
@@ -336,6 +337,14 @@ struct Projector final : ProjectorBase {
   void set_x(const double* xs) override {
     upload_T(x, xs, n);
     upload_T(x0, xs, n);
+  }
+
+  // current and initial y (n) and z (m), as set_x does for x.
+  void set_dual(const double* ys, const double* zs) override {
+    upload_T(y, ys, n);
+    upload_T(y0, ys, n);
+    upload_T(z, zs, m);
+    upload_T(z0, zs, m);
   }
 
   // set_target!  src/pdfp.jl:98-101: one copy into pinned memory, then an
@@ -950,6 +959,13 @@ int pdfp_set_x(pdfp_ctx* ctx, const double* x) {
   return guarded(ctx, [&] {
     if (!x) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null x");
     ctx->impl->set_x(x);
+  });
+}
+
+int pdfp_set_dual(pdfp_ctx* ctx, const double* y, const double* z) {
+  return guarded(ctx, [&] {
+    if (!y || !z) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null y or z");
+    ctx->impl->set_dual(y, z);
   });
 }
 

@@ -135,6 +135,14 @@ class PDFPState:
         x = f64(x_scaled)
         self._check(self._lib.pdfp_set_x(self._ctx, dptr(x)))
 
+    def set_dual(self, y, z) -> None:
+        """Current and initial y and z (no counterpart in the reference: the
+        FP64 polish starts from an FP32 context's duals)."""
+        yy, zz = f64(y), f64(z)
+        if len(yy) != self.n or len(zz) != self.m:
+            raise ValueError("y must have n entries and z m")
+        self._check(self._lib.pdfp_set_dual(self._ctx, dptr(yy), dptr(zz)))
+
     def info(self) -> _lib.PdfpInfo:
         out = _lib.PdfpInfo()
         self._check(self._lib.pdfp_get_info(self._ctx, C.byref(out)))

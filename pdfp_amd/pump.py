@@ -215,13 +215,15 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                      push_threshold: float = 0.01, violation_threshold: float = 5000.0,
                      propagation_budget: int = 100_000, feasibility_tolerance: float = 1e-6,
                      precision: str = "fp32", device: int = 0, scaling: str = "device",
-                     projector=None, root_lp=None) -> PumpResult:
+                     polish: bool = True, projector=None, root_lp=None) -> PumpResult:
     """Run the feasibility pump on ``milp`` until a feasible point turns up or
     the time limit passes (src/pump.jl:231-410).  The keyword arguments and defaults are the
     reference's (README.md:89-108); ``precision`` picks the projector's form
     ("fp32": FP32 data and iterates with FP64 reductions and polish; "fp64");
     ``scaling`` where the projector's matrix is equilibrated ("device" or
-    "host").
+    "host"); ``polish`` (FP32 form, on by default): each FP32 projection is
+    finished by an FP64 solve started at the FP32 point (x, y and z) with the
+    same target and tolerances, and the rounder receives the FP64 point.
 
     ``projector`` / ``root_lp`` replace the device solvers: a module with
     ``PDFPState``, ``set_target``, ``warm_start``, ``pdfp_solve``,
@@ -307,6 +309,21 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         if target_scaled is not None:
             proj["target"] = target_scaled
         P.pdfp_solve(st, params, primal_weight)
+        if polish and getattr(st, "precision", None) == "fp32":
+            # the FP64 finish: started at the FP32 point (current and initial
+            # x, y, z), the same target and tolerances
+            st64 = proj.get("polish")
+            if st64 is None:
+                st64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub,
+                                   pp.w, precision="fp64", device=device)
+                if mixed:
+                    P.original_primal_residual(st64, pp.D_r)
+                proj["polish"] = st64
+            st64.set_x(st.pdhg_x)
+            st64.set_dual(st.vector("pdhg_y"), st.vector("pdhg_z"))
+            P.set_target(st64, proj["target"])
+            P.pdfp_solve(st64, params, primal_weight)
+            return pp.D_c * st64.pdhg_x, st.total_count + st64.total_count
         return pp.D_c * st.pdhg_x, st.total_count
 
     traj = _trajectory(milp, cfg, seed, x, dual_slack, stats, emit)
@@ -525,7 +542,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                            violation_threshold: float = 5000.0,
                            propagation_budget: int = 100_000,
                            feasibility_tolerance: float = 1e-6, device: int = 0,
-                           scaling: str = "device") -> PumpResult:
+                           scaling: str = "device", polish: bool = True) -> PumpResult:
     """K concurrent pump trajectories on one GPU; the first feasible point
     wins.  Replaces solve_concurrent (scripts/solve.jl:121-148), which forks K
     processes with seeds 100k + seed, each holding its own copy of the
@@ -603,6 +620,23 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
     primal_weight = math.sqrt(1.0 + (len(milp.int_idx) / m if m > 0 else 0.0))
     fp64: list = [None] * K        # a trajectory's own FP64 projector, once it needs one
     target: list = [None] * K      # its last target (scaled)
+    fin64: list = [None] * K       # its FP64 context for the polish of its FP32 projections
+
+    def polished(k, x_s, y, z, count, params):
+        """Trajectory k's FP32 projection finished by an FP64 solve from its
+        point (x, y, z), on the host thread that then rounds it."""
+        st = fin64[k]
+        if st is None:
+            st = _pdfp_mod.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub,
+                                     pp.w, precision="fp64", device=device)
+            if mixed:
+                _pdfp_mod.original_primal_residual(st, pp.D_r)
+            fin64[k] = st
+        st.set_x(x_s)
+        st.set_dual(y, z)
+        _pdfp_mod.set_target(st, target[k])
+        _pdfp_mod.pdfp_solve(st, params, primal_weight)
+        return advance(k, (D_c * st.pdhg_x, count + st.total_count))
 
     def finish(k, found, xt, t=None):
         r = _finish(milp, found, xt, stats[k], cfg.elapsed() if t is None else t,
@@ -636,6 +670,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
 
     running: set = set()
     futures: dict = {}
+    batch_params: dict = {}
     winner = None
     while pending or running or futures:
         for k in sorted(pending):
@@ -668,6 +703,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                     batch.warm_start(k)
                 batch.begin(k, params, primal_weight)
                 running.add(k)
+                batch_params[k] = params
         pending = {}
         if running:
             # with projections waiting on the host, return after each block
@@ -676,8 +712,14 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             for k in sorted(running):
                 if terms[k] != "UNSPECIFIED":
                     running.discard(k)
-                    futures[k] = pool.submit(advance, k,
-                                             (D_c * batch.pdhg_x(k), batch.total_count(k)))
+                    if polish:
+                        futures[k] = pool.submit(polished, k, batch.pdhg_x(k),
+                                                 batch.vector(k, "pdhg_y"),
+                                                 batch.vector(k, "pdhg_z"),
+                                                 batch.total_count(k), batch_params[k])
+                    else:
+                        futures[k] = pool.submit(advance, k, (D_c * batch.pdhg_x(k),
+                                                              batch.total_count(k)))
         if futures:
             done = [k for k, f in futures.items() if f.done()]
             if not done and not running:

@@ -5,14 +5,16 @@
 - README.md:36-42 (p0201): the root LP ends OPTIMAL after 600 iterations,
   the first rounding violates the rows by 9, and pump iteration 1 has ℓ1
   distance 8.08613, violation 1 and 600 PDHG iterations — the reference's
-  log, reproduced by the FP64 form (the later iterations depend on Julia's
-  random stream and on the reference's CUDA summation order).
+  log, reproduced by the FP64 form (iteration 2 differs; see the oracle's
+  tests of the same log).
+- The FP64 polish of the FP32 form keeps the pump's end point feasible.
 """
 import re
 
 import numpy as np
 import pytest
 
+from pdfp_amd import generators as G
 from pdfp_amd.milp import is_feasible, read_mps
 from pdfp_amd.pump import feasibility_pump
 
@@ -82,3 +84,14 @@ def test_pack_pump_finds_feasible_for_every_instance():
     for m, r in zip(milps, rs):
         assert r.found and is_feasible(m, r.x), m.name
         assert abs(r.objective - (float(m.c @ r.x) + m.c0)) <= 1e-9 * (1 + abs(r.objective))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_pump_fp32_with_fp64_polish(seed):
+    """feasibility_pump(precision="fp32", polish=True): each FP32 projection
+    is finished by an FP64 solve from its point; the pump still ends with a
+    point that is_feasible accepts."""
+    print("pump polish", seed, flush=True)
+    m = G.mixed(2000, 4000, 1)
+    r = feasibility_pump(m, seed=seed, time_limit=60.0, precision="fp32", polish=True)
+    assert r.found and is_feasible(m, r.x)

@@ -136,7 +136,7 @@ def test_tight_projections_leave_the_fp32_form():
                           original_primal_residual=O.original_primal_residual)
     m = read_mps(example_path("pk1"))
     r = feasibility_pump(m, projector=mod, root_lp=oracle_root_lp, time_limit=120.0,
-                         precision="fp32", scaling="host")
+                         precision="fp32", scaling="host", polish=False)
     assert r.found and is_feasible(m, r.x)
     assert calls and calls[0][0] == "fp32"
     assert all(prec == "fp64" for prec, eps in calls if eps < FP32_TOLERANCE_FLOOR)
@@ -368,3 +368,41 @@ def test_p0201_second_target_is_not_an_arithmetic_knife_edge(start_rel, frac_rel
         assert lines[0][2:4] == ["1", "600"]
         assert lines[0][1] == "8.08613" or frac_rel > 0
         assert lines[1][:4] == ["2", "1.15587", "1", "600"]
+
+
+def test_fp64_polish_finishes_each_fp32_projection():
+    """feasibility_pump(polish=True) on the FP32 form: every FP32 projection
+    is followed by an FP64 solve started at the FP32 point (x, y and z), with
+    the same target and tolerances, and the rounder receives the FP64 point
+    (the oracle stands in for both forms here)."""
+    from types import SimpleNamespace
+
+    from oracle import oracle as O
+
+    events = []
+
+    class State(O.OraclePDFPState):
+        def __init__(self, *args, precision="fp32", **kw):
+            super().__init__(*args, **kw)
+            self.precision = precision
+
+        def set_dual(self, y, z):
+            events.append(("set_dual", self.precision))
+            super().set_dual(y, z)
+
+    def solve(s, p, pw):
+        events.append(("solve", s.precision, p.eps_optimal_relative, p.eps_feasible_relative))
+        return O.pdfp_solve(s, p, pw)
+
+    mod = SimpleNamespace(PDFPState=State, set_target=O.set_target, warm_start=O.warm_start,
+                          pdfp_solve=solve, original_primal_residual=O.original_primal_residual)
+    m = G.mixed(300, 600, 2)
+    r = feasibility_pump(m, projector=mod, root_lp=oracle_root_lp, time_limit=120.0,
+                         precision="fp32", scaling="host", polish=True, batch_policy="always")
+    assert r.found and is_feasible(m, r.x)
+    solves = [e for e in events if e[0] == "solve"]
+    fp32 = [i for i, e in enumerate(solves) if e[1] == "fp32"]
+    assert fp32
+    for i in fp32:      # each FP32 solve is finished by an FP64 one, same tolerances
+        assert solves[i + 1][1] == "fp64" and solves[i + 1][2:] == solves[i][2:]
+    assert events.count(("set_dual", "fp64")) == len(fp32)
