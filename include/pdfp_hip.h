@@ -523,16 +523,6 @@ int pdfp_scale_matrix(int64_t m, int64_t n, int64_t nnz, const int64_t* colptr,
                       double* scaled_nzval, double* D_r, double* D_c, int32_t device);
 const char* pdfp_scale_last_error(void);
 
-/* ---- measurement: the random-gather ceiling ------------------------------
- * No counterpart in the reference.  Mean microseconds of one pass of
- * n_gathers gathers of record_bytes-byte records (4, 8, 16, 32 or 64) at
- * uniformly random rows of a table_bytes-byte device table, with the 4-byte
- * row indices streamed from HBM: the floor of a CSR pass whose columns are
- * scattered at random, for the benchmark's roofline.  Allocates and frees
- * its own buffers on `device`. */
-int pdfp_gather_ceiling(int64_t table_bytes, int32_t record_bytes, int64_t n_gathers,
-                        int32_t device, double* us_per_pass);
-const char* pdfp_probe_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # PDFP_HIP_LIB selects another build of the library (e.g. a variant built
 # with `python -m pdfp_amd.build --out PATH -D NAME=VALUE`).
 LIB_PATH = os.environ.get("PDFP_HIP_LIB", os.path.join(HERE, "libpdfp_hip.so"))
+PROBE_PATH = os.path.join(HERE, "libpdfp_probe.so")
 
 TERMINATION = {0: "UNSPECIFIED", 1: "OPTIMAL", 2: "ITERATION_LIMIT", 3: "TIME_LIMIT",
                4: "PRIMAL_INFEASIBLE", 5: "DUAL_INFEASIBLE"}
@@ -171,7 +172,6 @@ SIGNATURES = {
     "pdfp_pack_time_kernels": (C.c_int, [_P, C.c_int32, _D, _D]),
     "pdfp_pack_destroy": (None, [_P]),
     "pdfp_batch_destroy": (None, [_P]),
-    "pdfp_gather_ceiling": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, _D]),
     "pdfp_scale_matrix": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32,
                                     C.c_int32, C.c_double, _D, _D, _D, C.c_int32]),
     "pdfp_scale_last_error": (C.c_char_p, []),
@@ -191,6 +191,11 @@ SIGNATURES = {
     # device pointers throughout; the last argument is a hipStream_t
     "pdfp_max_violation_noalloc": (C.c_int, [C.c_int64, C.c_int64, _P, _P, _P, _P, _P, _P,
                                              _P, _P]),
+}
+
+# libpdfp_probe (include/pdfp_probe.h): measurement only, loaded by bench.py.
+PROBE_SIGNATURES = {
+    "pdfp_gather_ceiling": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, _D]),
     "pdfp_probe_last_error": (C.c_char_p, []),
 }
 
@@ -234,16 +239,34 @@ class LibraryError(RuntimeError):
     pass
 
 
+_probe = None
+
+
+def load_probe() -> C.CDLL:
+    """Load libpdfp_probe.so (the benchmark's gather probe)."""
+    global _probe
+    if _probe is None:
+        if not os.path.exists(PROBE_PATH):
+            raise ImportError(f"{PROBE_PATH} is missing: build it with `python -m pdfp_amd.build`")
+        lib = C.CDLL(PROBE_PATH)
+        for name, (res, args) in PROBE_SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _probe = lib
+    return _probe
+
+
 def gather_ceiling_us(table_bytes: int, record_bytes: int, n_gathers: int,
                       device: int = 0) -> float:
     """Microseconds per pass of n_gathers random gathers of record_bytes-byte
     records from a table_bytes-byte device table (pdfp_gather_ceiling)."""
-    lib = load()
+    lib = load_probe()
     us = C.c_double()
     st = lib.pdfp_gather_ceiling(int(table_bytes), int(record_bytes), int(n_gathers), device,
                                  C.byref(us))
     if st != 0:
-        raise LibraryError(f"libpdfp_hip error {st}: {lib.pdfp_probe_last_error().decode()}")
+        raise LibraryError(f"libpdfp_probe error {st}: {lib.pdfp_probe_last_error().decode()}")
     return us.value
 
 

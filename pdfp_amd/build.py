@@ -3,7 +3,9 @@
     python -m pdfp_amd.build            # build the library
     python -m pdfp_amd.build --check    # print the exported C-ABI symbols
 
-The library is written next to this file (pdfp_amd/libpdfp_hip.so).
+The library is written next to this file (pdfp_amd/libpdfp_hip.so), and the
+benchmark's gather probe beside it as a library of its own
+(pdfp_amd/libpdfp_probe.so, include/pdfp_probe.h).
 """
 from __future__ import annotations
 
@@ -16,7 +18,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpdfp_hip.so")
-SOURCES = ["pdfp_hip.hip", "pdlp_hip.hip", "batch_hip.hip", "pack_hip.hip", "bench_hip.hip", "scale_hip.hip", "check_hip.hip", "rounder.cpp"]
+SOURCES = ["pdfp_hip.hip", "pdlp_hip.hip", "batch_hip.hip", "pack_hip.hip", "scale_hip.hip",
+           "check_hip.hip", "rounder.cpp"]
+PROBE = os.path.join(HERE, "libpdfp_probe.so")
+PROBE_SOURCES = ["bench_hip.hip"]
 
 # -ffp-contract=off: no a*b+c contraction, so the FP64 form performs the
 # reference's operations one by one.  The packed-FP32 target feature is
@@ -35,12 +40,27 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
+def _link(objs, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False, out: str = LIB, defines: tuple = (),
+          probe: str | None = PROBE) -> str:
     """Compiles every source to an object in parallel (one hipcc per file,
-    under pdfp_amd/_build/), then links the shared library."""
+    under pdfp_amd/_build/), then links the library and (probe) the probe
+    library."""
     from concurrent.futures import ThreadPoolExecutor
 
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    psrcs = [os.path.join(CSRC, s) for s in PROBE_SOURCES] if probe else []
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     flags = [f for f in HIPCC_FLAGS if f != "-shared"]
@@ -53,8 +73,9 @@ def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
             print(" ".join(cmd), flush=True)
         return obj, subprocess.run(cmd, capture_output=True, text=True)
 
-    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1, 8)) as ex:
-        results = list(ex.map(compile_one, srcs))
+    allsrc = srcs + psrcs
+    with ThreadPoolExecutor(max_workers=min(len(allsrc), os.cpu_count() or 1, 8)) as ex:
+        results = list(ex.map(compile_one, allsrc))
     failed = [(o, r) for o, r in results if r.returncode != 0]
     objs = [o for o, _ in results]
     try:
@@ -62,15 +83,9 @@ def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
             for _, r in failed:
                 sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed on {len(failed)} source(s)")
-        tmp = out + ".tmp"
-        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, capture_output=True, text=True)
-        if res.returncode != 0:
-            sys.stderr.write(res.stdout + res.stderr)
-            raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
-        os.replace(tmp, out)
+        _link(objs[:len(srcs)], out, verbose)
+        if probe:
+            _link(objs[len(srcs):], probe, verbose)
     finally:
         for o in objs:
             if os.path.exists(o):

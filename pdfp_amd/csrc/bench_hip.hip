@@ -1,5 +1,5 @@
-// libpdfp_hip: the random-gather ceiling of the GPU, for the benchmark's
-// roofline.
+// libpdfp_probe (beside libpdfp_hip, not part of it): the random-gather
+// ceiling of the GPU, for the benchmark's roofline.
 //
 // A CSR pass over a matrix whose columns are scattered at random gathers one
 // record of the vector per nonzero.  Once the vector outgrows an XCD's L2,
@@ -16,6 +16,7 @@
 #include <string>
 
 #include "../../include/pdfp_hip.h"
+#include "../../include/pdfp_probe.h"
 #include "host_util.h"
 
 namespace pdfp {

@@ -185,3 +185,17 @@ def test_julia_shim_structs_mirror_the_c_structs(julia, ctype):
     the same order with the same types, so `Ref{…}` passes them by layout."""
     want = [(f, _c_type_name(t)) for f, t in ctype._fields_]
     assert _julia_struct_fields(julia) == want
+
+
+def test_probe_library_is_separate():
+    """The benchmark's gather probe lives in libpdfp_probe.so (its own header,
+    include/pdfp_probe.h), not in the product library."""
+    src = open(os.path.join(ROOT, "include", "pdfp_probe.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(pdfp_[a-z0-9_]+)\s*\(", src)))
+    assert names == ["pdfp_gather_ceiling", "pdfp_probe_last_error"]
+    probe = _lib.load_probe()
+    lib = _lib.load()
+    for name in names:
+        assert hasattr(probe, name)
+        assert not hasattr(lib, name), name
