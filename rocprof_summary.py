@@ -5,6 +5,7 @@
         EPILOGUE [EPILOGUE ...]
     python rocprof_summary.py traffic --fetch CSV --write CSV \\
         --copy-fetch CSV --copy-write CSV
+    python rocprof_summary.py counters COUNTER_CSV EPILOGUE [EPILOGUE ...]
 
 ``stats`` turns a ``--kernel-trace --stats`` kernel_stats.csv into a
 markdown table.  ``window`` reads a ``--kernel-trace`` kernel_trace.csv and
@@ -15,6 +16,8 @@ files (FETCH_SIZE and WRITE_SIZE of the benchmark, and of the 1 GiB plain
 copy of ``bench.py --copy-calibration``), scales each counter by what it
 reads for the copy of known size, and prints a table plus a JSON block of
 HBM bytes per launch per CSR pass epilogue, which bench.py reads back.
+``counters`` gives the mean of every counter of one ``--pmc`` run per CSR
+pass epilogue (L2 hits and misses, L1→L2 read latency, wave cycles).
 """
 from __future__ import annotations
 
@@ -148,6 +151,45 @@ def traffic(fetch: str, write: str, copy_fetch: str, copy_write: str,
     return "\n".join(lines)
 
 
+def counters(path: str, keys: list) -> str:
+    """Mean of every counter in one --pmc counter_collection.csv per CSR
+    pass epilogue (the instantiation with the most dispatches), plus the
+    figures derived from them when their counters are present."""
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        key = pass_key(r["Kernel_Name"])
+        if key in keys:
+            per[(key, r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    best: dict = {}
+    for (key, name), cs in per.items():
+        n = max(len(v) for v in cs.values())
+        if key not in best or n > best[key][1]:
+            best[key] = (name, n)
+    out = []
+    for key in keys:
+        if key not in best:
+            continue
+        name, n = best[key]
+        mean = {c: sum(v) / len(v) for c, v in per[(key, name)].items()}
+        out += [f"`{key}`: `{short_name(name)}`, {n} dispatches", "",
+                "| counter | mean per dispatch |", "|---|---:|"]
+        out += [f"| {c} | {v:.1f} |" for c, v in sorted(mean.items())]
+        derived = []
+        if "TCC_HIT_sum" in mean and "TCC_MISS_sum" in mean:
+            h, m = mean["TCC_HIT_sum"], mean["TCC_MISS_sum"]
+            derived.append(f"L2 hit rate {h / max(h + m, 1.0):.3f}")
+        if "TCP_TCC_READ_REQ_LATENCY_sum" in mean and "TCP_TCC_READ_REQ_sum" in mean:
+            derived.append("mean L1→L2 read latency "
+                           f"{mean['TCP_TCC_READ_REQ_LATENCY_sum'] / max(mean['TCP_TCC_READ_REQ_sum'], 1.0):.0f} cycles")
+        if "SQ_WAIT_INST_ANY" in mean and "SQ_WAVE_CYCLES" in mean:
+            derived.append("SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES "
+                           f"{mean['SQ_WAIT_INST_ANY'] / max(mean['SQ_WAVE_CYCLES'], 1.0):.2f}")
+        if derived:
+            out += ["", "derived: " + "; ".join(derived)]
+        out.append("")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -161,11 +203,16 @@ def main():
     t = sub.add_parser("traffic")
     for a in ("--fetch", "--write", "--copy-fetch", "--copy-write"):
         t.add_argument(a, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("csv")
+    c.add_argument("keys", nargs="+")
     a = ap.parse_args()
     if a.cmd == "stats":
         print(stats(a.csv, a.top))
     elif a.cmd == "window":
         print(window(a.csv, a.keys, a.last))
+    elif a.cmd == "counters":
+        print(counters(a.csv, a.keys))
     else:
         print(traffic(a.fetch, a.write, a.copy_fetch, a.copy_write))
 
