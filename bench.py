@@ -514,6 +514,10 @@ def main():
                     help="launch every block's passes on the stream instead of replaying "
                          "captured block graphs (same kernels; for profilers that do not "
                          "trace graph replays)")
+    ap.add_argument("--cooperative", type=int, default=None,
+                    help="workgroups per CU of a cooperative launch running the middle "
+                         "iterations of each block of the batched passes (-1: as many as "
+                         "fit; 0: launch every pass)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
@@ -592,6 +596,8 @@ def main():
             bt.set_schedule(*[int(v) for v in args.batch_schedule.split(",")])
         if args.stream_launches:
             bt.set_block_graph(False)
+        if args.cooperative is not None:
+            bt.set_cooperative(args.cooperative)
         for k in range(K):
             bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]

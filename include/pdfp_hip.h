@@ -253,6 +253,19 @@ int pdfp_batch_set_schedule(pdfp_batch* ctx, int32_t npb_a, int32_t npb_at, int3
  * with its evaluations and readback, as one captured HIP graph; 0 launches
  * every kernel on the stream.  Both give identical results. */
 int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on);
+/* Cooperative blocks of the batch (no counterpart in the reference; it
+ * replaces the 2 launches per PDHG step of the reference's 200-step block
+ * loop, src/pdfp.jl:259-267): wg_per_cu != 0 runs iterations 2 .. freq-1 of
+ * each block of the batched passes in one cooperative launch of wg_per_cu
+ * workgroups per CU (-1: as many as fit), all resident, with a grid barrier
+ * between the passes; 0 (the default) launches each pass.  Both give
+ * identical results.  PDFP_ERR_ARGUMENT for wg_per_cu outside -1 .. 16 or
+ * above what fits; PDFP_ERR_DEVICE if the device has no cooperative launch,
+ * or if a barrier wait gives up (the call's iterates are then undefined).
+ * No effect on instances that run the one-workgroup path.  On MI355X the
+ * grid barriers cost more than the kernel boundaries they replace (75 vs
+ * 24 µs per iteration at 1M nonzeros), hence the default. */
+int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu);
 /* Kernel path of the batch (no counterpart in the reference), as
  * pdfp_set_kernel_path: 0 (the default) runs each block as one launch with
  * one workgroup per running trajectory when the instance fits in LDS (each

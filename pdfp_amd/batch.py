@@ -147,6 +147,14 @@ class PDFPBatch:
         stream.  Both give identical results."""
         self._check(self._lib.pdfp_batch_set_block_graph(self._ctx, int(bool(on))))
 
+    def set_cooperative(self, wg_per_cu) -> None:
+        """Run the middle iterations of each block of the batched passes in
+        one cooperative launch of `wg_per_cu` workgroups per CU (True or -1:
+        as many as fit), with a grid barrier between the passes, or launch
+        each pass (False or 0, the default).  Both give identical results."""
+        n = -1 if wg_per_cu is True else int(wg_per_cu)
+        self._check(self._lib.pdfp_batch_set_cooperative(self._ctx, n))
+
     def set_schedule(self, npb_a: int, npb_at: int, persist: int = 0, lane_rows_a: int = 0,
                      lane_rows_at: int = 0) -> None:
         """Replace the autotuned schedule of the batched passes (tests and

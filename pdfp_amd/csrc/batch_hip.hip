@@ -40,6 +40,36 @@ __global__ __launch_bounds__(BLOCK) void k_small_batch(const SmallBatchArgs<KB>*
   small_block_body<BLOCK, KB>(a, smem, red);
 }
 
+// Iterations i0 .. i1 of a block (none of them major: the block's first and
+// last iterations run as ordinary launches) in one cooperative launch: each
+// iteration is the Aᵀ pass and the A pass of step_t, every workgroup walking
+// its share of the row blocks as csr_pass_persistent does, with a grid
+// barrier after each pass in place of the kernel boundary.  The two passes
+// share one LDS array.  Returns early, with the barrier's error word set,
+// if a barrier wait gives up.
+template <int KB, int NPB_A, int NPB_T>
+__global__ __launch_bounds__(kBlock) void k_batch_coop(Csr<float> A, Csr<float> At,
+                                                       EpiBatchPrimalY<KB, false> ex,
+                                                       EpiBatchDualZ<KB, false> ez, int i0,
+                                                       int i1, GridBarrier* bar,
+                                                       unsigned int spin_limit) {
+  using LT = PassLds<kBlock, NPB_T, GatherBatch<KB>, EpiBatchPrimalY<KB, false>>;
+  using LA = PassLds<kBlock, NPB_A, GatherBatch<KB>, EpiBatchDualZ<KB, false>>;
+  __shared__ __align__(16) unsigned char lds[LT::kBytes > LA::kBytes ? LT::kBytes : LA::kBytes];
+  const GatherBatch<KB> gz{ez.z};   // the Aᵀ pass gathers z
+  const GatherBatch<KB> gx{ex.rx};  // the A pass gathers x̄
+  for (int i = i0; i <= i1; ++i) {
+    ex.k_offset = i;
+    ez.k_offset = i;
+    walk_blocks<kBlock, NPB_T, float, GatherBatch<KB>, EpiBatchPrimalY<KB, false>, 0, 0>(
+        At, gz, ex, nullptr, lds);
+    if (!grid_sync(bar, gridDim.x, spin_limit)) return;
+    walk_blocks<kBlock, NPB_A, float, GatherBatch<KB>, EpiBatchDualZ<KB, false>, 0, 0>(
+        A, gx, ez, nullptr, lds);
+    if (i < i1 && !grid_sync(bar, gridDim.x, spin_limit)) return;
+  }
+}
+
 struct Trajectory {
   RestartState r;
   bool orig_primal = false;
@@ -60,7 +90,9 @@ struct BatchBase {
   int K = 0;
   std::vector<Trajectory> tr;
   bool graph_on = true;
+  bool coop_on = false;
   virtual ~BatchBase() = default;
+  virtual void set_coop(int per_cu) = 0;
   virtual void set_original_primal_residual(const double* D_r) = 0;
   virtual void set_x(int k, const double* xs) = 0;
   virtual void set_target(int k, const double* xts) = 0;
@@ -635,6 +667,108 @@ struct Batch final : BatchBase {
     read_block_sums();
   }
 
+  // ---- the cooperative form of a block (coop_on) ----
+  // Iterations 2 .. freq-1 run in one k_batch_coop launch, the rest of the
+  // block as general_block launches it.  The barrier's state is zeroed once;
+  // its error word comes back with the block's sums and is checked after the
+  // block (check_coop).
+  static constexpr unsigned int kCoopSpin = 1u << 20;  // polls before a wait gives up
+  DevBuf<GridBarrier> bar;
+  PinnedBuf<GridBarrier> h_bar;
+  int coop_grid_cache[2][2] = {{0, 0}, {0, 0}};
+  int coop_per_cu = 0;  // workgroups per CU of the launch (0: as many as fit)
+  bool coop_pending = false;
+
+  void set_coop(int per_cu) override {
+    if (per_cu < -1 || per_cu > 16)
+      throw Error(PDFP_ERR_ARGUMENT, "cooperative workgroups per CU must be -1 .. 16");
+    const bool on = per_cu != 0;
+    if (on && !bar.p) {
+      int dev = 0, coop = 0;
+      PDFP_HIP(hipGetDevice(&dev));
+      PDFP_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+      if (!coop) throw Error(PDFP_ERR_DEVICE, "the device does not support cooperative launches");
+      bar.alloc(1);
+      h_bar.alloc(1);
+      PDFP_HIP(hipMemsetAsync(bar.p, 0, sizeof(GridBarrier), stream));
+      sync();
+    }
+    coop_on = on;
+    coop_per_cu = per_cu > 0 ? per_cu : 0;
+    for (auto& r : coop_grid_cache) r[0] = r[1] = 0;
+  }
+
+  // Workgroups of the cooperative launch: coop_per_cu per CU, or as many as
+  // are resident at once (the occupancy of the kernel × the CUs), a
+  // multiple of the 8 XCDs.
+  template <class Kern>
+  int coop_grid(Kern kern, int& cache) {
+    if (cache == 0) {
+      int per_cu = 0, dev = 0, cus = 0;
+      PDFP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
+      if (coop_per_cu > per_cu)
+        throw Error(PDFP_ERR_ARGUMENT, "more cooperative workgroups per CU than fit");
+      if (coop_per_cu > 0) per_cu = coop_per_cu;
+      PDFP_HIP(hipGetDevice(&dev));
+      PDFP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      cache = per_cu * cus / 8 * 8;
+      if (cache < 8) throw Error(PDFP_ERR_DEVICE, "the cooperative block does not fit the device");
+    }
+    return cache;
+  }
+
+  template <int NA, int NT>
+  void launch_coop_t(int i0, int i1, int& cache) {
+    auto kern = k_batch_coop<KB, NA, NT>;
+    const int grid = coop_grid(kern, cache);
+    Csr<float> av = A.view(), atv = At.view();
+    EpiBatchPrimalY<KB, false> ex{d_sp.p, d_wt.p, i0, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+                                  rx.p, px.p, ds.p, py.p, ry.p};
+    EpiBatchDualZ<KB, false> ez{d_sp.p, d_wt.p, i0, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    GridBarrier* bp = bar.p;
+    unsigned int spin = kCoopSpin;
+    void* args[] = {&av, &atv, &ex, &ez, &i0, &i1, &bp, &spin};
+    PDFP_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid),
+                                        dim3(kBlock), args, 0, stream));
+  }
+
+  // Iterations i0 .. i1 (non-major) in one cooperative launch.
+  void launch_coop(int i0, int i1) {
+    if (!bar.p) throw Error(PDFP_ERR_STATE, "cooperative blocks are not enabled");
+    if (i1 < i0) return;
+    if (A.max_rows > kBlock / kBatch || At.max_rows > kBlock / kBatch)
+      throw Error(PDFP_ERR_STATE, "batch schedule holds too many rows per block");
+    const int ia = A.npb == 512 ? 1 : 0, it = At.npb == 512 ? 1 : 0;
+    int& cache = coop_grid_cache[ia][it];
+    if (ia && it) launch_coop_t<512, 512>(i0, i1, cache);
+    else if (ia) launch_coop_t<512, 256>(i0, i1, cache);
+    else if (it) launch_coop_t<256, 512>(i0, i1, cache);
+    else launch_coop_t<256, 256>(i0, i1, cache);
+  }
+
+  void coop_block(int freq, bool any_restart) {
+    step(1, any_restart || freq == 1);
+    if (any_restart) evaluate_all(d_mask.p + 1, 1);
+    if (freq > 2) launch_coop(2, freq - 1);
+    if (freq >= 2) step(freq, true);
+    evaluate_all(d_mask.p, 0);
+    PDFP_HIP(hipMemcpyAsync(h_bar.p, bar.p, sizeof(GridBarrier), hipMemcpyDeviceToHost, stream));
+    read_block_sums();
+    coop_pending = true;
+  }
+
+  // After the stream has drained: a barrier wait that gave up leaves the
+  // block's iterates undefined; the state is reset and the call fails.
+  void check_coop() {
+    if (!coop_pending) return;
+    coop_pending = false;
+    if (h_bar.p->err[0] != 0u) {
+      PDFP_HIP(hipMemsetAsync(bar.p, 0, sizeof(GridBarrier), stream));
+      sync();
+      throw Error(PDFP_ERR_DEVICE, "a grid barrier wait of the cooperative block timed out");
+    }
+  }
+
   // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for
   // trajectory k: it starts a solve with its own parameters.
   void begin(int k, const pdfp_params& p, double pw) override {
@@ -717,11 +851,13 @@ struct Batch final : BatchBase {
                               stream));
       small_used = small_ok && path == 0;
       if (small_used) small_block(freq, omega, active);
+      else if (coop_on) coop_block(freq, any_restart);
       else if (graph_on) block_graph(freq, any_restart);
       else general_block(freq, any_restart);
       for (int k = 0; k < K; ++k)
         if (active[(size_t)k]) finish_l1_den(k);  // on the host while the block runs
       sync();
+      check_coop();
       const double now = wall_seconds();
       bool ended = false;
       for (int k = 0; k < K; ++k) {
@@ -903,13 +1039,23 @@ struct Batch final : BatchBase {
     PDFP_HIP(hipEventCreate(&e0));
     PDFP_HIP(hipEventCreate(&e1));
     PDFP_HIP(hipEventRecord(e0, stream));
-    for (int i = 0; i < iters; ++i) step(2 + (i % 198), false);
+    if (coop_on) {
+      // the cooperative form: runs of up to 198 iterations per launch
+      for (int i = 0; i < iters; i += 198) launch_coop(2, 2 + std::min(198, iters - i) - 1);
+      PDFP_HIP(hipMemcpyAsync(h_bar.p, bar.p, sizeof(GridBarrier), hipMemcpyDeviceToHost,
+                              stream));
+      coop_pending = true;
+    } else {
+      for (int i = 0; i < iters; ++i) step(2 + (i % 198), false);
+    }
     PDFP_HIP(hipEventRecord(e1, stream));
     PDFP_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     PDFP_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    sync();
+    check_coop();
     return (double)ms / std::max(iters, 1);
   }
 };
@@ -1108,6 +1254,10 @@ int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on) {
     if (on != 0 && on != 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "block graph switch must be 0 or 1");
     ctx->impl->graph_on = on != 0;
   });
+}
+
+int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu) {
+  return batch_guarded(ctx, [&] { ctx->impl->set_coop(wg_per_cu); });
 }
 
 void pdfp_batch_destroy(pdfp_batch* ctx) { delete ctx; }

@@ -248,10 +248,41 @@ constexpr int kLaneRows = 1;
 
 // ---- the pass ------------------------------------------------------------------
 
-// One row block b of the schedule, processed by the whole workgroup.
+// The LDS of one pass, carved from one byte array so that a kernel running
+// two passes back to back can give both the same storage:
+//   stage   products double[NV][NPB + 1], or raw float[NPB + 1][NV] values
+//           then float[NPB + 1] a_ij.  Slot NPB takes the stores of lanes
+//           past the end of the slice, so the staging stores need no branch
+//           (a branch would let the compiler sink each load into it, one
+//           load in flight at a time);
+//   rowsum  double[NV][MAXROWS], the reduced row sums;
+//   red     double[BLOCK / 64], the workgroup reduction of a long row;
+//   off     int[MAXROWS + 2], the row offsets (+ a pad slot for lanes past
+//           the last row).
+// A lane epilogue serves every (row, value) pair of a block in one pass of
+// the workgroup: its schedule holds at most BLOCK / kLanes rows.
+template <int BLOCK, int NPB, class Gather, class Epi>
+struct PassLds {
+  static constexpr int NV = Gather::NV;
+  static constexpr bool kRaw = raw_stage<Gather>::value;
+  static constexpr int NS = NPB + 1;
+  static constexpr int MAXROWS =
+      epi_lanes<Epi>::value > 0 ? BLOCK / epi_lanes<Epi>::value : BLOCK;
+  static constexpr int kStageBytes = kRaw ? NS * (NV + 1) * 4 : NV * NS * 8;
+  static constexpr int kRowsum = (kStageBytes + 15) / 16 * 16;
+  static constexpr int kRed = kRowsum + NV * MAXROWS * 8;
+  static constexpr int kOff = kRed + (BLOCK / kWave) * 8;
+  static constexpr int kBytes = kOff + (MAXROWS + 2) * 4;
+};
+
+// One row block b of the schedule, processed by the whole workgroup, with
+// the pass's LDS at `lds` (PassLds<...>::kBytes bytes, 16-byte aligned);
+// tid is threadIdx.x.
 template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX>
 __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, const Epi& epi,
-                                           double* __restrict__ partials, const int b) {
+                                           double* __restrict__ partials, const int b,
+                                           unsigned char* __restrict__ lds, const int tid) {
+  using L = PassLds<BLOCK, NPB, Gather, Epi>;
   constexpr int NV = Gather::NV;
   constexpr bool kRaw = raw_stage<Gather>::value;
   // sweeps of the slice in flight per lane; a gather of more than 8 values
@@ -263,24 +294,15 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   constexpr int NA = NACC > 0 ? NACC : 1;
   static_assert(!kRaw || (std::is_same<TV, float>::value && NV % 4 == 0),
                 "raw staging: FP32 matrix values, a multiple of 4 gathered values");
-  // products: double[NV][NPB + 1]; raw: float[NPB + 1][NV] values then
-  // float[NPB + 1] a_ij.  Slot NPB takes the stores of lanes past the end of
-  // the slice, so the staging stores need no branch (a branch would let the
-  // compiler sink each load into it, one load in flight at a time).
-  constexpr int NS = NPB + 1;
-  constexpr int kStageBytes = kRaw ? NS * (NV + 1) * 4 : NV * NS * 8;
-  __shared__ __align__(16) unsigned char stage_mem[kStageBytes];
-  double(*stage)[NS] = reinterpret_cast<double(*)[NS]>(stage_mem);
-  float* const sraw = reinterpret_cast<float*>(stage_mem);
+  constexpr int NS = L::NS;
+  constexpr int MAXROWS = L::MAXROWS;
+  double(*stage)[NS] = reinterpret_cast<double(*)[NS]>(lds);
+  float* const sraw = reinterpret_cast<float*>(lds);
   float* const sa = sraw + NS * NV;
-  // A lane epilogue serves every (row, value) pair of a block in one pass of
-  // the workgroup: its schedule holds at most BLOCK / kLanes rows.
-  constexpr int MAXROWS = epi_lanes<Epi>::value > 0 ? BLOCK / epi_lanes<Epi>::value : BLOCK;
-  __shared__ double rowsum[NV][MAXROWS];
-  __shared__ int off[MAXROWS + 2];  // + a pad slot for lanes past the last row
-  __shared__ double red[BLOCK / kWave];
+  double(*rowsum)[MAXROWS] = reinterpret_cast<double(*)[MAXROWS]>(lds + L::kRowsum);
+  double* const red = reinterpret_cast<double*>(lds + L::kRed);
+  int* const off = reinterpret_cast<int*>(lds + L::kOff);
 
-  const int tid = threadIdx.x;
   const int4 sb = A.sched[b];
   const int4 se = A.sched[b + 1];
   const int r0 = sb.x, r1 = se.x;
@@ -532,16 +554,19 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
 template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX = 0>
 __global__ __launch_bounds__(BLOCK) void csr_pass(Csr<TV> A, Gather g, Epi epi,
                                                   double* __restrict__ partials) {
+  __shared__ __align__(16) unsigned char lds[PassLds<BLOCK, NPB, Gather, Epi>::kBytes];
   pass_block<BLOCK, NPB, TV, Gather, Epi, NACC, NMAX>(A, g, epi, partials,
-                                                      xcd_block(blockIdx.x, gridDim.x));
+                                                      xcd_block(blockIdx.x, gridDim.x), lds,
+                                                      threadIdx.x);
 }
 
-// Persistent form: a grid of a few workgroups per CU (a multiple of 8) walks
+// Persistent walk: a grid of a few workgroups per CU (a multiple of 8) walks
 // the row blocks; the workgroups on XCD x take the x-th contiguous eighth of
 // the blocks, so each XCD's L2 serves a contiguous row range.
-template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX = 0>
-__global__ __launch_bounds__(BLOCK) void csr_pass_persistent(Csr<TV> A, Gather g, Epi epi,
-                                                             double* __restrict__ partials) {
+template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX>
+__device__ __forceinline__ void walk_blocks(const Csr<TV>& A, const Gather& g, const Epi& epi,
+                                            double* __restrict__ partials,
+                                            unsigned char* __restrict__ lds) {
   constexpr int kXcd = 8;
   const int nb = A.nblocks;
   const int xcd = blockIdx.x % kXcd;
@@ -551,9 +576,98 @@ __global__ __launch_bounds__(BLOCK) void csr_pass_persistent(Csr<TV> A, Gather g
   const int start = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
   const int count = per + (xcd < rem ? 1 : 0);
   for (int i = local; i < count; i += nlocal) {
-    pass_block<BLOCK, NPB, TV, Gather, Epi, NACC, NMAX>(A, g, epi, partials, start + i);
+    // tid passes through an empty asm statement in every block, so that
+    // nothing computed from it is hoisted out of the loop (the hoisted values
+    // would stay live in registers across every block: 15-30 more VGPRs and
+    // one or two waves per SIMD fewer)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    pass_block<BLOCK, NPB, TV, Gather, Epi, NACC, NMAX>(A, g, epi, partials, start + i, lds, tid);
     __syncthreads();  // the next block reuses the LDS arrays
   }
+}
+
+template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX = 0>
+__global__ __launch_bounds__(BLOCK) void csr_pass_persistent(Csr<TV> A, Gather g, Epi epi,
+                                                             double* __restrict__ partials) {
+  __shared__ __align__(16) unsigned char lds[PassLds<BLOCK, NPB, Gather, Epi>::kBytes];
+  walk_blocks<BLOCK, NPB, TV, Gather, Epi, NACC, NMAX>(A, g, epi, partials, lds);
+}
+
+// ---- grid barrier of a cooperative launch -------------------------------------
+
+// State of a grid barrier in device memory, zeroed once when it is
+// allocated; every word sits on a 256-byte line of its own, so that the
+// arrivals of one group, the arrivals of the groups and the polls of the
+// waiting workgroups do not contend for one line.  The workgroups arrive in
+// 8 groups (blockIdx.x % 8): `group[g]` counts group g's arrivals at the
+// current barrier, `top` the groups complete, `gen` the barriers passed.  The
+// last workgroup of a group resets its group's count and arrives at `top`;
+// the last group resets `top` and advances `gen`, so the state is ready for
+// the next barrier and the next launch.  `err` is set when a wait gives up.
+struct GridBarrier {
+  static constexpr int kLine = 64;  // words per 256-byte line
+  unsigned int group[8][kLine];
+  unsigned int top[kLine];
+  unsigned int gen[kLine];
+  unsigned int err[kLine];
+};
+
+// Every workgroup of a cooperative launch (all resident at once; a multiple
+// of 8 workgroups) waits here until all have arrived.  Each thread's stores
+// before the barrier are visible to every thread of the grid after it: the
+// workgroup barrier orders them before thread 0's release (agent scope,
+// which writes back the XCD's L2) and its arrival; thread 0 acquires (agent
+// scope, which invalidates the CU's and the XCD's stale lines) before the
+// second workgroup barrier releases the other threads.  The wait is
+// bounded: after `spin_limit` polls, or once another workgroup has given
+// up, the workgroup sets `err` and returns false; the caller then leaves
+// its loop, so every wave of the grid reaches the end of the kernel.
+__device__ __forceinline__ bool grid_sync(GridBarrier* bar, unsigned int nwg,
+                                          unsigned int spin_limit) {
+  __shared__ int ok_s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned int g = blockIdx.x % 8u;
+    const unsigned int gen = __hip_atomic_load(&bar->gen[0], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    bool advanced = false;
+    const unsigned int a = __hip_atomic_fetch_add(&bar->group[g][0], 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+    if (a == nwg / 8u - 1u) {
+      __hip_atomic_store(&bar->group[g][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned int t = __hip_atomic_fetch_add(&bar->top[0], 1u, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+      if (t == 7u) {
+        __hip_atomic_store(&bar->top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bar->gen[0], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        advanced = true;
+      }
+    }
+    if (!advanced) {
+      unsigned int spins = 0;
+      // relaxed polls (an acquiring load would invalidate the L2 under the
+      // workgroups still working); the acquire fence below orders the reads
+      while (__hip_atomic_load(&bar->gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             gen) {
+        ++spins;
+        if (spins > spin_limit ||
+            ((spins & 63u) == 0u && __hip_atomic_load(&bar->err[0], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) != 0u)) {
+          __hip_atomic_store(&bar->err[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    ok_s = ok;
+  }
+  __syncthreads();
+  return ok_s != 0;
 }
 
 // Combines per-block partials (nb blocks × nacc terms) in block order into

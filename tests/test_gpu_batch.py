@@ -295,6 +295,63 @@ def test_batch_block_graph_replays_equal_stream_launches(K):
             assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
 
 
+@pytest.mark.parametrize("K,sched", [(8, (256, 256, 0, 32, 32)), (16, (512, 256, 0, 0, 0)),
+                                     (5, (512, 512, 4, 128, 128))])
+def test_batch_cooperative_blocks_equal_stream_launches(K, sched):
+    """Blocks whose middle iterations run in one cooperative launch (grid
+    barriers between the passes) give the same trajectories, bit for bit, as
+    the same passes launched one by one: trajectories that start, end,
+    restart and start again at different blocks, with evaluation
+    frequencies of 1, 2, 3 and 40 (no cooperative launch, none, one
+    iteration in it, the usual case)."""
+    print("batch cooperative", K, sched, flush=True)
+    m = G.mixed(2000, 4000, 1)
+    pp = projector_problem(m)
+    tg = [G.binary_targets(m, 900 + k) / pp.D_c for k in range(K + 3)]
+    for freq in (40, 3, 2, 1):
+        loose = SolverParameters(eps_optimal_relative=1e-3, eps_feasible_relative=1e-4,
+                                 termination_evaluation_frequency=freq)
+        tight = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                                 iteration_limit=200_000 if freq > 3 else 4_000,
+                                 termination_evaluation_frequency=freq)
+        runs = []
+        for coop in (False, True):
+            print("  freq", freq, "cooperative", coop, flush=True)
+            bt = _batch(pp, K)
+            bt.set_kernel_path("general")
+            bt.set_schedule(*sched)
+            bt.set_cooperative(coop)
+            queue = list(range(K + 3))
+            ended = []
+            running = [True] * K
+            for k in range(K):
+                bt.set_target(k, tg[queue.pop(0)])
+                bt.begin(k, tight if k % 2 else loose, 0.6 + 0.1 * k)
+            while any(running):
+                terms, _ = bt.run(until_any=True)
+                for k in range(K):
+                    if not running[k] or terms[k] == "UNSPECIFIED":
+                        continue
+                    assert terms[k] in ("OPTIMAL", "ITERATION_LIMIT")
+                    info = bt.info(k)
+                    ended.append((k, terms[k], info.total_count, info.n_restarts,
+                                  info.l1_distance, bt.vector(k, "pdhg_x"),
+                                  bt.vector(k, "pdhg_z")))
+                    running[k] = False
+                    if queue:
+                        bt.set_target(k, tg[queue.pop(0)])
+                        bt.begin(k, loose, 1.0)
+                        running[k] = True
+            if coop:
+                assert bt.time_iterations(50) > 0.0
+            bt.close()
+            runs.append(ended)
+        assert len(runs[0]) == K + 3 == len(runs[1])
+        for a, b in zip(runs[0], runs[1]):
+            assert a[:5] == b[:5], (freq, a[:5], b[:5])
+            assert np.array_equal(a[5], b[5]) and np.array_equal(a[6], b[6]), (freq, a[0])
+
+
 @pytest.mark.parametrize("K", [8, 16])
 def test_batch_small_instance_trajectories_equal_single_projector_bitwise(K):
     """When the instance fits in LDS, the batch runs one workgroup per running
