@@ -480,10 +480,6 @@ def measured_traffic(workload: str, kernel_key: str):
     block = text.split("```json", 1)[1].split("```", 1)[0]
     recs = json.loads(block)
     rec = recs.get(kernel_key)
-    if rec is None and "<8, " in kernel_key:
-        # summaries taken before the batch width became a template argument
-        # name the same width-8 kernels without it
-        rec = recs.get(kernel_key.replace("<8, ", "<"))
     return None if rec is None else rec["traffic_bytes"]
 
 
