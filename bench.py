@@ -190,9 +190,12 @@ def run_batch(bt, pp, milp, target_sets, tol):
     return iters, time.perf_counter() - t0, statuses
 
 
-def cpu_baseline(pp, milp, target, iterations: int):
+def cpu_baseline(pp, milp, target, iterations: int, threads: int = 1):
     """The FP64 C oracle (the reference's CPU path restated) on one core:
-    PDHG iterations of one projection of the same workload, ‖A‖₂ excluded."""
+    PDHG iterations of one projection of the same workload, ‖A‖₂ excluded.
+    threads > 1: the same with each step's products and elementwise updates
+    split over that many host threads (bit-identical; not the reference's
+    path)."""
     from oracle import oracle as O
     from pdfp_amd.pdfp import SolverParameters
 
@@ -202,13 +205,24 @@ def cpu_baseline(pp, milp, target, iterations: int):
                          iteration_limit=iterations)
     pw = float(np.sqrt(1.0 + len(milp.int_idx) / max(len(milp.b), 1)))
     o.initialize(p, pw)                       # power iteration, untimed
-    t0 = time.perf_counter()
-    O.pdfp_solve(o, p, pw)
-    dt = time.perf_counter() - t0
+    O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        O.pdfp_solve(o, p, pw)
+        dt = time.perf_counter() - t0
+    finally:
+        O.set_threads(1)
     it = o.total_count
-    return {"value": it / dt, "unit": "PDHG iter/s", "cores": 1, "kind": "port",
-            "sample": f"{it} PDHG iterations of one projection of the same workload "
-                      f"(FP64 C oracle, single thread, ||A||_2 excluded), {dt:.1f} s"}
+    if threads == 1:
+        return {"value": it / dt, "unit": "PDHG iter/s", "cores": 1, "kind": "port",
+                "sample": f"{it} PDHG iterations of one projection of the same workload "
+                          f"(FP64 C oracle, single thread, ||A||_2 excluded), {dt:.1f} s"}
+    return {"value": it / dt, "unit": "PDHG iter/s", "cores": threads, "kind": "port",
+            "not_the_reference": True,
+            "sample": f"{it} PDHG iterations of one projection of the same workload (FP64 C "
+                      f"oracle, each step's products and updates split over {threads} "
+                      f"threads, bit-identical to one thread; the reference's CPU path is "
+                      f"single-threaded), {dt:.1f} s"}
 
 
 def _oracle_pump_backends():
@@ -706,6 +720,11 @@ def main():
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cpu_it = args.cpu_iterations or (400 if nnz > 2_000_000 else 1000)
         out["cpu_baseline"] = cpu_baseline(pp, milp, targets[0], cpu_it)
+        # the host's share of cores on the box (at most 16 threads)
+        thr = max(1, min(16, len(os.sched_getaffinity(0))))
+        if thr > 1:
+            print(f"[bench] cpu baseline, {thr} threads", file=sys.stderr, flush=True)
+            out["cpu_baseline_threaded"] = cpu_baseline(pp, milp, targets[0], 2 * cpu_it, thr)
     if not args.no_ttff:
         print("[bench] time to first feasible", file=sys.stderr, flush=True)
         out["time_to_first_feasible"] = time_to_first_feasible(3, cpu=not args.no_cpu,

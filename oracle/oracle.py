@@ -106,6 +106,7 @@ def load() -> C.CDLL:
         "orc_max_singular_value": (C.c_double, [C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32,
                                                 C.c_double]),
         "orc_spmv": (None, [C.c_int64, C.c_int64, _I64, _I64, _D, C.c_int32, _D, _D]),
+        "orc_set_threads": (None, [C.c_int32]),
         "orc_pdfp_create": (P, [C.c_int64, C.c_int64, _I64, _I64, _D, _D, _U8, _U8, _D, _D, _D]),
         "orc_pdfp_destroy": (None, [P]),
         "orc_pdfp_original_primal_residual": (None, [P, _D]),
@@ -206,6 +207,13 @@ def max_singular_value(A, max_iterations=5000, tolerance=1e-4) -> float:
     return load().orc_max_singular_value(m, n, colptr.ctypes.data_as(_I64),
                                          rowval.ctypes.data_as(_I64), _d(vals), max_iterations,
                                          tolerance)
+
+
+def set_threads(n: int) -> None:
+    """Host threads for the projector's step: its two products and three
+    elementwise updates (1, the reference's single-threaded CPU path, by
+    default); the iterates stay bit-identical."""
+    load().orc_set_threads(int(n))
 
 
 def spmv(A, x, transpose=False) -> np.ndarray:

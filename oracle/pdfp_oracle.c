@@ -97,7 +97,15 @@ typedef struct csc {
   int64_t* colptr; /* n+1 */
   int64_t* rowval; /* nnz */
   double* nzval;   /* nnz */
+  int sorted;      /* row indices strictly ascend within every column */
 } csc;
+
+static int csc_rows_sorted(const csc* A) {
+  for (int64_t j = 0; j < A->n; j++)
+    for (int64_t p = A->colptr[j] + 1; p < A->colptr[j + 1]; p++)
+      if (A->rowval[p] <= A->rowval[p - 1]) return 0;
+  return 1;
+}
 
 static csc csc_copy(int64_t m, int64_t n, const int64_t* colptr,
                     const int64_t* rowval, const double* nzval) {
@@ -111,6 +119,7 @@ static csc csc_copy(int64_t m, int64_t n, const int64_t* colptr,
     memcpy(A.rowval, rowval, sizeof(int64_t) * (size_t)A.nnz);
     memcpy(A.nzval, nzval, sizeof(double) * (size_t)A.nnz);
   }
+  A.sorted = csc_rows_sorted(&A);
   return A;
 }
 
@@ -134,6 +143,7 @@ static csc csc_transpose(const csc* A) {
       T.nzval[q] = A->nzval[p];
     }
   free(next);
+  T.sorted = csc_rows_sorted(&T);
   return T;
 }
 
@@ -149,6 +159,33 @@ static void spmv(const csc* A, const double* x, double* y) {
     double xj = x[j];
     for (int64_t p = A->colptr[j]; p < A->colptr[j + 1]; p++)
       y[A->rowval[p]] += A->nzval[p] * xj;
+  }
+}
+
+/* Host threads for the projector's step (orc_set_threads; 1, the
+ * reference's single-threaded CPU path, by default): its two products and
+ * its three elementwise updates, each element computed by one thread as the
+ * serial loop computes it (the reductions stay serial), so the iterates are
+ * bit-identical.  A timing option with no counterpart in the reference. */
+static int g_threads = 1;
+EXPORT void orc_set_threads(int32_t n) { g_threads = n < 1 ? 1 : n; }
+
+/* The same y = A x with g_threads > 1: each y[i] is summed by one thread
+ * from AT = Aᵀ's column i (A's row i, its entries in ascending column order
+ * when A's row indices ascend within each column, as csc_transpose and
+ * every canonical CSC give them), starting from 0.0 — the order in which
+ * the scatter adds them, so the result is bit-identical. */
+static void spmv_par(const csc* A, const csc* AT, const double* x, double* y) {
+  if (g_threads <= 1 || !AT->sorted) {
+    spmv(A, x, y);
+    return;
+  }
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1024)
+  for (int64_t i = 0; i < A->m; i++) {
+    double acc = 0.0;
+    for (int64_t p = AT->colptr[i]; p < AT->colptr[i + 1]; p++)
+      acc += AT->nzval[p] * x[AT->rowval[p]];
+    y[i] = acc;
   }
 }
 
@@ -545,7 +582,8 @@ static void pdfp_primal_step(orc_pdfp* s, int64_t k_offset, double omega) {
   int64_t k = s->r.inner_count + k_offset;
   double weight = (double)k / ((double)k + 1.0), omw = 1.0 - omega,
          omwt = 1.0 - weight;
-  spmv(&s->At, s->current_z, s->Atz);
+  spmv_par(&s->At, &s->A, s->current_z, s->Atz);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (int64_t j = 0; j < s->n; j++) {
     double temp = s->current_x[j] - tau * (s->Atz[j] - s->current_y[j]);
     double px = jclamp(temp, s->var_lb[j], s->var_ub[j]);
@@ -563,6 +601,7 @@ static void pdfp_dual_y_step(orc_pdfp* s, int64_t k_offset, double omega) {
   int64_t k = s->r.inner_count + k_offset;
   double weight = (double)k / ((double)k + 1.0), omw = 1.0 - omega,
          omwt = 1.0 - weight;
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (int64_t j = 0; j < s->n; j++) {
     double y = s->is_int[j]
                    ? jclamp(s->current_y[j] + sigma * (s->x_tilde[j] - s->reflected_x[j]),
@@ -581,7 +620,8 @@ static void pdfp_dual_z_step(orc_pdfp* s, int64_t k_offset, double omega) {
   int64_t k = s->r.inner_count + k_offset;
   double weight = (double)k / ((double)k + 1.0), omw = 1.0 - omega,
          omwt = 1.0 - weight;
-  spmv(&s->A, s->reflected_x, s->Ax);
+  spmv_par(&s->A, &s->At, s->reflected_x, s->Ax);
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1) schedule(static)
   for (int64_t i = 0; i < s->m; i++) {
     double raw = s->current_z[i] + sigma * (s->Ax[i] - s->b[i]);
     double z = s->is_eq[i] ? raw : jmax(raw, 0.0);
@@ -606,7 +646,7 @@ static double pdfp_fixed_point_error(orc_pdfp* s) {
   for (int64_t i = 0; i < m; i++) s->delta_z[i] = s->pdhg_z[i] - s->reflected_z[i];
   double movement = sumsq(s->delta_x, n) * s->r.primal_weight +
                     (sumsq(s->delta_y, n) + sumsq(s->delta_z, m)) / s->r.primal_weight;
-  spmv(&s->At, s->delta_z, s->Atz);
+  spmv_par(&s->At, &s->A, s->delta_z, s->Atz);
   for (int64_t j = 0; j < n; j++) s->tmp_n[j] = s->Atz[j] - s->delta_y[j];
   double interaction = 2.0 * s->step_size * dotp(s->tmp_n, s->delta_x, n);
   s->r.fixed_point_error = sqrt(jmax(movement + interaction, 0.0));
@@ -637,7 +677,7 @@ static void pdfp_perform_restart(orc_pdfp* s, const orc_params* p) {
 static void pdfp_residual_of(orc_pdfp* s, const double* x, const double* y,
                              const double* z, const double* ds, double* out) {
   int64_t n = s->n, m = s->m;
-  spmv(&s->A, x, s->Ax);
+  spmv_par(&s->A, &s->At, x, s->Ax);
   for (int64_t i = 0; i < m; i++) {
     double d = s->Ax[i] - s->b[i];
     s->tmp_m[i] = s->is_eq[i] ? fabs(d) : jmax(d, 0.0);
@@ -647,7 +687,7 @@ static void pdfp_residual_of(orc_pdfp* s, const double* x, const double* y,
   double b_norm = s->original_primal_residual ? s->b_norm_original
                                               : sqrt(sumsq(s->b, m));
   out[0] = sqrt(sumsq(s->tmp_m, m)) / (1.0 + b_norm);
-  spmv(&s->At, z, s->Atz);
+  spmv_par(&s->At, &s->A, z, s->Atz);
   for (int64_t j = 0; j < n; j++) s->tmp_n[j] = s->Atz[j] - y[j] - ds[j];
   out[1] = sqrt(sumsq(s->tmp_n, n)) / s->dual_residual_denominator;
   double l1 = 0.0, y_term = 0.0, z_term = 0.0;

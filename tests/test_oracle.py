@@ -217,6 +217,43 @@ def test_oracle_spmv_matches_scipy():
     np.testing.assert_allclose(O.spmv(m.A, z, transpose=True), m.A.T @ z, rtol=1e-14, atol=1e-14)
 
 
+@pytest.mark.parametrize("unsorted", [False, True])
+def test_threaded_products_are_bit_identical(unsorted):
+    """The oracle's optional threaded step (orc_set_threads, the all-core
+    CPU baseline) sums each row in the scatter's order: a projection
+    with 4 threads gives the single-threaded iterates, counts and distance
+    bit for bit.  With the caller's row indices out of order within a
+    column the threaded Aᵀ z would sum its rows in another order, so that
+    product falls back to the scatter (the same bits again)."""
+    from pdfp_amd.pdfp import projector_problem
+
+    m = G.mixed(300, 700, 4, power_law=True, nnz=6000, max_len=200)
+    pp = projector_problem(m, scaling="host")
+    A = pp.A.tocsc()
+    if unsorted:  # reverse every column's entries: the same matrix, unsorted
+        A = A.copy()
+        for j in range(A.shape[1]):
+            a, b = A.indptr[j], A.indptr[j + 1]
+            A.indices[a:b] = A.indices[a:b][::-1].copy()
+            A.data[a:b] = A.data[a:b][::-1].copy()
+        A.has_sorted_indices = False
+    xt = G.binary_targets(m, 3) / pp.D_c
+    p = SolverParameters(eps_optimal_relative=1e-6, eps_feasible_relative=1e-6,
+                         iteration_limit=3000)
+    runs = []
+    for threads in (1, 4):
+        O.set_threads(threads)
+        try:
+            o = O.OraclePDFPState(A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+            O.set_target(o, xt)
+            O.pdfp_solve(o, p, 1.0)
+            runs.append((o.total_count, o.info().l1_distance, o.pdhg_x, o.vector("pdhg_z")))
+        finally:
+            O.set_threads(1)
+    assert runs[0][0] == runs[1][0] and runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][2], runs[1][2]) and np.array_equal(runs[0][3], runs[1][3])
+
+
 def test_start_vector_is_standard_normal():
     v = O.start_vector(200_000)
     assert abs(v.mean()) < 0.01 and abs(v.std() - 1.0) < 0.01
