@@ -152,6 +152,27 @@ def kernel_bytes_batch(m: int, n: int, nnz: int, K: int = 8) -> tuple[float, flo
     return float(at), float(a)
 
 
+def slice_rows(A) -> int:
+    """Rows of the stacked column-slice matrix of the batch's sliced A pass:
+    the (row, slice) pairs of A with a nonzero, slices of n/8 columns."""
+    csr = A.tocsr()
+    n = csr.shape[1]
+    rows = np.repeat(np.arange(csr.shape[0], dtype=np.int64), np.diff(csr.indptr))
+    sl = (csr.indices.astype(np.int64) * 8) // max(n, 1)
+    return int(np.unique(rows * 8 + sl).size)
+
+
+def kernel_bytes_sliced_a(m: int, n: int, nnz: int, T: int, K: int = 8) -> float:
+    """Algorithmic bytes of the batch's A pass over 8 column slices (the
+    layout's own figure): the slices' pass 8·nnz + 4(T + 9) (stacked row
+    pointers) + 4T (each stacked row's partial position) + x̄ read once r·n
+    + its partials written r·T; the combine 8·m·r (the 8 partials of every
+    row) + z, z⁰, b, is_eq (2r + 5)·m + write z r·m."""
+    r = 4 * (8 if K <= 8 else 16)
+    return float(8 * nnz + 4 * (T + 9) + 4 * T + r * n + r * T + 8 * m * r + (2 * r + 5) * m
+                 + r * m)
+
+
 def run_batch(bt, pp, milp, target_sets, tol):
     """The K trajectory slots serve a queue of warm-started projections, the
     targets of all steps in order (step s, slot k): a slot whose projection
@@ -528,6 +549,9 @@ def main():
                     help="workgroups per CU of a cooperative launch running the middle "
                          "iterations of each block of the batched passes (-1: as many as "
                          "fit; 0: launch every pass)")
+    ap.add_argument("--column-slices", choices=("on", "off", "256", "512"), default=None,
+                    help="run the batch's A pass over A's 8 column slices (on) or in one "
+                         "pass (off) instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
@@ -608,6 +632,9 @@ def main():
             bt.set_block_graph(False)
         if args.cooperative is not None:
             bt.set_cooperative(args.cooperative)
+        if args.column_slices is not None:
+            bt.set_column_slices({"on": 1, "off": 0, "256": 256,
+                                  "512": 512}[args.column_slices])
         for k in range(K):
             bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]
@@ -628,13 +655,19 @@ def main():
                                      "A_persistent_wg_per_cu": bi.persist_a,
                                      "At_persistent_wg_per_cu": bi.persist_at,
                                      "A_lane_rows": bi.lane_rows_a,
-                                     "At_lane_rows": bi.lane_rows_at})
+                                     "At_lane_rows": bi.lane_rows_at,
+                                     "A_column_slices": bi.column_slices})
         print(f"[bench] fp32 batch of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; batched iteration "
               f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)
         bt.close()
         head = results["batch"]
         bat, ba = kernel_bytes_batch(mm, nn, nnz, K)
+        if bi.column_slices:
+            # the A pass runs over column slices: its own layout's bytes (the
+            # one-pass figure stays beside them in the line)
+            head["row_pass_bytes_one_pass"] = ba
+            ba = kernel_bytes_sliced_a(mm, nn, nnz, slice_rows(pp.A), K)
         steps_desc = (f"{K} warm-started l1 projections to new integer targets at relative KKT "
                       f"tolerance {args.tol:g}, served by the {K} trajectory slots of one batch "
                       f"sharing A (a slot that finishes takes the next target)")
@@ -648,6 +681,8 @@ def main():
     width = 8 if K <= 8 else 16
     keys = ((f"EpiBatchPrimalY<{width}, false>", f"EpiBatchDualZ<{width}, false>") if K > 1 else
             ("EpiPrimalY<float, false>", "EpiDualZ<float, false>"))
+    if K > 1 and head["npb"].get("A_column_slices"):
+        keys = (keys[0], f"EpiSlicePartial<{width}>")
     dom_bytes, dom_ms, dom_name, dom_key = max(
         (bat, head["ms_at"], "transpose pass (A^T z + x/y update)", keys[0]),
         (ba, head["ms_a"], "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])
@@ -700,6 +735,9 @@ def main():
     }
     if K > 1:
         out["batch_schedule_nnz_per_block"] = head["npb"]
+        if "row_pass_bytes_one_pass" in head:
+            out["row_pass_bytes"] = {"column_slices": ba,
+                                     "one_pass": head["row_pass_bytes_one_pass"]}
     if "fp64" in results:
         r64 = results["fp64"]
         v64 = r64["iters"] / r64["seconds"]

@@ -115,6 +115,8 @@ typedef struct pdfp_info {
                              /*   trajectory); 0: none (autotuned)              */
   int32_t block_graph;       /* 1: the last solve replayed each block of the    */
                              /*   general passes as a captured HIP graph        */
+  int32_t column_slices;     /* batch: 8 when the A pass runs over 8 column     */
+                             /*   slices of A, one per XCD; 0 otherwise         */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */
@@ -266,6 +268,18 @@ int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on);
  * grid barriers cost more than the kernel boundaries they replace (75 vs
  * 24 µs per iteration at 1M nonzeros), hence the default. */
 int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu);
+/* Column slices of the batch's A pass (no counterpart in the reference; it
+ * replaces the CSC product A x̄ of src/pdfp_kernels.jl:81 when x̄ outgrows an
+ * XCD's L2): 1 (or 256, 512: the nonzeros per workgroup of the slices'
+ * pass; 1 keeps the last) runs A x̄ as 8 partial products over A's 8 column slices,
+ * each walked by one XCD so that its gathers of x̄ come from that XCD's L2,
+ * then adds the 8 partials (FP32) per row in FP64 and updates z; 0 runs A x̄
+ * in one pass.  pdfp_batch_create's autotuner turns it on where an
+ * iteration is faster with it (x̄ larger than 4 MB and an eighth of it at
+ * most 5 MB); pdfp_info's column_slices says which ran.  The partial sums
+ * are rounded to FP32, so the iterates differ from the one-pass form's in
+ * the last bits. */
+int pdfp_batch_set_column_slices(pdfp_batch* ctx, int32_t on);
 /* Kernel path of the batch (no counterpart in the reference), as
  * pdfp_set_kernel_path: 0 (the default) runs each block as one launch with
  * one workgroup per running trajectory when the instance fits in LDS (each

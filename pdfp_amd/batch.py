@@ -155,6 +155,15 @@ class PDFPBatch:
         n = -1 if wg_per_cu is True else int(wg_per_cu)
         self._check(self._lib.pdfp_batch_set_cooperative(self._ctx, n))
 
+    def set_column_slices(self, on) -> None:
+        """Run A x̄ over A's 8 column slices, one per XCD (FP32 partial sums
+        added in FP64), or in one pass (False).  256 or 512 also sets the
+        nonzeros per workgroup of the slices' pass.  The autotuner picks the
+        faster where x̄ outgrows an XCD's L2; the iterates differ in the last
+        bits."""
+        v = on if isinstance(on, int) and not isinstance(on, bool) else int(bool(on))
+        self._check(self._lib.pdfp_batch_set_column_slices(self._ctx, v))
+
     def set_schedule(self, npb_a: int, npb_at: int, persist: int = 0, lane_rows_a: int = 0,
                      lane_rows_at: int = 0) -> None:
         """Replace the autotuned schedule of the batched passes (tests and

@@ -93,6 +93,7 @@ struct BatchBase {
   bool coop_on = false;
   virtual ~BatchBase() = default;
   virtual void set_coop(int per_cu) = 0;
+  virtual void set_slices(int on) = 0;
   virtual void set_original_primal_residual(const double* D_r) = 0;
   virtual void set_x(int k, const double* xs) = 0;
   virtual void set_target(int k, const double* xts) = 0;
@@ -391,6 +392,10 @@ struct Batch final : BatchBase {
     EpiBatchPrimalY<KB, MAJOR> ex{d_sp.p, d_wt.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
                               rx.p, px.p, ds.p, py.p, ry.p};
     launch_batch(At, GatherBatch<KB>{z.p}, ex);
+    if (slices_on) {
+      launch_sliced_a<MAJOR>(k_offset);
+      return;
+    }
     EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
     launch_batch(A, GatherBatch<KB>{rx.p}, ez);
   }
@@ -456,6 +461,26 @@ struct Batch final : BatchBase {
         }
       }
       M->schedule(M->npb, stream, kRows, bl);
+    }
+    // the A pass over column slices, where it can pay, at each block size;
+    // kept only if an iteration is at least 2% faster with it (the best of
+    // three timings of each form, taken in turn)
+    if (slices_pay(m, n, nnz)) {
+      build_slices();
+      float t_one = 1e30f, t_sl[2] = {1e30f, 1e30f};
+      const int npbs[2] = {512, 256};
+      for (int rep = 0; rep < 3; ++rep) {
+        slices_on = false;
+        t_one = std::min(t_one, time8());
+        slices_on = true;
+        for (int v = 0; v < 2; ++v) {
+          As.schedule(npbs[v], stream, kBlock);
+          t_sl[v] = std::min(t_sl[v], time8());
+        }
+      }
+      const int v = t_sl[0] <= t_sl[1] ? 0 : 1;
+      if (t_sl[v] < 0.98f * t_one) As.schedule(npbs[v], stream, kBlock);
+      else drop_slices();
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -769,6 +794,128 @@ struct Batch final : BatchBase {
     }
   }
 
+  // ---- the A pass over 8 column slices (slices_on) ----
+  // As: A's 8 column slices stacked (segment s: the rows with a nonzero in
+  // the columns of slice s, in row order), each walked by one XCD; part_s:
+  // the FP32 partial row sums, KB per (slice, row of A), at the position
+  // slice_dense gives each stacked row; k_batch_slice_dualz adds a row's 8
+  // partials and updates z.
+  DevCsr<float> As;
+  DevBuf<float> part_s;
+  DevBuf<int32_t> slice_dense;
+  bool slices_on = false;
+
+  // Column slices pay when x̄ (n·KB FP32 values) is larger than an XCD's
+  // 4 MB L2 and an eighth of it is not.
+  static bool slices_pay(int64_t m, int64_t n, int64_t nnz) {
+    const int64_t xbar = n * KB * 4;
+    return nnz >= kAutotuneMinNnz && m > 0 && xbar > (int64_t)4 << 20 &&
+           xbar / 8 <= (int64_t)5 << 20;
+  }
+
+  // Builds As from A's CSR (read back from the device once).
+  void build_slices() {
+    if (As.nrows > 0 || nnz == 0) return;
+    std::vector<int32_t> col((size_t)nnz);
+    std::vector<float> val((size_t)nnz);
+    A.col.download(col.data(), (size_t)nnz, stream);
+    A.val.download(val.data(), (size_t)nnz, stream);
+    sync();
+    const std::vector<int32_t>& rp = A.h_rowptr;
+    HostCsr h;
+    h.ncols = n;
+    h.rowptr.assign(1, 0);
+    h.col.reserve((size_t)nnz);
+    h.val.reserve((size_t)nnz);
+    std::vector<int32_t> cur(rp.begin(), rp.end() - 1);  // next entry of each row
+    std::vector<int32_t> srow;                            // A's row of each stacked row
+    As.segments.assign(1, 0);
+    for (int s = 0; s < 8; ++s) {
+      const int64_t hi = n * (s + 1) / 8;  // columns [n·s/8, hi) form slice s
+      for (int64_t i = 0; i < m; ++i) {
+        int32_t k = cur[(size_t)i];
+        const int32_t end = rp[(size_t)i + 1];
+        if (k == end || col[(size_t)k] >= hi) continue;  // no nonzero in this slice
+        for (; k < end && col[(size_t)k] < hi; ++k) {     // a row's columns ascend
+          h.col.push_back(col[(size_t)k]);
+          h.val.push_back((double)val[(size_t)k]);
+        }
+        cur[(size_t)i] = k;
+        h.rowptr.push_back((int32_t)h.col.size());
+        srow.push_back((int32_t)i);
+      }
+      As.segments.push_back((int64_t)srow.size());
+    }
+    if ((int64_t)h.col.size() != nnz) throw Error(PDFP_ERR_STATE, "column slices lost nonzeros");
+    h.nrows = (int64_t)srow.size();
+    // each stacked row's (slice, row) position in the dense partials
+    std::vector<int32_t> dense(srow.size());
+    for (int s = 0; s < 8; ++s)
+      for (int64_t r = As.segments[(size_t)s]; r < As.segments[(size_t)s + 1]; ++r)
+        dense[(size_t)r] = (int32_t)(s * m + srow[(size_t)r]);
+    slice_dense.alloc(std::max<size_t>(dense.size(), 1));
+    slice_dense.upload(dense.data(), dense.size(), stream);
+    As.upload(h, stream);
+    As.schedule(512, stream, kBlock);
+    part_s.alloc((size_t)(8 * m * KB));
+    PDFP_HIP(hipMemsetAsync(part_s.p, 0, part_s.bytes(), stream));  // the empty pairs stay 0
+    sync();
+  }
+
+  void drop_slices() {
+    for (auto* d : {&As.rowptr, &As.col}) d->release();
+    As.val.release();
+    As.sched.release();
+    As.nrows = As.nblocks = 0;
+    part_s.release();
+    slice_dense.release();
+    slices_on = false;
+  }
+
+  void set_slices(int on) override {
+    if (on != 0 && on != 1 && on != 256 && on != 512)
+      throw Error(PDFP_ERR_ARGUMENT, "column slices: 0, 1, 256 or 512");
+    if (on && nnz == 0) throw Error(PDFP_ERR_ARGUMENT, "no nonzeros to slice");
+    sync();
+    drop_graphs();
+    if (on) build_slices();
+    if (on > 1) As.schedule(on, stream, kBlock);
+    slices_on = on != 0;
+  }
+
+  // A x̄ and the z update of iteration k_offset over the column slices; t0 /
+  // t1 (optional) bracket both launches.
+  template <bool MAJOR>
+  void launch_sliced_a(int k_offset, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+    SegBlocks sb{};
+    int maxb = 0;
+    for (int s = 0; s <= 8; ++s) sb.b[s] = As.seg_blocks[(size_t)s];
+    for (int s = 0; s < 8; ++s) maxb = std::max(maxb, sb.b[s + 1] - sb.b[s]);
+    const GatherBatch<KB> g{rx.p};
+    const EpiSlicePartial<KB> ep{slice_dense.p, part_s.p};
+    const dim3 grid((unsigned)(8 * std::max(maxb, 1))), block(kBlock);
+    if (As.npb == 512) {
+      auto kern = csr_pass_xcd_segments<kBlock, 512, float, GatherBatch<KB>, EpiSlicePartial<KB>, 0, 0>;
+      if (t0) hipExtLaunchKernelGGL(kern, grid, block, 0, stream, t0, nullptr, 0, As.view(), g, ep,
+                                    (double*)nullptr, sb);
+      else kern<<<grid, block, 0, stream>>>(As.view(), g, ep, nullptr, sb);
+    } else {
+      auto kern = csr_pass_xcd_segments<kBlock, 256, float, GatherBatch<KB>, EpiSlicePartial<KB>, 0, 0>;
+      if (t0) hipExtLaunchKernelGGL(kern, grid, block, 0, stream, t0, nullptr, 0, As.view(), g, ep,
+                                    (double*)nullptr, sb);
+      else kern<<<grid, block, 0, stream>>>(As.view(), g, ep, nullptr, sb);
+    }
+    PDFP_HIP(hipGetLastError());
+    EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    const int g2 = grid_for(m * KB);
+    if (t1)
+      hipExtLaunchKernelGGL(k_batch_slice_dualz<KB, MAJOR>, dim3(g2), dim3(256), 0, stream,
+                            nullptr, t1, 0, (int)m, (const float*)part_s.p, ez);
+    else
+      k_batch_slice_dualz<KB, MAJOR><<<g2, 256, 0, stream>>>((int)m, part_s.p, ez);
+    PDFP_HIP(hipGetLastError());
+  }
+
   // initialize_step_size_and_primal_weight! (src/pdfp.jl:220-241) for
   // trajectory k: it starts a solve with its own parameters.
   void begin(int k, const pdfp_params& p, double pw) override {
@@ -851,7 +998,7 @@ struct Batch final : BatchBase {
                               stream));
       small_used = small_ok && path == 0;
       if (small_used) small_block(freq, omega, active);
-      else if (coop_on) coop_block(freq, any_restart);
+      else if (coop_on && !slices_on) coop_block(freq, any_restart);
       else if (graph_on) block_graph(freq, any_restart);
       else general_block(freq, any_restart);
       for (int k = 0; k < K; ++k)
@@ -992,6 +1139,7 @@ struct Batch final : BatchBase {
     o->lane_rows_at = At.lane_len;
     o->block_graph = graph_on && !small_used ? 1 : 0;
     o->small_path = small_used ? 1 : 0;
+    o->column_slices = slices_on && !small_used ? 8 : 0;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched
@@ -1006,6 +1154,10 @@ struct Batch final : BatchBase {
                                 rx.p, px.p, ds.p, py.p, ry.p};
       launch_batch(At, GatherBatch<KB>{z.p}, ex,
           ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
+      if (slices_on) {
+        launch_sliced_a<false>(k, ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
+        continue;
+      }
       EpiBatchDualZ<KB, false> ez{d_sp.p, d_wt.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
       launch_batch(A, GatherBatch<KB>{rx.p}, ez,
           ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
@@ -1039,7 +1191,7 @@ struct Batch final : BatchBase {
     PDFP_HIP(hipEventCreate(&e0));
     PDFP_HIP(hipEventCreate(&e1));
     PDFP_HIP(hipEventRecord(e0, stream));
-    if (coop_on) {
+    if (coop_on && !slices_on) {
       // the cooperative form: runs of up to 198 iterations per launch
       for (int i = 0; i < iters; i += 198) launch_coop(2, 2 + std::min(198, iters - i) - 1);
       PDFP_HIP(hipMemcpyAsync(h_bar.p, bar.p, sizeof(GridBarrier), hipMemcpyDeviceToHost,
@@ -1254,6 +1406,10 @@ int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on) {
     if (on != 0 && on != 1) throw pdfp::Error(PDFP_ERR_ARGUMENT, "block graph switch must be 0 or 1");
     ctx->impl->graph_on = on != 0;
   });
+}
+
+int pdfp_batch_set_column_slices(pdfp_batch* ctx, int32_t on) {
+  return batch_guarded(ctx, [&] { ctx->impl->set_slices(on); });
 }
 
 int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu) {

@@ -177,6 +177,52 @@ struct EpiBatchDualZ {
   }
 };
 
+// ---- the A pass over column slices -------------------------------------------
+//
+// A x̄ as 8 partial products: the matrix stacked from A's 8 column slices
+// (slice s: A's rows restricted to columns [s·n/8, (s+1)·n/8), only the rows
+// with a nonzero there, slice after slice), each slice walked by one XCD
+// (csr_pass_xcd_segments), so that its gathers of x̄ come from the eighth of
+// x̄ that fits in that XCD's L2 instead of filling a line from memory per
+// nonzero.  Stacked row r (row i of A in slice s) stores its KB sums (FP64
+// over the slice's nonzeros in ascending column order), rounded to FP32, at
+// part[(s·m + i)·KB ...], dense[r] = s·m + i; the (slice, row) pairs with no
+// nonzero keep the zeros they were allocated with.  k_batch_slice_dualz adds
+// row i's 8 partials in slice order in FP64 and runs the z update.
+
+template <int KB>
+struct EpiSlicePartial {
+  struct Pre {
+    int32_t d;
+  };
+  const int32_t* __restrict__ dense;
+  float* __restrict__ part;
+  __device__ __forceinline__ Pre pre(int r) const { return {dense[r]}; }
+  __device__ __forceinline__ void operator()(int, const Pre& p, const double* s, double*) const {
+    float4* o = reinterpret_cast<float4*>(part + (size_t)p.d * KB);
+#pragma unroll
+    for (int h = 0; h < KB / 4; ++h)
+      o[h] = make_float4((float)s[4 * h], (float)s[4 * h + 1], (float)s[4 * h + 2],
+                         (float)s[4 * h + 3]);
+  }
+};
+
+template <int KB, bool MAJOR>
+__global__ __launch_bounds__(256) void k_batch_slice_dualz(int m, const float* __restrict__ part,
+                                                           EpiBatchDualZ<KB, MAJOR> ez) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)m * KB) return;
+  const int i = (int)(e / KB), q = (int)(e % KB);
+  const typename EpiBatchDualZ<KB, MAJOR>::Pre p = ez.pre(i, q);
+  float v[8];
+#pragma unroll
+  for (int sl = 0; sl < 8; ++sl) v[sl] = part[(size_t)sl * (size_t)m * KB + (size_t)e];
+  double sum = 0.0;
+#pragma unroll
+  for (int sl = 0; sl < 8; ++sl) sum += (double)v[sl];
+  ez(i, q, p, sum, nullptr);
+}
+
 // ---- the evaluation terms of trajectory k (views of the interleaved vectors) --
 
 // The terms of EpiEvalCol (pdfp_kernels.h) for trajectory k.

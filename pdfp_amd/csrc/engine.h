@@ -594,6 +594,27 @@ __global__ __launch_bounds__(BLOCK) void csr_pass_persistent(Csr<TV> A, Gather g
   walk_blocks<BLOCK, NPB, TV, Gather, Epi, NACC, NMAX>(A, g, epi, partials, lds);
 }
 
+// Segmented form for a matrix stacked from 8 row segments (the column slices
+// of one matrix; b[s] .. b[s+1] are segment s's row blocks): the
+// workgroups with blockIdx % 8 == s take segment s's blocks.  Workgroups are
+// dealt to the 8 XCDs round-robin, so each XCD walks one segment, and the
+// gathers of a segment whose columns span an eighth of the vector stay in
+// that XCD's L2.
+struct SegBlocks {
+  int32_t b[9];
+};
+
+template <int BLOCK, int NPB, typename TV, class Gather, class Epi, int NACC, int NMAX = 0>
+__global__ __launch_bounds__(BLOCK) void csr_pass_xcd_segments(Csr<TV> A, Gather g, Epi epi,
+                                                               double* __restrict__ partials,
+                                                               SegBlocks sb) {
+  __shared__ __align__(16) unsigned char lds[PassLds<BLOCK, NPB, Gather, Epi>::kBytes];
+  const int s = blockIdx.x % 8;
+  const int b = sb.b[s] + (int)(blockIdx.x / 8);
+  if (b < sb.b[s + 1])
+    pass_block<BLOCK, NPB, TV, Gather, Epi, NACC, NMAX>(A, g, epi, partials, b, lds, threadIdx.x);
+}
+
 // ---- grid barrier of a cooperative launch -------------------------------------
 
 // State of a grid barrier in device memory, zeroed once when it is

@@ -28,10 +28,10 @@ import json
 
 
 def pass_key(name: str) -> str | None:
-    """The epilogue of a csr_pass / csr_pass_persistent instantiation (its
-    fifth template argument), e.g. 'EpiBatchPrimalY<false>'; None for other
-    kernels."""
-    for head in ("csr_pass_persistent<", "csr_pass<"):
+    """The epilogue of a csr_pass / csr_pass_persistent /
+    csr_pass_xcd_segments instantiation (its fifth template argument), e.g.
+    'EpiBatchPrimalY<8, false>'; None for other kernels."""
+    for head in ("csr_pass_persistent<", "csr_pass_xcd_segments<", "csr_pass<"):
         i = name.find(head)
         if i >= 0:
             break

@@ -244,6 +244,55 @@ def test_batch_schedules_agree(K):
             assert abs(l1b - ref[k]) <= 1e-4 * max(1.0, ref[k]), (sched, k, l1b, ref[k])
 
 
+@pytest.mark.parametrize("K", [8, 16, 5])
+def test_batch_column_slices_agree(K):
+    """The A pass over A's 8 column slices (one per XCD, FP32 partial sums
+    added in FP64) gives the same projections as the one-pass form: oracle
+    KKT within 10×tol and the single FP32 projector's ℓ1 distance to 1e-4,
+    on power-law rows with a dense column; the replayed block graphs equal
+    the stream launches bit for bit."""
+    print("batch column slices", K, flush=True)
+    m = _with_dense_column(G.mixed(2000, 5000, 5, power_law=True, nnz=60_000, max_len=3000))
+    pp = projector_problem(m)
+    tol = 1e-5
+    p = SolverParameters(eps_optimal_relative=tol, eps_feasible_relative=tol,
+                         iteration_limit=200_000)
+    targets = [G.binary_targets(m, 60 + k) / pp.D_c for k in range(K)]
+    ref = []
+    for k in range(K):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, targets[k])
+        assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
+        ref.append(s.info().l1_distance)
+    runs = []
+    for graph in (True, False):
+        bt = _batch(pp, K)
+        bt.set_kernel_path("general")
+        bt.set_schedule(512, 256, 0, 32, 32)
+        bt.set_column_slices(True)
+        bt.set_block_graph(graph)
+        for k in range(K):
+            bt.set_target(k, targets[k])
+        assert bt.solve(p, 1.0) == ["OPTIMAL"] * K
+        assert bt.info(0).column_slices == 8
+        run = []
+        for k in range(K):
+            x, z = bt.pdhg_x(k), bt.vector(k, "pdhg_z")
+            r = _oracle_kkt(pp, targets[k], x, bt.vector(k, "pdhg_y"), z,
+                            bt.vector(k, "dual_slack"))
+            assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, (k, r)
+            l1b = bt.info(k).l1_distance
+            assert abs(l1b - ref[k]) <= 1e-4 * max(1.0, ref[k]), (k, l1b, ref[k])
+            run.append((bt.info(k).total_count, x, z))
+        bt.set_column_slices(False)
+        assert bt.info(0).column_slices == 0
+        bt.close()
+        runs.append(run)
+    for a, b in zip(*runs):
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
 @pytest.mark.parametrize("K", [8, 16])
 def test_batch_block_graph_replays_equal_stream_launches(K):
     """Blocks replayed as captured graphs give the same trajectories, bit for
