@@ -688,6 +688,10 @@ def main():
         (ba, head["ms_a"], "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, dom_key)
+    if traffic is not None and dom_key.startswith("EpiSlicePartial"):
+        # the sliced A pass is two launches: the slices' pass and the combine
+        combine = measured_traffic(args.workload, f"k_batch_slice_dualz<{width}, false>")
+        traffic = None if combine is None else traffic + combine
     # The random-gather floor of each pass: nnz gathers of one record per
     # column (Aᵀz gathers z: m records; A x̄ gathers x̄: n records).
     from pdfp_amd._lib import gather_ceiling_us

@@ -115,8 +115,9 @@ typedef struct pdfp_info {
                              /*   trajectory); 0: none (autotuned)              */
   int32_t block_graph;       /* 1: the last solve replayed each block of the    */
                              /*   general passes as a captured HIP graph        */
-  int32_t column_slices;     /* batch: 8 when the A pass runs over 8 column     */
-                             /*   slices of A, one per XCD; 0 otherwise         */
+  int32_t column_slices;     /* batch: 0, or the nonzeros per workgroup (256,   */
+                             /*   512) of the A pass over 8 column slices of A, */
+                             /*   one per XCD                                   */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */

@@ -1139,7 +1139,7 @@ struct Batch final : BatchBase {
     o->lane_rows_at = At.lane_len;
     o->block_graph = graph_on && !small_used ? 1 : 0;
     o->small_path = small_used ? 1 : 0;
-    o->column_slices = slices_on && !small_used ? 8 : 0;
+    o->column_slices = slices_on && !small_used ? As.npb : 0;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched

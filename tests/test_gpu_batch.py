@@ -275,7 +275,7 @@ def test_batch_column_slices_agree(K):
         for k in range(K):
             bt.set_target(k, targets[k])
         assert bt.solve(p, 1.0) == ["OPTIMAL"] * K
-        assert bt.info(0).column_slices == 8
+        assert bt.info(0).column_slices == 512
         run = []
         for k in range(K):
             x, z = bt.pdhg_x(k), bt.vector(k, "pdhg_z")
