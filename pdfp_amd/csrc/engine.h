@@ -290,7 +290,10 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
   // (more waves per SIMD instead)
   constexpr int U0 = (NPB / BLOCK) < 4 ? (NPB / BLOCK > 0 ? NPB / BLOCK : 1) : 4;
   constexpr int U = (kRaw && NV > 8) ? (U0 > 1 ? U0 / 2 : 1) : U0;
-  constexpr int UV = NV > 8 ? 2 : 4;  // the same for the long-row loop
+  // the long-row loop: a raw gather holds NV/4 float4 per nonzero in flight,
+  // so it keeps 8 / (NV/4) nonzeros per lane in flight (more would cost the
+  // whole kernel waves per SIMD); other gathers keep 4 (2 above 8 values)
+  constexpr int UV = kRaw ? (NV >= 16 ? 1 : (NV >= 8 ? 2 : 4)) : (NV > 8 ? 2 : 4);
   constexpr int NA = NACC > 0 ? NACC : 1;
   static_assert(!kRaw || (std::is_same<TV, float>::value && NV % 4 == 0),
                 "raw staging: FP32 matrix values, a multiple of 4 gathered values");
@@ -484,16 +487,23 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
     for (int q = 0; q < NV; ++q) s[q] = 0.0;
     for (int k0 = 0; k0 < cnt; k0 += BLOCK * UV) {
       int cc[UV];
-      TV vv[UV];
+      TV a[UV];
 #pragma unroll
       for (int u = 0; u < UV; ++u) {
         const int k = k0 + u * BLOCK + tid;
         const int kc = k < cnt ? k : cnt - 1;
         cc[u] = ld_stream(A.col + base + kc);
-        vv[u] = ld_stream(A.val + base + kc);
+        a[u] = ld_stream(A.val + base + kc);
       }
-      // adding +0.0 past the end leaves s unchanged (s starts at +0.0 and a
-      // sum from +0.0 never becomes -0.0)
+      // Past the end a lane reloads the last nonzero and takes a_ij = 0: its
+      // products are ±0.0, and adding them leaves s unchanged (s starts at
+      // +0.0, and a sum from +0.0 never becomes -0.0).  Zeroing the matrix
+      // value, not the products, keeps every gathered load unconditional:
+      // a select on each product let the compiler sink loads into branches,
+      // one gather in flight at a time.
+#pragma unroll
+      for (int u = 0; u < UV; ++u)
+        if (k0 + u * BLOCK + tid >= cnt) a[u] = TV(0);
       if constexpr (kRaw) {
         // the gathered values stay FP32 in registers until their products
         float4 gv[UV][NV / 4];
@@ -503,26 +513,23 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
           for (int h = 0; h < NV / 4; ++h) gv[u][h] = g.raw4(cc[u], h);
 #pragma unroll
         for (int u = 0; u < UV; ++u) {
-          const bool in = k0 + u * BLOCK + tid < cnt;
-          const double a = (double)vv[u];
 #pragma unroll
           for (int h = 0; h < NV / 4; ++h) {
-            s[4 * h + 0] += in ? a * (double)gv[u][h].x : 0.0;
-            s[4 * h + 1] += in ? a * (double)gv[u][h].y : 0.0;
-            s[4 * h + 2] += in ? a * (double)gv[u][h].z : 0.0;
-            s[4 * h + 3] += in ? a * (double)gv[u][h].w : 0.0;
+            const double au = (double)a[u];
+            s[4 * h + 0] += au * (double)gv[u][h].x;
+            s[4 * h + 1] += au * (double)gv[u][h].y;
+            s[4 * h + 2] += au * (double)gv[u][h].z;
+            s[4 * h + 3] += au * (double)gv[u][h].w;
           }
         }
       } else {
         double p[UV][NV];
 #pragma unroll
-        for (int u = 0; u < UV; ++u) g(cc[u], (double)vv[u], p[u]);
+        for (int u = 0; u < UV; ++u) g(cc[u], (double)a[u], p[u]);
 #pragma unroll
-        for (int u = 0; u < UV; ++u) {
-          const bool in = k0 + u * BLOCK + tid < cnt;
+        for (int u = 0; u < UV; ++u)
 #pragma unroll
-          for (int q = 0; q < NV; ++q) s[q] += in ? p[u][q] : 0.0;
-        }
+          for (int q = 0; q < NV; ++q) s[q] += p[u][q];
       }
     }
     double t[NV];
