@@ -97,6 +97,13 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// s + a·v for FP32 a and v: their product is exact in FP64 (24 + 24
+// significand bits), so one fused multiply-add rounds exactly as the
+// separate multiply and add do, in one instruction instead of two.
+__device__ __forceinline__ double add_exact(double s, float a, float v) {
+  return __builtin_fma((double)a, (double)v, s);
+}
+
 // Keeps v's words in registers, as loaded, until this point.  The epilogue
 // operands are loaded before the row slice streams in; without this the
 // compiler converts them right after their loads and so waits for them there,
@@ -403,12 +410,12 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
             const float a0 = sa[k], a1 = sa[k + 1], a2 = sa[k + 2], a3 = sa[k + 3];
             const float v0 = sraw[k * NV + q], v1 = sraw[(k + 1) * NV + q];
             const float v2 = sraw[(k + 2) * NV + q], v3 = sraw[(k + 3) * NV + q];
-            s += (double)a0 * (double)v0;
-            s += (double)a1 * (double)v1;
-            s += (double)a2 * (double)v2;
-            s += (double)a3 * (double)v3;
+            s = add_exact(s, a0, v0);
+            s = add_exact(s, a1, v1);
+            s = add_exact(s, a2, v2);
+            s = add_exact(s, a3, v3);
           }
-          for (; k < k1; ++k) s += (double)sa[k] * (double)sraw[k * NV + q];
+          for (; k < k1; ++k) s = add_exact(s, sa[k], sraw[k * NV + q]);
           hold(pre);
           epi(r0 + lr, q, pre, s, acc);
         }
@@ -442,15 +449,15 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
 #pragma unroll 1
         for (; k < k1; k += tpr) {
           if constexpr (kRaw) {
-            const double a = (double)sa[k];
+            const float a = sa[k];
             const float4* v4 = reinterpret_cast<const float4*>(sraw + k * NV);
 #pragma unroll
             for (int q4 = 0; q4 < NV / 4; ++q4) {
               const float4 v = v4[q4];
-              s[4 * q4 + 0] += a * (double)v.x;
-              s[4 * q4 + 1] += a * (double)v.y;
-              s[4 * q4 + 2] += a * (double)v.z;
-              s[4 * q4 + 3] += a * (double)v.w;
+              s[4 * q4 + 0] = add_exact(s[4 * q4 + 0], a, v.x);
+              s[4 * q4 + 1] = add_exact(s[4 * q4 + 1], a, v.y);
+              s[4 * q4 + 2] = add_exact(s[4 * q4 + 2], a, v.z);
+              s[4 * q4 + 3] = add_exact(s[4 * q4 + 3], a, v.w);
             }
           } else {
 #pragma unroll
@@ -515,11 +522,10 @@ __device__ __forceinline__ void pass_block(const Csr<TV>& A, const Gather& g, co
         for (int u = 0; u < UV; ++u) {
 #pragma unroll
           for (int h = 0; h < NV / 4; ++h) {
-            const double au = (double)a[u];
-            s[4 * h + 0] += au * (double)gv[u][h].x;
-            s[4 * h + 1] += au * (double)gv[u][h].y;
-            s[4 * h + 2] += au * (double)gv[u][h].z;
-            s[4 * h + 3] += au * (double)gv[u][h].w;
+            s[4 * h + 0] = add_exact(s[4 * h + 0], a[u], gv[u][h].x);
+            s[4 * h + 1] = add_exact(s[4 * h + 1], a[u], gv[u][h].y);
+            s[4 * h + 2] = add_exact(s[4 * h + 2], a[u], gv[u][h].z);
+            s[4 * h + 3] = add_exact(s[4 * h + 3], a[u], gv[u][h].w);
           }
         }
       } else {
