@@ -414,8 +414,11 @@ struct Batch final : BatchBase {
       A.persist = At.persist = 0;
       return;
     }
+    // every trajectory active at zero step, as a solve runs (inactive
+    // trajectories store no iterate, which times the passes wrong); the
+    // iterates are zeroed at the end, as created
     std::vector<bool> none((size_t)K, false);
-    upload_steps(1.0, none);
+    upload_steps(1.0, std::vector<bool>((size_t)K, true));
     hipEvent_t e0, e1;
     PDFP_HIP(hipEventCreate(&e0));
     PDFP_HIP(hipEventCreate(&e1));
@@ -464,12 +467,8 @@ struct Batch final : BatchBase {
     }
     // the A pass over column slices, where it can pay, at each block size;
     // kept only if an iteration is at least 2% faster with it (the best of
-    // three timings of each form, taken in turn).  These timings run every
-    // trajectory active (zero steps), as a solve does: with the trajectories
-    // inactive no iterate is stored, and the one-pass form then timed as
-    // slower than it runs.  The iterates are zeroed afterwards, as created.
+    // three timings of each form, taken in turn)
     if (slices_pay(m, n, nnz)) {
-      upload_steps(1.0, std::vector<bool>((size_t)K, true));
       build_slices();
       float t_one = 1e30f, t_sl[2] = {1e30f, 1e30f};
       const int npbs[2] = {512, 256};
@@ -485,11 +484,11 @@ struct Batch final : BatchBase {
       const int v = t_sl[0] <= t_sl[1] ? 0 : 1;
       if (t_sl[v] < 0.98f * t_one) As.schedule(npbs[v], stream, kBlock);
       else drop_slices();
-      upload_steps(1.0, none);
-      for (DevBuf<float>* d : {&x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry, &z, &z0, &pz, &rz})
-        PDFP_HIP(hipMemsetAsync(d->p, 0, d->bytes(), stream));
-      sync();
     }
+    upload_steps(1.0, none);
+    for (DevBuf<float>* d : {&x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry, &z, &z0, &pz, &rz})
+      PDFP_HIP(hipMemsetAsync(d->p, 0, d->bytes(), stream));
+    sync();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }
