@@ -450,12 +450,16 @@ int pdfp_time_spmv(pdfp_ctx* ctx, int32_t transpose, int32_t iterations, double*
 /* Device memory held by the context, in bytes. */
 int64_t pdfp_device_bytes(const pdfp_ctx* ctx);
 
-/* Kernel path of the FP32 form (no counterpart in the reference): 0 (the
- * default) runs each block of iterations in one workgroup when the instance
- * fits in LDS (4(m+n+2) + 16 nnz + 36 n + 13 m bytes ≤ 152 KiB on gfx950: the
- * device's per-workgroup LDS less 8 KiB), else the
- * general passes; 1 always runs the general passes.  The FP64 form ignores
- * it. */
+/* Kernel path (no counterpart in the reference).  0 (the default): the FP32
+ * form runs each block of iterations in one workgroup when the instance fits
+ * in LDS (4(m+n+2) + 16 nnz + 36 n + 13 m bytes ≤ 152 KiB on gfx950: the
+ * device's per-workgroup LDS less 8 KiB), else the general passes; the FP64
+ * form keeps the reference's five launches per iteration.  1: the general
+ * passes in either form.  2: one workgroup per block in either form when the
+ * instance fits (FP64: 4(m+n+2) + 24 nnz + 72 n + 25 m bytes), as for the
+ * FP64 finish of FP32 projections on small instances; the power iteration
+ * then runs in one workgroup too.  Set before the first solve: σ_max is
+ * computed once. */
 int pdfp_set_kernel_path(pdfp_ctx* ctx, int32_t path);
 
 /* Block graphs of the FP32 form (no counterpart in the reference): 1 (the

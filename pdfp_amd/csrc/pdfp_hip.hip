@@ -86,8 +86,10 @@ struct Projector final : ProjectorBase {
   double relative_l1_gap = INFINITY, dual_den = 1.0, l1_den = 1.0;
   double l1_distance = NAN;
   int termination = PDFP_UNSPECIFIED;
-  // FP32 form: whole blocks in one workgroup when the operators fit in LDS
-  // (small_kernels.h); path 1 forces the general passes.
+  // Whole blocks in one workgroup when the operators fit in LDS
+  // (small_kernels.h): path 0 in the FP32 form, path 2 in either form;
+  // path 1 forces the general passes (and path 0 keeps the FP64 form on the
+  // reference's launch structure).
   bool small_ok = false;
   int small_lanes_a = 1, small_lanes_t = 1;
   int path = 0;
@@ -118,11 +120,12 @@ struct Projector final : ProjectorBase {
     nnz = (int64_t)ha.col.size();
     A.upload(ha, stream);
     At.upload(ht, stream);
-    small_ok = kFused && m > 0 && n > 0 && nnz > 0 &&
-               small_lds_bytes(m, n, nnz) <= small_lds_max();
+    small_ok = m > 0 && n > 0 && nnz > 0 && small_lds_bytes<T>(m, n, nnz) <= small_lds_max();
     if (small_ok) {
       small_lanes_a = small_lanes(A.h_rowptr, kSmallBlock);
       small_lanes_t = small_lanes(At.h_rowptr, kSmallBlock);
+    }
+    if (small_ok && kFused) {
       // the general passes only run when forced: untuned schedules
       A.schedule(512, stream);
       At.schedule(512, stream);
@@ -418,12 +421,15 @@ struct Projector final : ProjectorBase {
     }
   }
 
+  // Whether blocks run in one workgroup (see small_ok).
+  bool small_path() const { return small_ok && (path == 2 || (path == 0 && kFused)); }
+
   // One block of `freq` iterations and its evaluation(s) in one workgroup.
   void small_block(int freq, bool first_major) {
-    if constexpr (kFused) {
-      SmallArgs a;
-      a.A = SmallCsr{A.rowptr.p, A.col.p, A.val.p};
-      a.At = SmallCsr{At.rowptr.p, At.col.p, At.val.p};
+    {
+      SmallArgsV<T> a;
+      a.A = SmallCsrT<T>{A.rowptr.p, A.col.p, A.val.p};
+      a.At = SmallCsrT<T>{At.rowptr.p, At.col.p, At.val.p};
       a.m = (int)m;
       a.n = (int)n;
       a.nnz = (int)nnz;
@@ -451,11 +457,11 @@ struct Projector final : ProjectorBase {
       a.ry = ry.p;
       a.pz = pz.p;
       a.rz = rz.p;
-      a.ec = EpiEvalCol<float>{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p};
-      a.er = EpiEvalRow<float>{pz.p, rz.p, b.p, z0.p, is_eq.p,
-                               orig_primal ? row_unscale.p : nullptr};
+      a.ec = EpiEvalCol<T>{px.p, rx.p, py.p, ry.p, ds.p, xt.p, w.p, x0.p, y0.p};
+      a.er = EpiEvalRow<T>{pz.p, rz.p, b.p, z0.p, is_eq.p, orig_primal ? row_unscale.p : nullptr};
       a.sums = sums.p;
-      k_small_block<kSmallBlock><<<1, kSmallBlock, (size_t)small_lds_bytes(m, n, nnz), stream>>>(a);
+      k_small_block<kSmallBlock, T>
+          <<<1, kSmallBlock, (size_t)small_lds_bytes<T>(m, n, nnz), stream>>>(a);
       PDFP_HIP(hipGetLastError());
     }
   }
@@ -506,7 +512,7 @@ struct Projector final : ProjectorBase {
   }
 
   void set_path(int p) override {
-    if (p != 0 && p != 1) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0 or 1");
+    if (p < 0 || p > 2) throw Error(PDFP_ERR_ARGUMENT, "kernel path must be 0, 1 or 2");
     path = p;
   }
 
@@ -543,8 +549,8 @@ struct Projector final : ProjectorBase {
 
   // estimate_max_singular_value  src/preconditioner.jl:108-125
   double max_singular_value(int max_iterations, double tolerance) override {
-    // FP32 form on a small instance: the whole iteration in one workgroup
-    if (kFused && small_ok && path == 0) {
+    // blocks in one workgroup: the whole power iteration in one too
+    if (small_path()) {
       const double s = small_sigma(A, At, m, n, max_iterations, tolerance, stream);
       if (!std::isnan(s)) return s;
     }
@@ -609,7 +615,7 @@ struct Projector final : ProjectorBase {
     n_restarts = 0;
     n_evals = 0;
     x64_valid = false;
-    small_used = small_ok && path == 0;
+    small_used = small_path();
     graph_used = kFused && !small_used && block_graph_on;
     while (r.total_count < p.iteration_limit) {
       upload_step_params(omega);

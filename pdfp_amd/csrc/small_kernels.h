@@ -1,17 +1,19 @@
-// Whole evaluation blocks of the fused FP32 form in one workgroup.
+// Whole evaluation blocks of the projector in one workgroup.
 //
 // The pump's small instances (the examples: p0201 is 133 × 201 with 1,923
 // nonzeros) make a PDHG iteration launch-bound: the reference issues five
-// launches per iteration (src/pdfp.jl:158-163), the general form here two,
-// each a few microseconds of launch and drain for well under a microsecond
-// of work.  When the instance fits in LDS, one launch of one workgroup runs
-// a whole block of `freq` iterations and its evaluation
-// (src/pdfp.jl:252-264):
+// launches per iteration (src/pdfp.jl:158-163), the general FP32 form here
+// two, each a few microseconds of launch and drain for well under a
+// microsecond of work.  When the instance fits in LDS, one launch of one
+// workgroup runs a whole block of `freq` iterations and its evaluation
+// (src/pdfp.jl:252-264).  The iterate type V is FP32 (the fused form, and
+// the batch's and pack's trajectories) or FP64 (the FP64 form, when its
+// caller asks for this path):
 //
-//   LDS: CSR(Aᵀ) and CSR(A) (int32 pointers and columns, FP32 values) and
-//        every iterate and problem vector the iteration touches (FP32): z,
-//        z⁰, b, is_eq per row; x, y, x⁰, y⁰, ℓ, u, x̃, w, x̄ per column.
-//        See small_lds_bytes.
+//   LDS: CSR(Aᵀ) and CSR(A) (int32 pointers and columns, values in V) and
+//        every iterate and problem vector the iteration touches (in V): z,
+//        z⁰, b per row; x, y, x⁰, y⁰, ℓ, u, x̃, w, x̄ per column; is_eq
+//        (bytes).  See small_lds_bytes.
 //   per iteration: a pass over the columns, s_j = Σ a·z (FP64), and the x /
 //        y steps; a barrier; a pass over the rows, s_i = Σ a·x̄, and the z
 //        step; a barrier.  Each pass gives every row (of Aᵀ or A) a group
@@ -55,8 +57,13 @@ inline int64_t small_lds_max() {
   return std::max<int64_t>(std::min<int64_t>((int64_t)bytes - 8192, PDFP_SMALL_LDS_MAX), 0);
 }
 
+// Values of both operators (2 nnz), 9 column and 3 row vectors in V, the
+// int32 row pointers and columns, and is_eq: for FP32, 4(n+1) + 4(m+1) +
+// 16 nnz + 36 n + 13 m bytes.
+template <typename V = float>
 inline int64_t small_lds_bytes(int64_t m, int64_t n, int64_t nnz) {
-  return 4 * (n + 1) + 4 * (m + 1) + 16 * nnz + 36 * n + 13 * m;
+  constexpr int64_t v = sizeof(V);
+  return (2 * v + 8) * nnz + 9 * v * n + 3 * v * m + 4 * (n + 1) + 4 * (m + 1) + m;
 }
 
 // Lanes per row L ∈ {1, 2, ..., 64} of one operator in the one-workgroup
@@ -80,35 +87,39 @@ inline int small_lanes(const std::vector<int32_t>& rowptr, int block) {
   return best;
 }
 
-struct SmallCsr {
+template <typename TV>
+struct SmallCsrT {
   const int32_t* __restrict__ ptr;
   const int32_t* __restrict__ col;
-  const float* __restrict__ val;
+  const TV* __restrict__ val;
 };
+using SmallCsr = SmallCsrT<float>;
 
 // The iterate vectors (x, y, z, their initial, reflected and pdhg copies,
 // x̃, the dual slack) are read at index·S: S = 1 for one instance's own
 // vectors, S = 8 for one trajectory's view of the batch's interleaved ones
 // (the pointers then start at the trajectory's slot).  The problem data (ℓ,
 // u, w, b, is_eq) are always dense.
-template <class EC, class ER>
+template <class EC, class ER, typename V = float>
 struct SmallArgsT {
-  SmallCsr A, At;
+  SmallCsrT<V> A, At;
   int m, n, nnz;
   int lanes_a, lanes_t;  // lanes per row of A and of Aᵀ (powers of two ≤ 64)
   int freq;         // iterations in this launch
   int first_major;  // iteration 1 is a major step, evaluated into slot 1
   const StepParams* sp;
-  // HBM vectors (FP32)
-  float *x, *y, *z;
-  const float *x0, *y0, *z0, *lb, *ub, *xt, *w, *b;
+  // HBM vectors (V)
+  V *x, *y, *z;
+  const V *x0, *y0, *z0, *lb, *ub, *xt, *w, *b;
   const uint8_t* is_eq;
-  float *rx, *px, *ds, *py, *ry, *pz, *rz;
+  V *rx, *px, *ds, *py, *ry, *pz, *rz;
   EC ec;
   ER er;
   double* sums;  // slot 0 at sums[0..16), slot 1 at sums[16..32)
 };
-using SmallArgs = SmallArgsT<EpiEvalCol<float>, EpiEvalRow<float>>;
+template <typename V>
+using SmallArgsV = SmallArgsT<EpiEvalCol<V>, EpiEvalRow<V>, V>;
+using SmallArgs = SmallArgsV<float>;
 
 
 template <int BLOCK, int NACC>
@@ -132,8 +143,8 @@ __device__ __forceinline__ void small_reduce(double (&acc)[NACC], double* __rest
 }
 
 // The evaluation of the last major step from the HBM copies it wrote.
-template <int BLOCK, int S, class EC, class ER>
-__device__ __forceinline__ void small_evaluate(const SmallArgsT<EC, ER>& a,
+template <int BLOCK, int S, class EC, class ER, typename V>
+__device__ __forceinline__ void small_evaluate(const SmallArgsT<EC, ER, V>& a,
                                                double* __restrict__ out, double* red) {
   const int tid = threadIdx.x;
   {
@@ -168,31 +179,32 @@ __device__ __forceinline__ void small_evaluate(const SmallArgsT<EC, ER>& a,
 }
 
 // One block of a.freq iterations of one instance in the calling workgroup;
-// smem holds small_lds_bytes(m, n, nnz) bytes, red the reduction scratch.
-template <int BLOCK, int S, class EC, class ER>
-__device__ __forceinline__ void small_block_body(const SmallArgsT<EC, ER>& a,
+// smem holds small_lds_bytes<V>(m, n, nnz) bytes, red the reduction scratch.
+// The V arrays come first, so each starts at a multiple of sizeof(V).
+template <int BLOCK, int S, class EC, class ER, typename V>
+__device__ __forceinline__ void small_block_body(const SmallArgsT<EC, ER, V>& a,
                                                  unsigned char* smem, double* red) {
   const int tid = threadIdx.x;
   const int n = a.n, m = a.m, nnz = a.nnz;
-  int32_t* t_ptr = reinterpret_cast<int32_t*>(smem);
+  V* t_val = reinterpret_cast<V*>(smem);
+  V* a_val = t_val + nnz;
+  V* s_z = a_val + nnz;
+  V* s_z0 = s_z + m;
+  V* s_b = s_z0 + m;
+  V* s_rx = s_b + m;
+  V* s_x = s_rx + n;
+  V* s_y = s_x + n;
+  V* s_x0 = s_y + n;
+  V* s_y0 = s_x0 + n;
+  V* s_lb = s_y0 + n;
+  V* s_ub = s_lb + n;
+  V* s_xt = s_ub + n;
+  V* s_w = s_xt + n;
+  int32_t* t_ptr = reinterpret_cast<int32_t*>(s_w + n);
   int32_t* a_ptr = t_ptr + (n + 1);
   int32_t* t_col = a_ptr + (m + 1);
   int32_t* a_col = t_col + nnz;
-  float* t_val = reinterpret_cast<float*>(a_col + nnz);
-  float* a_val = t_val + nnz;
-  float* s_z = a_val + nnz;
-  float* s_z0 = s_z + m;
-  float* s_b = s_z0 + m;
-  float* s_rx = s_b + m;
-  float* s_x = s_rx + n;
-  float* s_y = s_x + n;
-  float* s_x0 = s_y + n;
-  float* s_y0 = s_x0 + n;
-  float* s_lb = s_y0 + n;
-  float* s_ub = s_lb + n;
-  float* s_xt = s_ub + n;
-  float* s_w = s_xt + n;
-  uint8_t* s_eq = reinterpret_cast<uint8_t*>(s_w + n);
+  uint8_t* s_eq = reinterpret_cast<uint8_t*>(a_col + nnz);
 
   for (int k = tid; k <= n; k += BLOCK) t_ptr[k] = a.At.ptr[k];
   for (int k = tid; k <= m; k += BLOCK) a_ptr[k] = a.A.ptr[k];
@@ -241,15 +253,15 @@ __device__ __forceinline__ void small_block_body(const SmallArgsT<EC, ER>& a,
       const PrimalYOut o = primal_y_math(c, s, (double)s_x[j], (double)s_y[j], (double)s_x0[j],
                                          (double)s_y0[j], (double)s_lb[j], (double)s_ub[j],
                                          (double)s_xt[j], (double)s_w[j]);
-      s_x[j] = (float)o.x;
-      s_y[j] = (float)o.y;
-      s_rx[j] = (float)o.rx;
+      s_x[j] = (V)o.x;
+      s_y[j] = (V)o.y;
+      s_rx[j] = (V)o.rx;
       if (major) {
-        a.rx[(size_t)j * S] = (float)o.rx;
-        a.px[(size_t)j * S] = (float)o.px;
-        a.ds[(size_t)j * S] = (float)o.ds;
-        a.py[(size_t)j * S] = (float)o.py;
-        a.ry[(size_t)j * S] = (float)o.ry;
+        a.rx[(size_t)j * S] = (V)o.rx;
+        a.px[(size_t)j * S] = (V)o.px;
+        a.ds[(size_t)j * S] = (V)o.ds;
+        a.py[(size_t)j * S] = (V)o.py;
+        a.ry[(size_t)j * S] = (V)o.ry;
       }
     }
     __syncthreads();
@@ -267,10 +279,10 @@ __device__ __forceinline__ void small_block_body(const SmallArgsT<EC, ER>& a,
       if (i >= m || la != 0) continue;
       const DualZOut o = dual_z_math(c, s, (double)s_z[i], (double)s_z0[i], (double)s_b[i],
                                      s_eq[i] != 0);
-      s_z[i] = (float)o.z;
+      s_z[i] = (V)o.z;
       if (major) {
-        a.pz[(size_t)i * S] = (float)o.pz;
-        a.rz[(size_t)i * S] = (float)o.rz;
+        a.pz[(size_t)i * S] = (V)o.pz;
+        a.rz[(size_t)i * S] = (V)o.rz;
       }
     }
     __syncthreads();
@@ -288,8 +300,8 @@ __device__ __forceinline__ void small_block_body(const SmallArgsT<EC, ER>& a,
   }
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgs a) {
+template <int BLOCK, typename V>
+__global__ __launch_bounds__(BLOCK) void k_small_block(SmallArgsV<V> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[(BLOCK / kWave) * kEvalColTerms];
   small_block_body<BLOCK, 1>(a, smem, red);
@@ -311,13 +323,6 @@ __global__ __launch_bounds__(BLOCK) void k_small_pack(const SmallArgs* __restric
 // (each iteration is two SpMVs and three reductions, a few microseconds),
 // instead of seven launches per iteration.  The sums are FP64, in a fixed
 // order.  LDS: small_power_lds_bytes.
-
-template <typename TV>
-struct SmallCsrT {
-  const int32_t* __restrict__ ptr;
-  const int32_t* __restrict__ col;
-  const TV* __restrict__ val;
-};
 
 template <typename TV>
 inline int64_t small_power_lds_bytes(int64_t m, int64_t n, int64_t nnz) {

@@ -202,12 +202,15 @@ class PDFPState:
         return int(self._lib.pdfp_device_bytes(self._ctx))
 
     def set_kernel_path(self, path: str) -> None:
-        """FP32 form: "auto" runs each block of iterations in one workgroup
-        when the operators fit in LDS (the default); "general" always runs
-        the general passes."""
-        codes = {"auto": 0, "general": 1}
+        """"auto" (the default): the FP32 form runs each block of iterations
+        in one workgroup when the operators fit in LDS, the FP64 form the
+        reference's five launches per iteration; "general" always runs the
+        general passes; "one-workgroup" runs blocks (and the power iteration)
+        in one workgroup in either form when the instance fits.  Set it
+        before the first solve."""
+        codes = {"auto": 0, "general": 1, "one-workgroup": 2}
         if path not in codes:
-            raise ValueError("kernel path must be 'auto' or 'general'")
+            raise ValueError("kernel path must be 'auto', 'general' or 'one-workgroup'")
         self._check(self._lib.pdfp_set_kernel_path(self._ctx, codes[path]))
 
     def set_block_graph(self, on: bool) -> None:

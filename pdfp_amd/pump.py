@@ -32,6 +32,21 @@ from .rounding import BatchPolicy, Rounder, jround
 FP32_TOLERANCE_FLOOR = 1e-5
 
 
+def _fp64_successor(P, pp, mixed: bool, device: int):
+    """An FP64 projector context that carries on from FP32 projections (the
+    polish, or the FP64 form below the FP32 tolerance floor): on an instance
+    that fits in LDS its blocks and its power iteration run in one workgroup
+    (a user's "fp64" pump keeps the reference's launch structure).  A
+    substituted projector without kernel paths is used as it comes."""
+    st = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                     precision="fp64", device=device)
+    if hasattr(st, "set_kernel_path"):
+        st.set_kernel_path("one-workgroup")
+    if mixed:
+        P.original_primal_residual(st, pp.D_r)
+    return st
+
+
 @dataclass
 class PumpResult:
     """Outcome of feasibility_pump (src/pump.jl:22-35).
@@ -293,10 +308,7 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         if (getattr(st, "precision", None) == "fp32" and
                 min(params.eps_optimal_relative,
                     params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR):
-            st64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
-                               precision="fp64", device=device)
-            if mixed:
-                P.original_primal_residual(st64, pp.D_r)
+            st64 = _fp64_successor(P, pp, mixed, device)
             st64.set_x(st.pdhg_x)
             P.set_target(st64, proj["target"] if target_scaled is None else target_scaled)
             emit(f"projector   FP64 form from gap tolerance "
@@ -314,10 +326,7 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
             # x, y, z), the same target and tolerances
             st64 = proj.get("polish")
             if st64 is None:
-                st64 = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub,
-                                   pp.w, precision="fp64", device=device)
-                if mixed:
-                    P.original_primal_residual(st64, pp.D_r)
+                st64 = _fp64_successor(P, pp, mixed, device)
                 proj["polish"] = st64
             st64.set_x(st.pdhg_x)
             st64.set_dual(st.vector("pdhg_y"), st.vector("pdhg_z"))
@@ -627,10 +636,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
         point (x, y, z), on the host thread that then rounds it."""
         st = fin64[k]
         if st is None:
-            st = _pdfp_mod.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub,
-                                     pp.w, precision="fp64", device=device)
-            if mixed:
-                _pdfp_mod.original_primal_residual(st, pp.D_r)
+            st = _fp64_successor(_pdfp_mod, pp, mixed, device)
             fin64[k] = st
         st.set_x(x_s)
         st.set_dual(y, z)
@@ -680,10 +686,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             tight = min(params.eps_optimal_relative,
                         params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR
             if fp64[k] is None and tight:
-                st64 = _pdfp_mod.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb,
-                                           pp.var_ub, pp.w, precision="fp64", device=device)
-                if mixed:
-                    _pdfp_mod.original_primal_residual(st64, pp.D_r)
+                st64 = _fp64_successor(_pdfp_mod, pp, mixed, device)
                 st64.set_x(batch.pdhg_x(k))
                 _pdfp_mod.set_target(st64, target[k])
                 fp64[k] = st64
@@ -849,10 +852,7 @@ def feasibility_pump_pack(milps, *, seed: int = 1, time_limit: float = 1800.0,
             tight = min(params.eps_optimal_relative,
                         params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR
             if fp64[k] is None and tight:
-                st64 = _pdfp_mod.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb,
-                                           pp.var_ub, pp.w, precision="fp64", device=device)
-                if mixed[k]:
-                    _pdfp_mod.original_primal_residual(st64, pp.D_r)
+                st64 = _fp64_successor(_pdfp_mod, pp, mixed[k], device)
                 st64.set_x(pack.pdhg_x(k))
                 _pdfp_mod.set_target(st64, target[k])
                 fp64[k] = st64

@@ -107,13 +107,35 @@ def _target(m, kind):
     return np.zeros(len(m.c)) if kind == "zeros" else G.binary_targets(m, 77)
 
 
+def fp64_state(path):
+    """A PDFPState constructor that sets the kernel path before any solve."""
+    def make(*a, **kw):
+        s = P.PDFPState(*a, **kw)
+        s.set_kernel_path(path)
+        return s
+    return make
+
+
+def fp64_fits_one_workgroup(pp) -> bool:
+    # MI355X: 160 KiB of LDS per workgroup, less 8 KiB for static LDS
+    m_, n_ = pp.A.shape
+    return (m_ > 0 and n_ > 0 and pp.A.nnz > 0 and
+            4 * (n_ + 1) + 4 * (m_ + 1) + 24 * pp.A.nnz + 72 * n_ + 25 * m_ <= 152 * 1024)
+
+
+@pytest.mark.parametrize("path", ["auto", "one-workgroup"])
 @pytest.mark.parametrize("name,maker,kind", SMALL, ids=[c[0] for c in SMALL])
-def test_fp64_trajectory_matches_oracle(name, maker, kind):
-    say("fp64 trajectory", name)
+def test_fp64_trajectory_matches_oracle(name, maker, kind, path):
+    """The FP64 form on the reference's launch structure ("auto") and with
+    whole blocks and the power iteration in one workgroup ("one-workgroup",
+    on the instances that fit in LDS): the same bar."""
+    say("fp64 trajectory", name, path)
     m = maker()
     xt = _target(m, kind)
-    pp, s, st = project_with(P, P.PDFPState, m, xt, tol=1e-7, precision="fp64")
+    pp, s, st = project_with(P, fp64_state(path), m, xt, tol=1e-7, precision="fp64")
     pq, o, so = project_with(O, O.OraclePDFPState, m, xt, tol=1e-7)
+    assert s.info().small_path == (1 if path == "one-workgroup" and fp64_fits_one_workgroup(pp)
+                                   else 0)
     assert st == so == "OPTIMAL"
     assert s.total_count == o.total_count
     np.testing.assert_allclose(s.pdhg_x, o.pdhg_x, rtol=0, atol=1e-9)
@@ -127,12 +149,15 @@ def test_fp64_trajectory_matches_oracle(name, maker, kind):
         assert abs(a - b) <= 1e-6 * abs(b) + 1e-14, f
 
 
+@pytest.mark.parametrize("path", ["auto", "one-workgroup"])
 @pytest.mark.parametrize("case", l1_fixtures(), ids=lambda c: c["name"])
-def test_fp64_l1_matches_highs(case):
-    say("fp64 highs", case["name"])
+def test_fp64_l1_matches_highs(case, path):
+    say("fp64 highs", case["name"], path)
     milp = build_instance(case["spec"])
     xt = np.array(case["target"])
-    pp, s, status = project_with(P, P.PDFPState, milp, xt, tol=1e-8, precision="fp64")
+    pp, s, status = project_with(P, fp64_state(path), milp, xt, tol=1e-8, precision="fp64")
+    assert s.info().small_path == (1 if path == "one-workgroup" and fp64_fits_one_workgroup(pp)
+                                   else 0)
     assert status == "OPTIMAL"
     x = pp.D_c * s.pdhg_x
     ref = case["l1_optimum"]
