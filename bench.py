@@ -152,25 +152,40 @@ def kernel_bytes_batch(m: int, n: int, nnz: int, K: int = 8) -> tuple[float, flo
     return float(at), float(a)
 
 
-def slice_rows(A) -> int:
-    """Rows of the stacked column-slice matrix of the batch's sliced A pass:
-    the (row, slice) pairs of A with a nonzero, slices of n/8 columns."""
+def slice_rows(A, min_len: int = 0) -> tuple[int, int]:
+    """Rows of the stacked column-slice matrix of the batch's sliced A pass
+    (the (row, slice) pairs of A's long rows with a nonzero, slices of n/8
+    columns) and the number of long rows: rows with more than ``min_len``
+    nonzeros (0: every row)."""
     csr = A.tocsr()
     n = csr.shape[1]
-    rows = np.repeat(np.arange(csr.shape[0], dtype=np.int64), np.diff(csr.indptr))
+    lens = np.diff(csr.indptr)
+    long_rows = lens > min_len if min_len > 0 else np.ones(csr.shape[0], dtype=bool)
+    rows = np.repeat(np.arange(csr.shape[0], dtype=np.int64), lens)
+    keep = np.repeat(long_rows, lens)
     sl = (csr.indices.astype(np.int64) * 8) // max(n, 1)
-    return int(np.unique(rows * 8 + sl).size)
+    return int(np.unique(rows[keep] * 8 + sl[keep]).size), int(long_rows.sum())
 
 
-def kernel_bytes_sliced_a(m: int, n: int, nnz: int, T: int, K: int = 8) -> float:
+def kernel_bytes_sliced_a(m: int, n: int, nnz: int, T: int, K: int = 8, *, min_len: int = 0,
+                          nnz_long: int | None = None, n_long: int | None = None) -> float:
     """Algorithmic bytes of the batch's A pass over 8 column slices (the
-    layout's own figure): the slices' pass 8·nnz + 4(T + 9) (stacked row
-    pointers) + 4T (each stacked row's partial position) + x̄ read once r·n
-    + its partials written r·T; the combine 8·m·r (the 8 partials of every
-    row) + z, z⁰, b, is_eq (2r + 5)·m + write z r·m."""
+    layout's own figure).  Every row over the slices (min_len 0): the
+    slices' pass 8·nnz + 4(T + 9) (stacked row pointers) + 4T (each stacked
+    row's partial position) + x̄ read once r·n + its partials written r·T;
+    the combine 8·m·r (the 8 partials of every row) + z, z⁰, b, is_eq
+    (2r + 5)·m + write z r·m.  Only the rows longer than min_len over the
+    slices: the slices' pass over the long rows' nnz_long nonzeros as above
+    (x̄ still counted once), then the one pass over the short rows
+    8·(nnz - nnz_long) + 4(m+1) + 4m (each row's long-row index) + the long
+    rows' partials 8·n_long·r + the z update's (2r + 5)·m + r·m."""
     r = 4 * (8 if K <= 8 else 16)
-    return float(8 * nnz + 4 * (T + 9) + 4 * T + r * n + r * T + 8 * m * r + (2 * r + 5) * m
-                 + r * m)
+    if min_len == 0:
+        return float(8 * nnz + 4 * (T + 9) + 4 * T + r * n + r * T + 8 * m * r
+                     + (2 * r + 5) * m + r * m)
+    p1 = 8 * nnz_long + 4 * (T + 9) + 4 * T + r * n + r * T
+    p2 = 8 * (nnz - nnz_long) + 4 * (m + 1) + 4 * m + 8 * n_long * r + (2 * r + 5) * m + r * m
+    return float(p1 + p2)
 
 
 def run_batch(bt, pp, milp, target_sets, tol):
@@ -552,6 +567,10 @@ def main():
     ap.add_argument("--column-slices", choices=("on", "off", "256", "512"), default=None,
                     help="run the batch's A pass over A's 8 column slices (on) or in one "
                          "pass (off) instead of the autotuner's choice")
+    ap.add_argument("--slice-min-len", type=int, default=None,
+                    help="with column slices, run only the rows with more than this many "
+                         "nonzeros over the slices (0: every row) instead of the "
+                         "autotuner's choice")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
@@ -635,6 +654,8 @@ def main():
         if args.column_slices is not None:
             bt.set_column_slices({"on": 1, "off": 0, "256": 256,
                                   "512": 512}[args.column_slices])
+        if args.slice_min_len is not None:
+            bt.set_slice_min_len(args.slice_min_len)
         for k in range(K):
             bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]
@@ -656,7 +677,8 @@ def main():
                                      "At_persistent_wg_per_cu": bi.persist_at,
                                      "A_lane_rows": bi.lane_rows_a,
                                      "At_lane_rows": bi.lane_rows_at,
-                                     "A_column_slices": bi.column_slices})
+                                     "A_column_slices": bi.column_slices,
+                                     "A_slice_min_len": bi.slice_min_len})
         print(f"[bench] fp32 batch of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; batched iteration "
               f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)
@@ -667,7 +689,11 @@ def main():
             # the A pass runs over column slices: its own layout's bytes (the
             # one-pass figure stays beside them in the line)
             head["row_pass_bytes_one_pass"] = ba
-            ba = kernel_bytes_sliced_a(mm, nn, nnz, slice_rows(pp.A), K)
+            T, n_long = slice_rows(pp.A, bi.slice_min_len)
+            lens = np.diff(pp.A.tocsr().indptr)
+            ba = kernel_bytes_sliced_a(mm, nn, nnz, T, K, min_len=bi.slice_min_len,
+                                       nnz_long=int(lens[lens > bi.slice_min_len].sum()),
+                                       n_long=n_long)
         steps_desc = (f"{K} warm-started l1 projections to new integer targets at relative KKT "
                       f"tolerance {args.tol:g}, served by the {K} trajectory slots of one batch "
                       f"sharing A (a slot that finishes takes the next target)")
@@ -689,8 +715,11 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(args.workload, dom_key)
     if traffic is not None and dom_key.startswith("EpiSlicePartial"):
-        # the sliced A pass is two launches: the slices' pass and the combine
-        combine = measured_traffic(args.workload, f"k_batch_slice_dualz<{width}, false>")
+        # the sliced A pass is two launches: the slices' pass, then the combine
+        # (every row over the slices) or the short rows' pass
+        second = (f"k_batch_slice_dualz<{width}, false>" if not head["npb"]["A_slice_min_len"]
+                  else f"EpiBatchDualZLong<{width}, false>")
+        combine = measured_traffic(args.workload, second)
         traffic = None if combine is None else traffic + combine
     # The random-gather floor of each pass: nnz gathers of one record per
     # column (Aᵀz gathers z: m records; A x̄ gathers x̄: n records).

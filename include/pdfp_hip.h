@@ -118,6 +118,9 @@ typedef struct pdfp_info {
   int32_t column_slices;     /* batch: 0, or the nonzeros per workgroup (256,   */
                              /*   512) of the A pass over 8 column slices of A, */
                              /*   one per XCD                                   */
+  int32_t slice_min_len;     /* batch, with column slices: the rows longer than */
+                             /*   this run over the slices, the others in one   */
+                             /*   pass; 0: every row                            */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */
@@ -281,6 +284,16 @@ int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu);
  * are rounded to FP32, so the iterates differ from the one-pass form's in
  * the last bits. */
 int pdfp_batch_set_column_slices(pdfp_batch* ctx, int32_t on);
+/* The rows of A that run over the column slices (no counterpart in the
+ * reference): those with more than min_len nonzeros; the others run in one
+ * pass over A with the long rows emptied, whose epilogue adds each long
+ * row's 8 partials.  0 runs every row over the slices.  On power-law rows a
+ * few long rows hold most nonzeros: their gathers come from the XCDs' L2s,
+ * while the many short rows need no partials.  Takes effect when column
+ * slices are on; pdfp_batch_create's autotuner picks the bound with the
+ * slices (0, or 32, 64, 128 or 256 where the longer rows hold 20-95% of the
+ * nonzeros).  PDFP_ERR_ARGUMENT for min_len < 0. */
+int pdfp_batch_set_slice_min_len(pdfp_batch* ctx, int32_t min_len);
 /* Kernel path of the batch (no counterpart in the reference), as
  * pdfp_set_kernel_path: 0 (the default) runs each block as one launch with
  * one workgroup per running trajectory when the instance fits in LDS (each

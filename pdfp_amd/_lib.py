@@ -68,6 +68,7 @@ class PdfpInfo(C.Structure):
         ("lane_rows_at", C.c_int32),
         ("block_graph", C.c_int32),
         ("column_slices", C.c_int32),
+        ("slice_min_len", C.c_int32),
     ]
 
 
@@ -151,6 +152,7 @@ SIGNATURES = {
     "pdfp_batch_set_block_graph": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_set_cooperative": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_set_column_slices": (C.c_int, [_P, C.c_int32]),
+    "pdfp_batch_set_slice_min_len": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_set_kernel_path": (C.c_int, [_P, C.c_int32]),
     "pdfp_pack_last_error": (C.c_char_p, [_P]),
     "pdfp_pack_create": (C.c_int, [C.POINTER(_P), C.c_int32, _I64, _I64, C.POINTER(_I64),

@@ -164,6 +164,13 @@ class PDFPBatch:
         v = on if isinstance(on, int) and not isinstance(on, bool) else int(bool(on))
         self._check(self._lib.pdfp_batch_set_column_slices(self._ctx, v))
 
+    def set_slice_min_len(self, min_len: int) -> None:
+        """With column slices on, run only the rows of A with more than
+        `min_len` nonzeros over the slices and the others in one pass (0:
+        every row over the slices).  The autotuner picks it with the
+        slices."""
+        self._check(self._lib.pdfp_batch_set_slice_min_len(self._ctx, int(min_len)))
+
     def set_schedule(self, npb_a: int, npb_at: int, persist: int = 0, lane_rows_a: int = 0,
                      lane_rows_at: int = 0) -> None:
         """Replace the autotuned schedule of the batched passes (tests and

@@ -94,6 +94,7 @@ struct BatchBase {
   virtual ~BatchBase() = default;
   virtual void set_coop(int per_cu) = 0;
   virtual void set_slices(int on) = 0;
+  virtual void set_slice_min_len(int len) = 0;
   virtual void set_original_primal_residual(const double* D_r) = 0;
   virtual void set_x(int k, const double* xs) = 0;
   virtual void set_target(int k, const double* xts) = 0;
@@ -374,10 +375,8 @@ struct Batch final : BatchBase {
   void launch_batch(const DevCsr<float>& M, G g, E e, hipEvent_t t0 = nullptr,
                     hipEvent_t t1 = nullptr) {
     if (M.nblocks == 0) {
-      if (t0) {
-        PDFP_HIP(hipEventRecord(t0, stream));
-        PDFP_HIP(hipEventRecord(t1, stream));
-      }
+      for (hipEvent_t t : {t0, t1})
+        if (t) PDFP_HIP(hipEventRecord(t, stream));
       return;
     }
     // the lane epilogues serve a block's (row, trajectory) pairs in one pass
@@ -465,25 +464,50 @@ struct Batch final : BatchBase {
       }
       M->schedule(M->npb, stream, kRows, bl);
     }
-    // the A pass over column slices, where it can pay, at each block size;
-    // kept only if an iteration is at least 2% faster with it (the best of
-    // three timings of each form, taken in turn)
+    // the A pass over column slices, where it can pay: every row over the
+    // slices, or only the rows longer than a bound whose rows hold 20-95% of
+    // the nonzeros (power-law rows), each at both block sizes of the slices'
+    // pass; kept only if an iteration is at least 2% faster than in one pass
+    // (the best of three timings of each form, taken in turn)
     if (slices_pay(m, n, nnz)) {
-      build_slices();
-      float t_one = 1e30f, t_sl[2] = {1e30f, 1e30f};
-      const int npbs[2] = {512, 256};
-      for (int rep = 0; rep < 3; ++rep) {
-        slices_on = false;
-        t_one = std::min(t_one, time8());
-        slices_on = true;
-        for (int v = 0; v < 2; ++v) {
-          As.schedule(npbs[v], stream, kBlock);
-          t_sl[v] = std::min(t_sl[v], time8());
-        }
+      std::vector<int32_t> bounds = {0};
+      for (int32_t len : {32, 64, 128, 256}) {
+        const double f = (double)nnz_in_rows_over(len) / (double)nnz;
+        if (f >= 0.2 && f <= 0.95) bounds.push_back(len);
       }
-      const int v = t_sl[0] <= t_sl[1] ? 0 : 1;
-      if (t_sl[v] < 0.98f * t_one) As.schedule(npbs[v], stream, kBlock);
-      else drop_slices();
+      const int npbs[2] = {512, 256};
+      float t_one = 1e30f, t_best = 1e30f;
+      int32_t best_len = 0;
+      int best_npb = 512;
+      for (int32_t len : bounds) {
+        slice_min_len = len;
+        build_slices();
+        float t_sl[2] = {1e30f, 1e30f};
+        for (int rep = 0; rep < 3; ++rep) {
+          slices_on = false;
+          t_one = std::min(t_one, time8());
+          slices_on = true;
+          for (int v = 0; v < 2; ++v) {
+            slice_schedule(npbs[v]);
+            t_sl[v] = std::min(t_sl[v], time8());
+          }
+        }
+        for (int v = 0; v < 2; ++v)
+          if (t_sl[v] < t_best) {
+            t_best = t_sl[v];
+            best_len = len;
+            best_npb = npbs[v];
+          }
+      }
+      if (t_best < 0.98f * t_one) {
+        slice_min_len = best_len;
+        build_slices();
+        slice_schedule(best_npb);
+        slices_on = true;
+      } else {
+        slice_min_len = 0;
+        drop_slices();
+      }
     }
     upload_steps(1.0, none);
     for (DevBuf<float>* d : {&x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry, &z, &z0, &pz, &rz})
@@ -503,6 +527,7 @@ struct Batch final : BatchBase {
     A.schedule(npb_a, stream, kRows, lane_a);
     At.schedule(npb_at, stream, kRows, lane_at);
     A.persist = At.persist = persist;
+    short_schedule();
     const size_t need = (size_t)std::max<int32_t>(A.nblocks, 1) * kBatch * kEvalRowTerms;
     if (part_a.n < need) part_a.alloc(need);
   }
@@ -802,15 +827,23 @@ struct Batch final : BatchBase {
   }
 
   // ---- the A pass over 8 column slices (slices_on) ----
-  // As: A's 8 column slices stacked (segment s: the rows with a nonzero in
-  // the columns of slice s, in row order), each walked by one XCD; part_s:
-  // the FP32 partial row sums, KB per (slice, row of A), at the position
-  // slice_dense gives each stacked row; k_batch_slice_dualz adds a row's 8
-  // partials and updates z.
-  DevCsr<float> As;
+  // The long rows of A (more than slice_min_len nonzeros; 0: every row) run
+  // over A's 8 column slices.  As: their slices stacked (segment s: the long
+  // rows with a nonzero in the columns of slice s, in row order), each
+  // segment walked by one XCD; part_s: the FP32 partial row sums, KB per
+  // (slice, long row), at the position slice_dense gives each stacked row
+  // (s·n_long + l for long row l).  With every row long, k_batch_slice_dualz
+  // adds a row's 8 partials and updates z; otherwise the short rows run in
+  // one pass over Ash (A with the long rows emptied) whose epilogue adds the
+  // long rows' partials (EpiBatchDualZLong; long_idx: each row's index among
+  // the long rows, -1 for a short row).
+  DevCsr<float> As, Ash;
   DevBuf<float> part_s;
-  DevBuf<int32_t> slice_dense;
+  DevBuf<int32_t> slice_dense, long_idx;
   bool slices_on = false;
+  int32_t slice_min_len = 0;
+  int32_t built_min_len = -1;  // the bound As was built for; -1: not built
+  int64_t n_long = 0;
 
   // Column slices pay when x̄ (n·KB FP32 values) is larger than an XCD's
   // 4 MB L2 and an eighth of it is not.
@@ -820,62 +853,137 @@ struct Batch final : BatchBase {
            xbar / 8 <= (int64_t)5 << 20;
   }
 
-  // Builds As from A's CSR (read back from the device once).
+  // Nonzeros of A in rows with more than len nonzeros.
+  int64_t nnz_in_rows_over(int32_t len) const {
+    const std::vector<int32_t>& rp = A.h_rowptr;
+    int64_t c = 0;
+    for (int64_t i = 0; i < m; ++i) {
+      const int32_t l = rp[(size_t)i + 1] - rp[(size_t)i];
+      if (l > len) c += l;
+    }
+    return c;
+  }
+
+  // Builds As (and, for a bound above 0, Ash and long_idx) for slice_min_len
+  // from A's CSR (read back from the device once per build).
   void build_slices() {
-    if (As.nrows > 0 || nnz == 0) return;
+    if (built_min_len == slice_min_len || nnz == 0) return;
+    release_slices();
     std::vector<int32_t> col((size_t)nnz);
     std::vector<float> val((size_t)nnz);
     A.col.download(col.data(), (size_t)nnz, stream);
     A.val.download(val.data(), (size_t)nnz, stream);
     sync();
     const std::vector<int32_t>& rp = A.h_rowptr;
+    std::vector<int32_t> li((size_t)m, -1);
+    n_long = 0;
+    for (int64_t i = 0; i < m; ++i)
+      if (slice_min_len == 0 || rp[(size_t)i + 1] - rp[(size_t)i] > slice_min_len)
+        li[(size_t)i] = (int32_t)n_long++;
     HostCsr h;
     h.ncols = n;
     h.rowptr.assign(1, 0);
-    h.col.reserve((size_t)nnz);
-    h.val.reserve((size_t)nnz);
     std::vector<int32_t> cur(rp.begin(), rp.end() - 1);  // next entry of each row
-    std::vector<int32_t> srow;                            // A's row of each stacked row
+    std::vector<int32_t> srow;                            // long-row index of each stacked row
+    struct Piece {
+      int32_t l, k0, k1;  // long-row index, its nonzeros [k0, k1) in the slice
+    };
+    std::vector<Piece> pieces;
     As.segments.assign(1, 0);
     for (int s = 0; s < 8; ++s) {
       const int64_t hi = n * (s + 1) / 8;  // columns [n·s/8, hi) form slice s
+      pieces.clear();
       for (int64_t i = 0; i < m; ++i) {
+        if (li[(size_t)i] < 0) continue;
         int32_t k = cur[(size_t)i];
         const int32_t end = rp[(size_t)i + 1];
-        if (k == end || col[(size_t)k] >= hi) continue;  // no nonzero in this slice
-        for (; k < end && col[(size_t)k] < hi; ++k) {     // a row's columns ascend
+        const int32_t k0 = k;
+        while (k < end && col[(size_t)k] < hi) ++k;  // a row's columns ascend
+        cur[(size_t)i] = k;
+        if (k > k0) pieces.push_back({li[(size_t)i], k0, k});
+      }
+      // longest first: a block's rows are alike, and the long ones start early
+      std::stable_sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) {
+        return a.k1 - a.k0 > b.k1 - b.k0;
+      });
+      for (const Piece& pc : pieces) {
+        for (int32_t k = pc.k0; k < pc.k1; ++k) {
           h.col.push_back(col[(size_t)k]);
           h.val.push_back((double)val[(size_t)k]);
         }
-        cur[(size_t)i] = k;
         h.rowptr.push_back((int32_t)h.col.size());
-        srow.push_back((int32_t)i);
+        srow.push_back(pc.l);
       }
       As.segments.push_back((int64_t)srow.size());
     }
-    if ((int64_t)h.col.size() != nnz) throw Error(PDFP_ERR_STATE, "column slices lost nonzeros");
     h.nrows = (int64_t)srow.size();
-    // each stacked row's (slice, row) position in the dense partials
+    // each stacked row's (slice, long row) position in the dense partials
     std::vector<int32_t> dense(srow.size());
     for (int s = 0; s < 8; ++s)
       for (int64_t r = As.segments[(size_t)s]; r < As.segments[(size_t)s + 1]; ++r)
-        dense[(size_t)r] = (int32_t)(s * m + srow[(size_t)r]);
+        dense[(size_t)r] = (int32_t)(s * n_long + srow[(size_t)r]);
+    if (slice_min_len > 0) {
+      HostCsr hs;  // A with the long rows emptied
+      hs.nrows = m;
+      hs.ncols = n;
+      hs.rowptr.assign(1, 0);
+      for (int64_t i = 0; i < m; ++i) {
+        if (li[(size_t)i] < 0)
+          for (int32_t k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) {
+            hs.col.push_back(col[(size_t)k]);
+            hs.val.push_back((double)val[(size_t)k]);
+          }
+        hs.rowptr.push_back((int32_t)hs.col.size());
+      }
+      if ((int64_t)(h.col.size() + hs.col.size()) != nnz)
+        throw Error(PDFP_ERR_STATE, "column slices lost nonzeros");
+      Ash.upload(hs, stream);
+      long_idx.alloc((size_t)std::max<int64_t>(m, 1));
+      long_idx.upload(li.data(), (size_t)m, stream);
+      short_schedule();
+    } else if ((int64_t)h.col.size() != nnz) {
+      throw Error(PDFP_ERR_STATE, "column slices lost nonzeros");
+    }
     slice_dense.alloc(std::max<size_t>(dense.size(), 1));
     slice_dense.upload(dense.data(), dense.size(), stream);
     As.upload(h, stream);
-    As.schedule(512, stream, kBlock);
-    part_s.alloc((size_t)(8 * m * KB));
+    slice_schedule(512);
+    part_s.alloc((size_t)std::max<int64_t>(8 * n_long * KB, 1));
     PDFP_HIP(hipMemsetAsync(part_s.p, 0, part_s.bytes(), stream));  // the empty pairs stay 0
     sync();
+    built_min_len = slice_min_len;
+  }
+
+  // The slices' pass: npb nonzeros and at most 32 rows per block, rows of
+  // up to 128 nonzeros reduced one lane per (row, trajectory).
+  void slice_schedule(int npb) { As.schedule(npb, stream, kBlock / kBatch, 128); }
+
+  // The short rows' pass follows A's schedule (block size, lane rows, grid).
+  void short_schedule() {
+    if (slice_min_len == 0 || Ash.nrows == 0) return;
+    Ash.schedule(A.npb, stream, kBlock / kBatch, A.lane_len);
+    Ash.persist = A.persist;
+  }
+
+  void release_slices() {
+    for (DevCsr<float>* M : {&As, &Ash}) {
+      M->rowptr.release();
+      M->col.release();
+      M->val.release();
+      M->sched.release();
+      M->nrows = M->nblocks = 0;
+      M->segments.clear();
+      M->seg_blocks.clear();
+    }
+    part_s.release();
+    slice_dense.release();
+    long_idx.release();
+    n_long = 0;
+    built_min_len = -1;
   }
 
   void drop_slices() {
-    for (auto* d : {&As.rowptr, &As.col}) d->release();
-    As.val.release();
-    As.sched.release();
-    As.nrows = As.nblocks = 0;
-    part_s.release();
-    slice_dense.release();
+    release_slices();
     slices_on = false;
   }
 
@@ -886,8 +994,20 @@ struct Batch final : BatchBase {
     sync();
     drop_graphs();
     if (on) build_slices();
-    if (on > 1) As.schedule(on, stream, kBlock);
+    if (on > 1) slice_schedule(on);
     slices_on = on != 0;
+  }
+
+  void set_slice_min_len(int len) override {
+    if (len < 0) throw Error(PDFP_ERR_ARGUMENT, "the long-row bound must be at least 0");
+    sync();
+    drop_graphs();
+    const int npb = As.nrows > 0 ? As.npb : 512;
+    slice_min_len = len;
+    if (slices_on) {
+      build_slices();
+      slice_schedule(npb);
+    }
   }
 
   // A x̄ and the z update of iteration k_offset over the column slices; t0 /
@@ -914,6 +1034,11 @@ struct Batch final : BatchBase {
     }
     PDFP_HIP(hipGetLastError());
     EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    if (slice_min_len > 0) {
+      const EpiBatchDualZLong<KB, MAJOR> el{ez, long_idx.p, part_s.p, (int32_t)n_long};
+      launch_batch(Ash, GatherBatch<KB>{rx.p}, el, nullptr, t1);
+      return;
+    }
     const int g2 = grid_for(m * KB);
     if (t1)
       hipExtLaunchKernelGGL(k_batch_slice_dualz<KB, MAJOR>, dim3(g2), dim3(256), 0, stream,
@@ -1147,6 +1272,7 @@ struct Batch final : BatchBase {
     o->block_graph = graph_on && !small_used ? 1 : 0;
     o->small_path = small_used ? 1 : 0;
     o->column_slices = slices_on && !small_used ? As.npb : 0;
+    o->slice_min_len = slices_on && !small_used ? slice_min_len : 0;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched
@@ -1417,6 +1543,10 @@ int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on) {
 
 int pdfp_batch_set_column_slices(pdfp_batch* ctx, int32_t on) {
   return batch_guarded(ctx, [&] { ctx->impl->set_slices(on); });
+}
+
+int pdfp_batch_set_slice_min_len(pdfp_batch* ctx, int32_t min_len) {
+  return batch_guarded(ctx, [&] { ctx->impl->set_slice_min_len(min_len); });
 }
 
 int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu) {

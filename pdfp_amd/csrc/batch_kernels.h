@@ -181,29 +181,30 @@ struct EpiBatchDualZ {
 //
 // A x̄ as 8 partial products: the matrix stacked from A's 8 column slices
 // (slice s: A's rows restricted to columns [s·n/8, (s+1)·n/8), only the rows
-// with a nonzero there, slice after slice), each slice walked by one XCD
+// with a nonzero there), each slice walked by one XCD
 // (csr_pass_xcd_segments), so that its gathers of x̄ come from the eighth of
 // x̄ that fits in that XCD's L2 instead of filling a line from memory per
-// nonzero.  Stacked row r (row i of A in slice s) stores its KB sums (FP64
+// nonzero.  Within a slice the stacked rows are ordered by length, so that a
+// block's rows are alike and its lanes (one per row and trajectory) finish
+// together.  Stacked row r (row i of A in slice s) stores its KB sums (FP64
 // over the slice's nonzeros in ascending column order), rounded to FP32, at
-// part[(s·m + i)·KB ...], dense[r] = s·m + i; the (slice, row) pairs with no
-// nonzero keep the zeros they were allocated with.  k_batch_slice_dualz adds
-// row i's 8 partials in slice order in FP64 and runs the z update.
+// part[dense[r]·KB ...]; the (slice, row) pairs with no nonzero keep the
+// zeros they were allocated with.  k_batch_slice_dualz (or, when only the
+// long rows are sliced, EpiBatchDualZLong) adds row i's 8 partials in slice
+// order in FP64 and runs the z update.
 
+// Lane (r, q) stores stacked row r's sum for trajectory q.
 template <int KB>
 struct EpiSlicePartial {
+  static constexpr int kLanes = KB;
   struct Pre {
     int32_t d;
   };
   const int32_t* __restrict__ dense;
   float* __restrict__ part;
-  __device__ __forceinline__ Pre pre(int r) const { return {dense[r]}; }
-  __device__ __forceinline__ void operator()(int, const Pre& p, const double* s, double*) const {
-    float4* o = reinterpret_cast<float4*>(part + (size_t)p.d * KB);
-#pragma unroll
-    for (int h = 0; h < KB / 4; ++h)
-      o[h] = make_float4((float)s[4 * h], (float)s[4 * h + 1], (float)s[4 * h + 2],
-                         (float)s[4 * h + 3]);
+  __device__ __forceinline__ Pre pre(int r, int) const { return {dense[r]}; }
+  __device__ __forceinline__ void operator()(int, int q, const Pre& p, double s, double*) const {
+    part[(size_t)p.d * KB + q] = (float)s;
   }
 };
 
@@ -222,6 +223,42 @@ __global__ __launch_bounds__(256) void k_batch_slice_dualz(int m, const float* _
   for (int sl = 0; sl < 8; ++sl) sum += (double)v[sl];
   ez(i, q, p, sum, nullptr);
 }
+
+// The A pass when only A's long rows run over the column slices: the short
+// rows are summed in one pass over A with the long rows emptied, and this
+// epilogue adds a long row's 8 slice partials (part[(s·nlong + l)·KB + q],
+// l = long_idx[i] ≥ 0; -1 marks a short row) in slice order in FP64 to the
+// pass's sum, which is +0.0 for an emptied row, so the sum is formed as
+// k_batch_slice_dualz forms it; then the z update.  On power-law rows most
+// nonzeros sit in a few long rows: their gathers come from the XCDs' L2s,
+// while the many short rows keep one pass and no partials.
+template <int KB, bool MAJOR>
+struct EpiBatchDualZLong {
+  static constexpr int kBatch = KB;
+  static constexpr int kLanes = KB;
+  using Z = EpiBatchDualZ<KB, MAJOR>;
+  struct Pre {
+    typename Z::Pre z;
+    int32_t l;
+  };
+  Z ez;
+  const int32_t* __restrict__ long_idx;
+  const float* __restrict__ part;
+  int32_t nlong;
+  __device__ __forceinline__ Pre pre(int i, int q) const { return {ez.pre(i, q), long_idx[i]}; }
+  __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
+                                             double* acc) const {
+    if (p.l >= 0) {
+      float v[8];
+#pragma unroll
+      for (int sl = 0; sl < 8; ++sl)
+        v[sl] = part[((size_t)sl * (size_t)nlong + (size_t)p.l) * KB + q];
+#pragma unroll
+      for (int sl = 0; sl < 8; ++sl) s += (double)v[sl];
+    }
+    ez(i, q, p.z, s, acc);
+  }
+};
 
 // ---- the evaluation terms of trajectory k (views of the interleaved vectors) --
 

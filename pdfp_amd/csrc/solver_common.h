@@ -200,13 +200,14 @@ struct DevCsr {
 // operator's `persist` > 0) a persistent grid of persist × 256 workgroups
 // (256 CUs) that walks the blocks.  With events t0/t1 the launch records the
 // kernel's own start and end (hipExtLaunchKernel), which times the kernel
-// alone, without the gaps that events recorded around it on the stream add.
+// alone, without the gaps that events recorded around it on the stream add;
+// either may be null (a timed span of several launches).
 template <int NPB, int NACC, int NMAX, typename TV, class G, class E>
 inline void launch_pass_npb(const DevCsr<TV>& A, G g, E e, double* partials, hipStream_t s,
                             hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
   const int grid = A.persist * 256;
   const bool persist = A.persist > 0 && grid < A.nblocks;
-  if (t0) {
+  if (t0 || t1) {
     if (persist)
       hipExtLaunchKernelGGL(csr_pass_persistent<kBlock, NPB, TV, G, E, NACC, NMAX>, dim3(grid),
                             dim3(kBlock), 0, s, t0, t1, 0, A.view(), g, e, partials);

@@ -244,14 +244,16 @@ def test_batch_schedules_agree(K):
             assert abs(l1b - ref[k]) <= 1e-4 * max(1.0, ref[k]), (sched, k, l1b, ref[k])
 
 
+@pytest.mark.parametrize("min_len", [0, 64])
 @pytest.mark.parametrize("K", [8, 16, 5])
-def test_batch_column_slices_agree(K):
+def test_batch_column_slices_agree(K, min_len):
     """The A pass over A's 8 column slices (one per XCD, FP32 partial sums
     added in FP64) gives the same projections as the one-pass form: oracle
     KKT within 10×tol and the single FP32 projector's ℓ1 distance to 1e-4,
     on power-law rows with a dense column; the replayed block graphs equal
-    the stream launches bit for bit."""
-    print("batch column slices", K, flush=True)
+    the stream launches bit for bit.  min_len 64: only the rows longer than
+    64 nonzeros run over the slices, the others in one pass."""
+    print("batch column slices", K, min_len, flush=True)
     m = _with_dense_column(G.mixed(2000, 5000, 5, power_law=True, nnz=60_000, max_len=3000))
     pp = projector_problem(m)
     tol = 1e-5
@@ -271,11 +273,13 @@ def test_batch_column_slices_agree(K):
         bt.set_kernel_path("general")
         bt.set_schedule(512, 256, 0, 32, 32)
         bt.set_column_slices(True)
+        bt.set_slice_min_len(min_len)
         bt.set_block_graph(graph)
         for k in range(K):
             bt.set_target(k, targets[k])
         assert bt.solve(p, 1.0) == ["OPTIMAL"] * K
         assert bt.info(0).column_slices == 512
+        assert bt.info(0).slice_min_len == min_len
         run = []
         for k in range(K):
             x, z = bt.pdhg_x(k), bt.vector(k, "pdhg_z")
