@@ -708,7 +708,8 @@ def main():
     keys = ((f"EpiBatchPrimalY<{width}, false>", f"EpiBatchDualZ<{width}, false>") if K > 1 else
             ("EpiPrimalY<float, false>", "EpiDualZ<float, false>"))
     if K > 1 and head["npb"].get("A_column_slices"):
-        keys = (keys[0], f"EpiSlicePartial<{width}>")
+        keys = (keys[0], f"EpiSlicePartialLanes<{width}>" if head["npb"]["A_slice_min_len"]
+                else f"EpiSlicePartial<{width}>")
     dom_bytes, dom_ms, dom_name, dom_key = max(
         (bat, head["ms_at"], "transpose pass (A^T z + x/y update)", keys[0]),
         (ba, head["ms_a"], "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])

@@ -902,10 +902,14 @@ struct Batch final : BatchBase {
         cur[(size_t)i] = k;
         if (k > k0) pieces.push_back({li[(size_t)i], k0, k});
       }
-      // longest first: a block's rows are alike, and the long ones start early
-      std::stable_sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) {
-        return a.k1 - a.k0 > b.k1 - b.k0;
-      });
+      // only the long rows sliced: longest first, so a block's rows are alike
+      // (its lanes, one per row and trajectory, finish together) and the long
+      // ones start early; every row sliced: row order, so a block's partial
+      // sums go to neighbouring rows
+      if (slice_min_len > 0)
+        std::stable_sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) {
+          return a.k1 - a.k0 > b.k1 - b.k0;
+        });
       for (const Piece& pc : pieces) {
         for (int32_t k = pc.k0; k < pc.k1; ++k) {
           h.col.push_back(col[(size_t)k]);
@@ -954,9 +958,14 @@ struct Batch final : BatchBase {
     built_min_len = slice_min_len;
   }
 
-  // The slices' pass: npb nonzeros and at most 32 rows per block, rows of
-  // up to 128 nonzeros reduced one lane per (row, trajectory).
-  void slice_schedule(int npb) { As.schedule(npb, stream, kBlock / kBatch, 128); }
+  // The slices' pass: npb nonzeros per block; with every row sliced, up to
+  // 256 rows per block reduced by lane groups (EpiSlicePartial); with only
+  // the long rows, up to 256/KB rows per block, those of up to 128 nonzeros
+  // reduced one lane per (row, trajectory) (EpiSlicePartialLanes).
+  void slice_schedule(int npb) {
+    if (slice_min_len == 0) As.schedule(npb, stream, kBlock);
+    else As.schedule(npb, stream, kBlock / kBatch, 128);
+  }
 
   // The short rows' pass follows A's schedule (block size, lane rows, grid).
   void short_schedule() {
@@ -1019,18 +1028,25 @@ struct Batch final : BatchBase {
     for (int s = 0; s <= 8; ++s) sb.b[s] = As.seg_blocks[(size_t)s];
     for (int s = 0; s < 8; ++s) maxb = std::max(maxb, sb.b[s + 1] - sb.b[s]);
     const GatherBatch<KB> g{rx.p};
-    const EpiSlicePartial<KB> ep{slice_dense.p, part_s.p};
     const dim3 grid((unsigned)(8 * std::max(maxb, 1))), block(kBlock);
-    if (As.npb == 512) {
-      auto kern = csr_pass_xcd_segments<kBlock, 512, float, GatherBatch<KB>, EpiSlicePartial<KB>, 0, 0>;
+    auto launch = [&](auto kern, auto ep) {
       if (t0) hipExtLaunchKernelGGL(kern, grid, block, 0, stream, t0, nullptr, 0, As.view(), g, ep,
                                     (double*)nullptr, sb);
       else kern<<<grid, block, 0, stream>>>(As.view(), g, ep, nullptr, sb);
+    };
+    if (slice_min_len == 0) {
+      if (As.max_rows > kBlock) throw Error(PDFP_ERR_STATE, "slice schedule holds too many rows");
+      using E = EpiSlicePartial<KB>;
+      const E ep{slice_dense.p, part_s.p};
+      if (As.npb == 512) launch(csr_pass_xcd_segments<kBlock, 512, float, GatherBatch<KB>, E, 0, 0>, ep);
+      else launch(csr_pass_xcd_segments<kBlock, 256, float, GatherBatch<KB>, E, 0, 0>, ep);
     } else {
-      auto kern = csr_pass_xcd_segments<kBlock, 256, float, GatherBatch<KB>, EpiSlicePartial<KB>, 0, 0>;
-      if (t0) hipExtLaunchKernelGGL(kern, grid, block, 0, stream, t0, nullptr, 0, As.view(), g, ep,
-                                    (double*)nullptr, sb);
-      else kern<<<grid, block, 0, stream>>>(As.view(), g, ep, nullptr, sb);
+      if (As.max_rows > kBlock / kBatch)
+        throw Error(PDFP_ERR_STATE, "slice schedule holds too many rows per block");
+      using E = EpiSlicePartialLanes<KB>;
+      const E ep{slice_dense.p, part_s.p};
+      if (As.npb == 512) launch(csr_pass_xcd_segments<kBlock, 512, float, GatherBatch<KB>, E, 0, 0>, ep);
+      else launch(csr_pass_xcd_segments<kBlock, 256, float, GatherBatch<KB>, E, 0, 0>, ep);
     }
     PDFP_HIP(hipGetLastError());
     EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};

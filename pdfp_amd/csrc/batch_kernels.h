@@ -184,18 +184,40 @@ struct EpiBatchDualZ {
 // with a nonzero there), each slice walked by one XCD
 // (csr_pass_xcd_segments), so that its gathers of x̄ come from the eighth of
 // x̄ that fits in that XCD's L2 instead of filling a line from memory per
-// nonzero.  Within a slice the stacked rows are ordered by length, so that a
-// block's rows are alike and its lanes (one per row and trajectory) finish
-// together.  Stacked row r (row i of A in slice s) stores its KB sums (FP64
+// nonzero.  When only the long rows are sliced, the stacked rows of a slice
+// are ordered by length, so that a block's rows are alike and its lanes (one
+// per row and trajectory) finish together; otherwise they keep row order.
+// Stacked row r (row i of A in slice s) stores its KB sums (FP64
 // over the slice's nonzeros in ascending column order), rounded to FP32, at
 // part[dense[r]·KB ...]; the (slice, row) pairs with no nonzero keep the
 // zeros they were allocated with.  k_batch_slice_dualz (or, when only the
 // long rows are sliced, EpiBatchDualZLong) adds row i's 8 partials in slice
 // order in FP64 and runs the z update.
 
-// Lane (r, q) stores stacked row r's sum for trajectory q.
+// Stacked row r's KB sums, by the row's lane (blocks of up to 256 rows:
+// every row sliced, so the slices' rows are short and many).
 template <int KB>
 struct EpiSlicePartial {
+  struct Pre {
+    int32_t d;
+  };
+  const int32_t* __restrict__ dense;
+  float* __restrict__ part;
+  __device__ __forceinline__ Pre pre(int r) const { return {dense[r]}; }
+  __device__ __forceinline__ void operator()(int, const Pre& p, const double* s, double*) const {
+    float4* o = reinterpret_cast<float4*>(part + (size_t)p.d * KB);
+#pragma unroll
+    for (int h = 0; h < KB / 4; ++h)
+      o[h] = make_float4((float)s[4 * h], (float)s[4 * h + 1], (float)s[4 * h + 2],
+                         (float)s[4 * h + 3]);
+  }
+};
+
+// The same stores by lane (r, q), one per trajectory (blocks of up to
+// 256/KB rows, each reduced one lane per (row, trajectory): only the long
+// rows sliced, whose slices' rows are longer and fewer).
+template <int KB>
+struct EpiSlicePartialLanes {
   static constexpr int kLanes = KB;
   struct Pre {
     int32_t d;
