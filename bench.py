@@ -34,6 +34,11 @@ counter summary of the same workload, when there is one), the CPU baseline
 same workload) and the time to the first feasible point of four MILPs and of the
 four example instances' pumps on one pack.
 
+    python bench.py --pump --workload general-int-4m [--pump-time-limit 120]
+
+runs the pump to its first feasible point on a workload's MILP (FP32 with the
+FP64 polish, then 8 trajectories on the batch) and prints its times.
+
     python bench.py --copy-calibration
 
 copies 1 GiB device to device ten times (the plain copy kernel whose
@@ -276,6 +281,29 @@ def _oracle_pump_backends():
                                      cupdlpx_termination=cupdlpx_termination)
 
     return proj, root_lp
+
+
+def pump_on_workload(name: str, time_limit: float) -> None:
+    """The pump on a benchmark workload's MILP (one run of each form, at
+    most `time_limit` seconds): prints one JSON line with PumpResult's time,
+    pump and PDHG iterations and the root-LP, projection and rounding shares
+    (src/pump.jl:412-430)."""
+    from pdfp_amd.milp import is_feasible
+    from pdfp_amd.pump import feasibility_pump, feasibility_pump_batch
+
+    milp, desc = workload(name)
+    out = {"workload": desc, "time_limit_s": time_limit}
+    for form, run in (("fp32", lambda: feasibility_pump(milp, time_limit=time_limit,
+                                                        precision="fp32")),
+                      ("fp32_batch8", lambda: feasibility_pump_batch(milp, trajectories=8,
+                                                                     time_limit=time_limit))):
+        print(f"[bench] pump {form} on {name}", file=sys.stderr, flush=True)
+        r = run()
+        out[form] = {"seconds": r.time, "found": bool(r.found and is_feasible(milp, r.x)),
+                     "iterations": r.iterations, "pdhg_iterations": r.pdhg_iterations,
+                     "root_lp_s": r.root_lp_time, "projection_s": r.projection_time,
+                     "rounding_s": r.rounding_time}
+    print(json.dumps(out), flush=True)
 
 
 def time_to_first_feasible(runs: int, cpu: bool, large: bool = True):
@@ -571,11 +599,19 @@ def main():
                     help="with column slices, run only the rows with more than this many "
                          "nonzeros over the slices (0: every row) instead of the "
                          "autotuner's choice")
+    ap.add_argument("--pump", action="store_true",
+                    help="only time the pump to its first feasible point on the workload's "
+                         "MILP (FP32 polished, then 8 trajectories), each for at most "
+                         "--pump-time-limit seconds")
+    ap.add_argument("--pump-time-limit", type=float, default=120.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device 10 times (counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.pump:
+        pump_on_workload(args.workload, args.pump_time_limit)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the projector has no multi-GPU path in this build "
