@@ -546,9 +546,10 @@ def bench_pack(args) -> None:
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(workload: str, kernel_key: str):
-    """HBM bytes per launch of a kernel from the committed counter summary
-    (profiles/pmc_traffic_<workload>.md, its ```json block), or None."""
+def measured_record(workload: str, kernel_key: str):
+    """The committed counter summary's record of a pass
+    (profiles/pmc_traffic_<workload>.md, its ```json block): HBM bytes per
+    launch and the kernel instantiation they were counted on; or None."""
     path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload.replace('-', '_')}.md")
     if not os.path.exists(path):
         return None
@@ -556,9 +557,20 @@ def measured_traffic(workload: str, kernel_key: str):
     if "```json" not in text:
         return None
     block = text.split("```json", 1)[1].split("```", 1)[0]
-    recs = json.loads(block)
-    rec = recs.get(kernel_key)
+    return json.loads(block).get(kernel_key)
+
+
+def measured_traffic(workload: str, kernel_key: str):
+    """HBM bytes per launch of a pass from the committed counter summary, or None."""
+    rec = measured_record(workload, kernel_key)
     return None if rec is None else rec["traffic_bytes"]
+
+
+def instantiation(kernel_name: str) -> tuple[str, int]:
+    """(kernel template, nonzeros per block) of a CSR pass's name, e.g.
+    ('csr_pass', 256) for 'csr_pass<256, 256, float, ...>'."""
+    head, args = kernel_name.split("<", 1)
+    return head.strip(), int(args.split(",")[1])
 
 
 def main():
@@ -750,14 +762,34 @@ def main():
         (bat, head["ms_at"], "transpose pass (A^T z + x/y update)", keys[0]),
         (ba, head["ms_a"], "row pass (A xbar + z update)", keys[1]), key=lambda t: t[1])
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    traffic = measured_traffic(args.workload, dom_key)
-    if traffic is not None and dom_key.startswith("EpiSlicePartial"):
+    # The timed launches' instantiations (kernel template, nonzeros per
+    # block), to check that the committed counts were taken on the same ones.
+    sched = head["npb"]
+    if K > 1:
+        def grid(p):
+            return "csr_pass_persistent" if p else "csr_pass"
+        timed = ([(grid(sched["At_persistent_wg_per_cu"]), sched["At"])] if dom_key == keys[0]
+                 else [(grid(sched["A_persistent_wg_per_cu"]), sched["A"])])
+    else:
+        timed = ([("csr_pass" if not sched["At_persistent_wg_per_cu"] else
+                   "csr_pass_persistent", sched["At"])] if dom_key == keys[0] else
+                 [("csr_pass" if not sched["A_persistent_wg_per_cu"] else
+                   "csr_pass_persistent", sched["A"])])
+    recs = [measured_record(args.workload, dom_key)]
+    if dom_key.startswith("EpiSlicePartial"):
         # the sliced A pass is two launches: the slices' pass, then the combine
         # (every row over the slices) or the short rows' pass
-        second = (f"k_batch_slice_dualz<{width}, false>" if not head["npb"]["A_slice_min_len"]
+        second = (f"k_batch_slice_dualz<{width}, false>" if not sched["A_slice_min_len"]
                   else f"EpiBatchDualZLong<{width}, false>")
-        combine = measured_traffic(args.workload, second)
-        traffic = None if combine is None else traffic + combine
+        recs.append(measured_record(args.workload, second))
+        timed = [("csr_pass_xcd_segments", sched["A_column_slices"])]
+        if sched["A_slice_min_len"]:
+            timed.append((grid(sched["A_persistent_wg_per_cu"]), sched["A"]))
+    traffic = (None if any(r is None for r in recs)
+               else sum(r["traffic_bytes"] for r in recs))
+    counted_on = [r["kernel_fetch_run"] for r in recs if r is not None]
+    counted = [instantiation(k) for k in counted_on if k.startswith("csr_pass")]
+    same_kernels = counted == timed
     # The random-gather floor of each pass: nnz gathers of one record per
     # column (Aᵀz gathers z: m records; A x̄ gathers x̄: n records).
     from pdfp_amd._lib import gather_ceiling_us
@@ -791,7 +823,12 @@ def main():
                      "traffic_gbs": (None if traffic is None
                                      else traffic / (dom_ms * 1e-3) / 1e9),
                      "traffic_frac": (None if traffic is None
-                                      else traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS)},
+                                      else traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
+                     # the kernels the committed counts come from, and whether
+                     # they are the timed launches' instantiations (the
+                     # autotuner may pick another block size on another run)
+                     "traffic_counted_on": counted_on,
+                     "traffic_same_instantiation": same_kernels},
         "iteration_us": head["ms_iter"] * 1e3,
         "kernel_us": {"transpose_pass": head["ms_at"] * 1e3, "row_pass": head["ms_a"] * 1e3},
         "gather_ceiling_us": ceil,

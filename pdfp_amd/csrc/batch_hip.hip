@@ -403,7 +403,9 @@ struct Batch final : BatchBase {
   // best of two or three timings of each form) for each nonzeros-per-block
   // of A and Aᵀ and each grid form, then for each operator the row-length
   // bounds of the lane-per-(row, trajectory) reduction, then the column
-  // slices of the A pass; keep the fastest.
+  // slices of the A pass.  A form replaces the one kept so far only if it is
+  // at least 2% faster, so that timing noise does not decide between forms
+  // that run alike (the first form tried is kept then).
   void autotune() {
     if (nnz == 0) return;
     constexpr int kRows = kBlock / kBatch;
@@ -436,6 +438,7 @@ struct Batch final : BatchBase {
       for (int r = 0; r < reps; ++r) t = std::min(t, time8());
       return t;
     };
+    constexpr float kGain = 0.98f;  // the margin a form must win by
     float best = 1e30f;
     int bA = 256, bAt = 256, bP = 0;
     for (int na : {256, 512})
@@ -445,7 +448,7 @@ struct Batch final : BatchBase {
           At.schedule(nt, stream, kRows);
           A.persist = At.persist = persist;
           const float ms = best_of(2);
-          if (ms < best) {
+          if (ms < kGain * best) {
             best = ms;
             bA = na;
             bAt = nt;
@@ -461,7 +464,7 @@ struct Batch final : BatchBase {
       for (int len : {32, 128}) {
         M->schedule(M->npb, stream, kRows, len);
         const float ms = best_of(2);
-        if (ms < best) {
+        if (ms < kGain * best) {
           best = ms;
           bl = len;
         }
@@ -497,13 +500,13 @@ struct Batch final : BatchBase {
           }
         }
         for (int v = 0; v < 2; ++v)
-          if (t_sl[v] < t_best) {
+          if (t_sl[v] < kGain * t_best) {
             t_best = t_sl[v];
             best_len = len;
             best_npb = npbs[v];
           }
       }
-      if (t_best < 0.98f * t_one) {
+      if (t_best < kGain * t_one) {
         slice_min_len = best_len;
         build_slices();
         slice_schedule(best_npb);
@@ -521,7 +524,7 @@ struct Batch final : BatchBase {
               A.schedule(na, stream, kRows, ln);
               short_schedule();
               const float ms = best_of(3);
-              if (ms < t) {
+              if (ms < kGain * t) {
                 t = ms;
                 na_best = na;
                 ln_best = ln;

@@ -88,6 +88,24 @@ def test_root_lp_infeasible_detected():
     assert st == so and it == ito
 
 
+@pytest.mark.parametrize("path", ["auto", "general"])
+@pytest.mark.parametrize("maker", ["_unbounded_lp", "_unbounded_mixed"])
+def test_root_lp_dual_infeasible_detected(maker, path):
+    """LPs whose objective improves without bound (HiGHS: unbounded, pinned
+    in tests/test_oracle_residuals.py) end DUAL_INFEASIBLE under cuPDLPx
+    termination (src/pdlp.jl:289-290, 359-362) on the device, after the
+    oracle's iteration count, on both kernel paths."""
+    from . import test_oracle_residuals as R
+
+    print("dual infeasible", maker, path, flush=True)
+    milp = getattr(R, maker)()
+    xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-4, cupdlpx_termination=True)
+    x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-4, time_limit=30.0,
+                                          cupdlpx_termination=True, kernel_path=path)
+    assert so == "DUAL_INFEASIBLE"
+    assert st == so and it == ito
+
+
 def test_root_lp_mixed_generated():
     m = G.mixed(500, 1000, 8)
     m.c = np.random.default_rng(3).standard_normal(1000)
