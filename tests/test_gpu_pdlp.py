@@ -73,9 +73,11 @@ def test_root_lp_small_path_runs_on_small_instances():
     np.testing.assert_allclose(out["auto"][1], out["general"][1], rtol=0, atol=1e-9)
 
 
-def test_root_lp_infeasible_detected():
+@pytest.mark.parametrize("path", ["auto", "general"])
+def test_root_lp_infeasible_detected(path):
     """An infeasible mixed LP ends PRIMAL_INFEASIBLE under cuPDLPx
-    termination (src/pdlp.jl:355-358), on the device as on the oracle."""
+    termination (src/pdlp.jl:355-358), on the device as on the oracle, on
+    both kernel paths."""
     import scipy.sparse as sp
     A = sp.csc_matrix(np.array([[1.0, 1.0], [-1.0, -1.0]]))
     milp = G.MILP(c=np.array([1.0, 1.0]), c0=0.0, A=A, b=np.array([1.0, -3.0]),
@@ -83,7 +85,7 @@ def test_root_lp_infeasible_detected():
                   int_idx=np.array([0]))
     xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-4, cupdlpx_termination=True)
     x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-4, time_limit=30.0,
-                                          cupdlpx_termination=True)
+                                          cupdlpx_termination=True, kernel_path=path)
     assert so == "PRIMAL_INFEASIBLE"
     assert st == so and it == ito
 
