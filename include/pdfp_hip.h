@@ -263,7 +263,7 @@ int pdfp_batch_set_block_graph(pdfp_batch* ctx, int32_t on);
  * replaces the 2 launches per PDHG step of the reference's 200-step block
  * loop, src/pdfp.jl:259-267): wg_per_cu != 0 runs iterations 2 .. freq-1 of
  * each block of the batched passes in one cooperative launch of wg_per_cu
- * workgroups per CU (-1: as many as fit), all resident, with a grid barrier
+ * workgroups per CU (-1: one fewer than fit), all resident, with a grid barrier
  * between the passes; 0 (the default) launches each pass.  Both give
  * identical results.  PDFP_ERR_ARGUMENT for wg_per_cu outside -1 .. 16 or
  * above what fits; PDFP_ERR_DEVICE if the device has no cooperative launch,

@@ -150,7 +150,7 @@ class PDFPBatch:
     def set_cooperative(self, wg_per_cu) -> None:
         """Run the middle iterations of each block of the batched passes in
         one cooperative launch of `wg_per_cu` workgroups per CU (True or -1:
-        as many as fit), with a grid barrier between the passes, or launch
+        one fewer than fit), with a grid barrier between the passes, or launch
         each pass (False or 0, the default).  Both give identical results."""
         n = -1 if wg_per_cu is True else int(wg_per_cu)
         self._check(self._lib.pdfp_batch_set_cooperative(self._ctx, n))

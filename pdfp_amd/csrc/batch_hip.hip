@@ -785,9 +785,11 @@ struct Batch final : BatchBase {
     for (auto& r : coop_grid_cache) r[0] = r[1] = 0;
   }
 
-  // Workgroups of the cooperative launch: coop_per_cu per CU, or as many as
-  // are resident at once (the occupancy of the kernel × the CUs), a
-  // multiple of the 8 XCDs.
+  // Workgroups of the cooperative launch: coop_per_cu per CU, or one per CU
+  // fewer than the occupancy of the kernel allows (× the CUs), a multiple of
+  // the 8 XCDs.  A grid at the occupancy limit leaves no slack for a CU that
+  // holds fewer: at 7 per CU (1792 workgroups) a barrier wait timed out (the
+  // bounded wait ended the launch) where 6 per CU had run.
   template <class Kern>
   int coop_grid(Kern kern, int& cache) {
     if (cache == 0) {
@@ -796,6 +798,7 @@ struct Batch final : BatchBase {
       if (coop_per_cu > per_cu)
         throw Error(PDFP_ERR_ARGUMENT, "more cooperative workgroups per CU than fit");
       if (coop_per_cu > 0) per_cu = coop_per_cu;
+      else if (per_cu > 1) per_cu -= 1;
       PDFP_HIP(hipGetDevice(&dev));
       PDFP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
       cache = per_cu * cus / 8 * 8;
