@@ -602,8 +602,8 @@ def main():
                          "trace graph replays)")
     ap.add_argument("--cooperative", type=int, default=None,
                     help="workgroups per CU of a cooperative launch running the middle "
-                         "iterations of each block of the batched passes (-1: as many as "
-                         "fit; 0: launch every pass)")
+                         "iterations of each block of the batched passes (-1: one fewer "
+                         "than fit; 0: launch every pass)")
     ap.add_argument("--column-slices", choices=("on", "off", "256", "512"), default=None,
                     help="run the batch's A pass over A's 8 column slices (on) or in one "
                          "pass (off) instead of the autotuner's choice")

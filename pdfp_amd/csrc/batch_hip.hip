@@ -763,7 +763,7 @@ struct Batch final : BatchBase {
   DevBuf<GridBarrier> bar;
   PinnedBuf<GridBarrier> h_bar;
   int coop_grid_cache[2][2] = {{0, 0}, {0, 0}};
-  int coop_per_cu = 0;  // workgroups per CU of the launch (0: as many as fit)
+  int coop_per_cu = 0;  // workgroups per CU of the launch (0: one fewer than fit)
   bool coop_pending = false;
 
   void set_coop(int per_cu) override {
