@@ -244,7 +244,7 @@ def test_batch_schedules_agree(K):
             assert abs(l1b - ref[k]) <= 1e-4 * max(1.0, ref[k]), (sched, k, l1b, ref[k])
 
 
-@pytest.mark.parametrize("min_len", [0, 64])
+@pytest.mark.parametrize("min_len", [0, 64, 100_000])
 @pytest.mark.parametrize("K", [8, 16, 5])
 def test_batch_column_slices_agree(K, min_len):
     """The A pass over A's 8 column slices (one per XCD, FP32 partial sums
@@ -252,7 +252,8 @@ def test_batch_column_slices_agree(K, min_len):
     KKT within 10×tol and the single FP32 projector's ℓ1 distance to 1e-4,
     on power-law rows with a dense column; the replayed block graphs equal
     the stream launches bit for bit.  min_len 64: only the rows longer than
-    64 nonzeros run over the slices, the others in one pass."""
+    64 nonzeros run over the slices, the others in one pass; 100,000: no row
+    is that long, so the slices' pass has nothing to do."""
     print("batch column slices", K, min_len, flush=True)
     m = _with_dense_column(G.mixed(2000, 5000, 5, power_law=True, nnz=60_000, max_len=3000))
     pp = projector_problem(m)
