@@ -611,6 +611,9 @@ def main():
                     help="with column slices, run only the rows with more than this many "
                          "nonzeros over the slices (0: every row) instead of the "
                          "autotuner's choice")
+    ap.add_argument("--batch-nt", choices=("on", "off"), default=None,
+                    help="the batched passes' operand loads: non-temporal (on) or plain "
+                         "(off) instead of the autotuner's choice")
     ap.add_argument("--pump", action="store_true",
                     help="only time the pump to its first feasible point on the workload's "
                          "MILP (FP32 polished, then 8 trajectories), each for at most "
@@ -704,6 +707,8 @@ def main():
                                   "512": 512}[args.column_slices])
         if args.slice_min_len is not None:
             bt.set_slice_min_len(args.slice_min_len)
+        if args.batch_nt is not None:
+            bt.set_nontemporal(args.batch_nt == "on")
         for k in range(K):
             bt.set_x(k, np.clip(np.zeros(nn), milp.l, milp.u) / pp.D_c)
         sets_w = [[G.binary_targets(milp, 3000 + 100 * s + k) for k in range(K)]
@@ -726,7 +731,8 @@ def main():
                                      "A_lane_rows": bi.lane_rows_a,
                                      "At_lane_rows": bi.lane_rows_at,
                                      "A_column_slices": bi.column_slices,
-                                     "A_slice_min_len": bi.slice_min_len})
+                                     "A_slice_min_len": bi.slice_min_len,
+                                     "nontemporal": bi.nontemporal})
         print(f"[bench] fp32 batch of {K}: {it} iterations in {dt:.3f}s = {it / dt:.0f} it/s; "
               f"kernels AT {ms_at * 1e3:.1f}us A {ms_a * 1e3:.1f}us; batched iteration "
               f"{ms_iter * 1e3:.1f}us", file=sys.stderr, flush=True)
@@ -753,8 +759,9 @@ def main():
     r32 = results["fp32"]
     value = head["iters"] / head["seconds"]
     width = 8 if K <= 8 else 16
-    keys = ((f"EpiBatchPrimalY<{width}, false>", f"EpiBatchDualZ<{width}, false>") if K > 1 else
-            ("EpiPrimalY<float, false>", "EpiDualZ<float, false>"))
+    nt = "NT" if K > 1 and head["npb"].get("nontemporal") else ""
+    keys = ((f"EpiBatchPrimalY{nt}<{width}, false>", f"EpiBatchDualZ{nt}<{width}, false>")
+            if K > 1 else ("EpiPrimalY<float, false>", "EpiDualZ<float, false>"))
     if K > 1 and head["npb"].get("A_column_slices"):
         keys = (keys[0], f"EpiSlicePartialLanes<{width}>" if head["npb"]["A_slice_min_len"]
                 else f"EpiSlicePartial<{width}>")
@@ -779,8 +786,9 @@ def main():
     if dom_key.startswith("EpiSlicePartial"):
         # the sliced A pass is two launches: the slices' pass, then the combine
         # (every row over the slices) or the short rows' pass
-        second = (f"k_batch_slice_dualz<{width}, false>" if not sched["A_slice_min_len"]
-                  else f"EpiBatchDualZLong<{width}, false>")
+        second = ((f"k_batch_slice_dualz_nt<{width}, false>" if nt else
+                   f"k_batch_slice_dualz<{width}, false>") if not sched["A_slice_min_len"]
+                  else f"EpiBatchDualZLong{nt}<{width}, false>")
         recs.append(measured_record(args.workload, second))
         timed = [("csr_pass_xcd_segments", sched["A_column_slices"])]
         if sched["A_slice_min_len"]:

@@ -121,6 +121,8 @@ typedef struct pdfp_info {
   int32_t slice_min_len;     /* batch, with column slices: the rows longer than */
                              /*   this run over the slices, the others in one   */
                              /*   pass; 0: every row                            */
+  int32_t nontemporal;       /* batch: 1 if the passes load their per-row       */
+                             /*   operands with non-temporal loads (autotuned)  */
 } pdfp_info;
 
 /* Fills `p` with the reference defaults. */
@@ -294,6 +296,13 @@ int pdfp_batch_set_column_slices(pdfp_batch* ctx, int32_t on);
  * slices (0, or 32, 64, 128 or 256 where the longer rows hold 20-95% of the
  * nonzeros).  PDFP_ERR_ARGUMENT for min_len < 0. */
 int pdfp_batch_set_slice_min_len(pdfp_batch* ctx, int32_t min_len);
+/* How the batched passes load their per-row and per-column operands (no
+ * counterpart in the reference): 1 with non-temporal loads, which keep them
+ * from displacing the gathered vectors' lines in L2 (faster where the
+ * iterates outgrow the memory-side cache), 0 with plain loads.  The results
+ * are the same; pdfp_batch_create's autotuner picks (pdfp_info's
+ * nontemporal).  PDFP_ERR_ARGUMENT unless 0 or 1. */
+int pdfp_batch_set_nontemporal(pdfp_batch* ctx, int32_t on);
 /* Kernel path of the batch (no counterpart in the reference), as
  * pdfp_set_kernel_path: 0 (the default) runs each block as one launch with
  * one workgroup per running trajectory when the instance fits in LDS (each

@@ -69,6 +69,7 @@ class PdfpInfo(C.Structure):
         ("block_graph", C.c_int32),
         ("column_slices", C.c_int32),
         ("slice_min_len", C.c_int32),
+        ("nontemporal", C.c_int32),
     ]
 
 
@@ -153,6 +154,7 @@ SIGNATURES = {
     "pdfp_batch_set_cooperative": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_set_column_slices": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_set_slice_min_len": (C.c_int, [_P, C.c_int32]),
+    "pdfp_batch_set_nontemporal": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_set_kernel_path": (C.c_int, [_P, C.c_int32]),
     "pdfp_pack_last_error": (C.c_char_p, [_P]),
     "pdfp_pack_create": (C.c_int, [C.POINTER(_P), C.c_int32, _I64, _I64, C.POINTER(_I64),

@@ -171,6 +171,12 @@ class PDFPBatch:
         slices."""
         self._check(self._lib.pdfp_batch_set_slice_min_len(self._ctx, int(min_len)))
 
+    def set_nontemporal(self, on: bool) -> None:
+        """Load the passes' per-row and per-column operands with
+        non-temporal loads (True) or plain loads (False); same results.  The
+        autotuner picks."""
+        self._check(self._lib.pdfp_batch_set_nontemporal(self._ctx, int(bool(on))))
+
     def set_schedule(self, npb_a: int, npb_at: int, persist: int = 0, lane_rows_a: int = 0,
                      lane_rows_at: int = 0) -> None:
         """Replace the autotuned schedule of the batched passes (tests and

@@ -95,6 +95,7 @@ struct BatchBase {
   virtual void set_coop(int per_cu) = 0;
   virtual void set_slices(int on) = 0;
   virtual void set_slice_min_len(int len) = 0;
+  virtual void set_nt(int on) = 0;
   virtual void set_original_primal_residual(const double* D_r) = 0;
   virtual void set_x(int k, const double* xs) = 0;
   virtual void set_target(int k, const double* xts) = 0;
@@ -386,17 +387,26 @@ struct Batch final : BatchBase {
     else launch_pass_npb<256, 0, 0>(M, g, e, nullptr, stream, t0, t1);
   }
 
-  template <bool MAJOR>
-  void step_t(int k_offset) {
-    EpiBatchPrimalY<KB, MAJOR> ex{d_sp.p, d_wt.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
-                              rx.p, px.p, ds.p, py.p, ry.p};
-    launch_batch(At, GatherBatch<KB>{z.p}, ex);
+  // One iteration's two passes; with ev, ev[0..1] bracket the Aᵀ pass and
+  // ev[2..3] the A pass (kernel start and end events).
+  template <bool MAJOR, bool NT>
+  void step_nt(int k_offset, hipEvent_t* ev) {
+    using EX = std::conditional_t<NT, EpiBatchPrimalYNT<KB, MAJOR>, EpiBatchPrimalY<KB, MAJOR>>;
+    using EZ = std::conditional_t<NT, EpiBatchDualZNT<KB, MAJOR>, EpiBatchDualZ<KB, MAJOR>>;
+    const EX ex{d_sp.p, d_wt.p, k_offset, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
+                rx.p, px.p, ds.p, py.p, ry.p};
+    launch_batch(At, GatherBatch<KB>{z.p}, ex, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
     if (slices_on) {
-      launch_sliced_a<MAJOR>(k_offset);
+      launch_sliced_a<MAJOR, NT>(k_offset, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
       return;
     }
-    EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-    launch_batch(A, GatherBatch<KB>{rx.p}, ez);
+    const EZ ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    launch_batch(A, GatherBatch<KB>{rx.p}, ez, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+  }
+  template <bool MAJOR>
+  void step_t(int k_offset, hipEvent_t* ev = nullptr) {
+    if (epi_nt) step_nt<MAJOR, true>(k_offset, ev);
+    else step_nt<MAJOR, false>(k_offset, ev);
   }
 
   // Schedule autotuner of the batched passes: time whole iterations (the
@@ -538,6 +548,14 @@ struct Batch final : BatchBase {
         slice_min_len = 0;
         drop_slices();
       }
+    }
+    // the epilogues' operand loads: non-temporal only if 2% faster
+    {
+      epi_nt = false;
+      const float t_plain = best_of(3);
+      epi_nt = true;
+      const float t_nt = best_of(3);
+      epi_nt = t_nt < kGain * t_plain;
     }
     upload_steps(1.0, none);
     for (DevBuf<float>* d : {&x, &x0, &y, &y0, &rx, &px, &ds, &py, &ry, &z, &z0, &pz, &rz})
@@ -876,6 +894,7 @@ struct Batch final : BatchBase {
   bool slices_on = false;
   int32_t slice_min_len = 0;
   int32_t built_min_len = -1;  // the bound As was built for; -1: not built
+  bool epi_nt = false;  // the epilogues' operands by non-temporal loads (ld_epi)
   int64_t n_long = 0;
 
   // Column slices pay when x̄ (n·KB FP32 values) is larger than an XCD's
@@ -1040,6 +1059,13 @@ struct Batch final : BatchBase {
     slices_on = on != 0;
   }
 
+  void set_nt(int on) override {
+    if (on != 0 && on != 1) throw Error(PDFP_ERR_ARGUMENT, "non-temporal loads: 0 or 1");
+    sync();
+    drop_graphs();  // the captured launches name the epilogues' kernels
+    epi_nt = on != 0;
+  }
+
   void set_slice_min_len(int len) override {
     if (len < 0) throw Error(PDFP_ERR_ARGUMENT, "the long-row bound must be at least 0");
     sync();
@@ -1054,7 +1080,7 @@ struct Batch final : BatchBase {
 
   // A x̄ and the z update of iteration k_offset over the column slices; t0 /
   // t1 (optional) bracket both launches.
-  template <bool MAJOR>
+  template <bool MAJOR, bool NT>
   void launch_sliced_a(int k_offset, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
     SegBlocks sb{};
     int maxb = 0;
@@ -1082,18 +1108,24 @@ struct Batch final : BatchBase {
       else launch(csr_pass_xcd_segments<kBlock, 256, float, GatherBatch<KB>, E, 0, 0>, ep);
     }
     PDFP_HIP(hipGetLastError());
-    EpiBatchDualZ<KB, MAJOR> ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
+    using EZ = std::conditional_t<NT, EpiBatchDualZNT<KB, MAJOR>, EpiBatchDualZ<KB, MAJOR>>;
+    using EL = std::conditional_t<NT, EpiBatchDualZLongNT<KB, MAJOR>, EpiBatchDualZLong<KB, MAJOR>>;
+    const EZ ez{d_sp.p, d_wt.p, k_offset, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
     if (slice_min_len > 0) {
-      const EpiBatchDualZLong<KB, MAJOR> el{ez, long_idx.p, part_s.p, (int32_t)n_long};
+      const EL el{ez, long_idx.p, part_s.p, (int32_t)n_long};
       launch_batch(Ash, GatherBatch<KB>{rx.p}, el, nullptr, t1);
       return;
     }
     const int g2 = grid_for(m * KB);
+    auto kern = [] {
+      if constexpr (NT) return k_batch_slice_dualz_nt<KB, MAJOR>;
+      else return k_batch_slice_dualz<KB, MAJOR>;
+    }();
     if (t1)
-      hipExtLaunchKernelGGL(k_batch_slice_dualz<KB, MAJOR>, dim3(g2), dim3(256), 0, stream,
-                            nullptr, t1, 0, (int)m, (const float*)part_s.p, ez);
+      hipExtLaunchKernelGGL(kern, dim3(g2), dim3(256), 0, stream, nullptr, t1, 0, (int)m,
+                            (const float*)part_s.p, ez);
     else
-      k_batch_slice_dualz<KB, MAJOR><<<g2, 256, 0, stream>>>((int)m, part_s.p, ez);
+      kern<<<g2, 256, 0, stream>>>((int)m, part_s.p, ez);
     PDFP_HIP(hipGetLastError());
   }
 
@@ -1322,6 +1354,7 @@ struct Batch final : BatchBase {
     o->small_path = small_used ? 1 : 0;
     o->column_slices = slices_on && !small_used ? As.npb : 0;
     o->slice_min_len = slices_on && !small_used ? slice_min_len : 0;
+    o->nontemporal = epi_nt && !small_used ? 1 : 0;
   }
 
   // Mean duration of the transpose pass and of the A pass of a batched
@@ -1330,20 +1363,7 @@ struct Batch final : BatchBase {
     time_iterations(1);
     std::vector<hipEvent_t> ev((size_t)iters * 4);
     for (auto& e : ev) PDFP_HIP(hipEventCreate(&e));
-    for (int i = 0; i < iters; ++i) {
-      const int k = 2 + (i % 198);
-      EpiBatchPrimalY<KB, false> ex{d_sp.p, d_wt.p, k, x.p, x0.p, y.p, y0.p, xt.p, lb.p, ub.p, w.p,
-                                rx.p, px.p, ds.p, py.p, ry.p};
-      launch_batch(At, GatherBatch<KB>{z.p}, ex,
-          ev[(size_t)i * 4 + 0], ev[(size_t)i * 4 + 1]);
-      if (slices_on) {
-        launch_sliced_a<false>(k, ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
-        continue;
-      }
-      EpiBatchDualZ<KB, false> ez{d_sp.p, d_wt.p, k, z.p, z0.p, b.p, is_eq.p, pz.p, rz.p};
-      launch_batch(A, GatherBatch<KB>{rx.p}, ez,
-          ev[(size_t)i * 4 + 2], ev[(size_t)i * 4 + 3]);
-    }
+    for (int i = 0; i < iters; ++i) step_t<false>(2 + (i % 198), &ev[(size_t)i * 4]);
     sync();
     double sa = 0.0, sb = 0.0;
     for (int i = 0; i < iters; ++i) {
@@ -1596,6 +1616,10 @@ int pdfp_batch_set_column_slices(pdfp_batch* ctx, int32_t on) {
 
 int pdfp_batch_set_slice_min_len(pdfp_batch* ctx, int32_t min_len) {
   return batch_guarded(ctx, [&] { ctx->impl->set_slice_min_len(min_len); });
+}
+
+int pdfp_batch_set_nontemporal(pdfp_batch* ctx, int32_t on) {
+  return batch_guarded(ctx, [&] { ctx->impl->set_nt(on); });
 }
 
 int pdfp_batch_set_cooperative(pdfp_batch* ctx, int32_t wg_per_cu) {

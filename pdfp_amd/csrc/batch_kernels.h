@@ -87,12 +87,25 @@ __device__ __forceinline__ BatchScalars load_batch_scalars(const BatchStep* __re
   return {batch_scalars<KB>(st, wt, k_offset, q), st.active};
 }
 
+// Loads of an epilogue's per-trajectory operands, which a pass reads once:
+// plain, or non-temporal (NT), which keeps them from displacing the gathered
+// vector's lines in L2.  Non-temporal loads measured 5 % faster per batched
+// iteration on the 5M-nonzero power-law workload and 8 % slower on the
+// 1M-nonzero one, whose iterates stay in the memory-side cache between
+// passes; the autotuner picks per instance (the NT epilogues are the ones
+// named ...NT below).
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_epi(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // Column j, trajectory q (lane q of the row's KB): the fused primal and dual-y
 // step (src/pdfp.jl:119-143) with s = (Aᵀ z_q)_j.  An inactive trajectory
 // is left untouched.  MAJOR steps also store pdhg x, the dual slack, pdhg y
 // and reflected y.
-template <int KB, bool MAJOR>
-struct EpiBatchPrimalY {
+template <int KB, bool MAJOR, bool NT>
+struct EpiBatchPrimalYT {
   static constexpr int kBatch = KB;
   static constexpr int kLanes = KB;
   struct Pre {
@@ -116,7 +129,8 @@ struct EpiBatchPrimalY {
   float* __restrict__ ry;
   __device__ __forceinline__ Pre pre(int j, int q) const {
     const size_t o = (size_t)j * kBatch + q;
-    return {x[o], y[o], x0[o], y0[o], xt[o], lb[j], ub[j], w[j]};
+    return {ld_epi<NT>(x + o), ld_epi<NT>(y + o), ld_epi<NT>(x0 + o), ld_epi<NT>(y0 + o),
+            ld_epi<NT>(xt + o), lb[j], ub[j], w[j]};
   }
   __device__ __forceinline__ void operator()(int j, int q, const Pre& p, double s,
                                              double*) const {
@@ -140,10 +154,15 @@ struct EpiBatchPrimalY {
   }
 };
 
+template <int KB, bool MAJOR>
+struct EpiBatchPrimalY : EpiBatchPrimalYT<KB, MAJOR, false> {};
+template <int KB, bool MAJOR>
+struct EpiBatchPrimalYNT : EpiBatchPrimalYT<KB, MAJOR, true> {};
+
 // Row i, trajectory q: the dual-z step (src/pdfp.jl:146-156) with
 // s = (A x̄_q)_i.
-template <int KB, bool MAJOR>
-struct EpiBatchDualZ {
+template <int KB, bool MAJOR, bool NT>
+struct EpiBatchDualZT {
   static constexpr int kBatch = KB;
   static constexpr int kLanes = KB;
   struct Pre {
@@ -161,7 +180,7 @@ struct EpiBatchDualZ {
   float* __restrict__ rz;
   __device__ __forceinline__ Pre pre(int i, int q) const {
     const size_t o = (size_t)i * kBatch + q;
-    return {z[o], z0[o], b[i], (uint32_t)is_eq[i]};
+    return {ld_epi<NT>(z + o), ld_epi<NT>(z0 + o), b[i], (uint32_t)is_eq[i]};
   }
   __device__ __forceinline__ void operator()(int i, int q, const Pre& p, double s,
                                              double*) const {
@@ -176,6 +195,11 @@ struct EpiBatchDualZ {
     }
   }
 };
+
+template <int KB, bool MAJOR>
+struct EpiBatchDualZ : EpiBatchDualZT<KB, MAJOR, false> {};
+template <int KB, bool MAJOR>
+struct EpiBatchDualZNT : EpiBatchDualZT<KB, MAJOR, true> {};
 
 // ---- the A pass over column slices -------------------------------------------
 //
@@ -230,13 +254,14 @@ struct EpiSlicePartialLanes {
   }
 };
 
-template <int KB, bool MAJOR>
-__global__ __launch_bounds__(256) void k_batch_slice_dualz(int m, const float* __restrict__ part,
-                                                           EpiBatchDualZ<KB, MAJOR> ez) {
+// Row i, trajectory q of the combine: the 8 partials in slice order in
+// FP64, then the z update.
+template <int KB, class EZ>
+__device__ __forceinline__ void slice_dualz(int m, const float* __restrict__ part, const EZ& ez) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (e >= (int64_t)m * KB) return;
   const int i = (int)(e / KB), q = (int)(e % KB);
-  const typename EpiBatchDualZ<KB, MAJOR>::Pre p = ez.pre(i, q);
+  const typename EZ::Pre p = ez.pre(i, q);
   float v[8];
 #pragma unroll
   for (int sl = 0; sl < 8; ++sl) v[sl] = part[(size_t)sl * (size_t)m * KB + (size_t)e];
@@ -244,6 +269,18 @@ __global__ __launch_bounds__(256) void k_batch_slice_dualz(int m, const float* _
 #pragma unroll
   for (int sl = 0; sl < 8; ++sl) sum += (double)v[sl];
   ez(i, q, p, sum, nullptr);
+}
+
+template <int KB, bool MAJOR>
+__global__ __launch_bounds__(256) void k_batch_slice_dualz(int m, const float* __restrict__ part,
+                                                           EpiBatchDualZ<KB, MAJOR> ez) {
+  slice_dualz<KB>(m, part, ez);
+}
+template <int KB, bool MAJOR>
+__global__ __launch_bounds__(256) void k_batch_slice_dualz_nt(int m,
+                                                              const float* __restrict__ part,
+                                                              EpiBatchDualZNT<KB, MAJOR> ez) {
+  slice_dualz<KB>(m, part, ez);
 }
 
 // The A pass when only A's long rows run over the column slices: the short
@@ -254,11 +291,11 @@ __global__ __launch_bounds__(256) void k_batch_slice_dualz(int m, const float* _
 // k_batch_slice_dualz forms it; then the z update.  On power-law rows most
 // nonzeros sit in a few long rows: their gathers come from the XCDs' L2s,
 // while the many short rows keep one pass and no partials.
-template <int KB, bool MAJOR>
-struct EpiBatchDualZLong {
+template <int KB, bool MAJOR, bool NT>
+struct EpiBatchDualZLongT {
   static constexpr int kBatch = KB;
   static constexpr int kLanes = KB;
-  using Z = EpiBatchDualZ<KB, MAJOR>;
+  using Z = EpiBatchDualZT<KB, MAJOR, NT>;
   struct Pre {
     typename Z::Pre z;
     int32_t l;
@@ -281,6 +318,11 @@ struct EpiBatchDualZLong {
     ez(i, q, p.z, s, acc);
   }
 };
+
+template <int KB, bool MAJOR>
+struct EpiBatchDualZLong : EpiBatchDualZLongT<KB, MAJOR, false> {};
+template <int KB, bool MAJOR>
+struct EpiBatchDualZLongNT : EpiBatchDualZLongT<KB, MAJOR, true> {};
 
 // ---- the evaluation terms of trajectory k (views of the interleaved vectors) --
 

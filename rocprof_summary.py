@@ -30,11 +30,12 @@ import json
 def pass_key(name: str) -> str | None:
     """The epilogue of a csr_pass / csr_pass_persistent /
     csr_pass_xcd_segments instantiation (its fifth template argument), e.g.
-    'EpiBatchPrimalY<8, false>', or 'k_batch_slice_dualz<8, false>' for the
+    'EpiBatchPrimalY<8, false>', or 'k_batch_slice_dualz<8, false>' (or _nt) for the
     combine launch of the sliced A pass; None for other kernels."""
-    j = name.find("k_batch_slice_dualz<")
-    if j >= 0:  # the second launch of the sliced A pass: its own name
-        return name[j:name.index(">", j) + 1]
+    for combine in ("k_batch_slice_dualz<", "k_batch_slice_dualz_nt<"):
+        j = name.find(combine)
+        if j >= 0:  # the second launch of the sliced A pass: its own name
+            return name[j:name.index(">", j) + 1]
     for head in ("csr_pass_persistent<", "csr_pass_xcd_segments<", "csr_pass<"):
         i = name.find(head)
         if i >= 0:

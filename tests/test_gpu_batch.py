@@ -298,6 +298,38 @@ def test_batch_column_slices_agree(K, min_len):
         assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("slices", [0, 64])
+@pytest.mark.parametrize("K", [8, 16])
+def test_batch_nontemporal_loads_change_nothing(K, slices):
+    """The passes' operands by non-temporal loads give the same trajectories,
+    bit for bit, as by plain loads (one pass, and the long rows over the
+    column slices), and pdfp_info reports the choice."""
+    print("batch nontemporal", K, slices, flush=True)
+    m = _with_dense_column(G.mixed(2000, 5000, 5, power_law=True, nnz=60_000, max_len=3000))
+    pp = projector_problem(m)
+    p = SolverParameters(eps_optimal_relative=1e-5, eps_feasible_relative=1e-5,
+                         iteration_limit=200_000)
+    targets = [G.binary_targets(m, 70 + k) / pp.D_c for k in range(K)]
+    runs = []
+    for nt in (False, True):
+        bt = _batch(pp, K)
+        bt.set_kernel_path("general")
+        bt.set_schedule(512, 256, 0, 32, 32)
+        if slices:
+            bt.set_column_slices(True)
+            bt.set_slice_min_len(slices)
+        bt.set_nontemporal(nt)
+        for k in range(K):
+            bt.set_target(k, targets[k])
+        assert bt.solve(p, 1.0) == ["OPTIMAL"] * K
+        assert bt.info(0).nontemporal == int(nt)
+        runs.append([(bt.info(k).total_count, bt.pdhg_x(k), bt.vector(k, "pdhg_z"))
+                     for k in range(K)])
+        bt.close()
+    for a, b in zip(*runs):
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
 @pytest.mark.parametrize("K", [8, 16])
 def test_batch_block_graph_replays_equal_stream_launches(K):
     """Blocks replayed as captured graphs give the same trajectories, bit for

@@ -148,8 +148,8 @@ def test_uneven_set_covers_follow_their_recipe():
 @pytest.mark.parametrize("workload,keys", [
     ("mixed-1m", ("EpiBatchPrimalY<8, false>", "EpiBatchDualZ<8, false>",
                   "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
-    ("powerlaw-5m", ("EpiBatchPrimalY<8, false>", "EpiSlicePartialLanes<8>",
-                     "EpiBatchDualZLong<8, false>",
+    ("powerlaw-5m", ("EpiBatchPrimalYNT<8, false>", "EpiSlicePartialLanes<8>",
+                     "EpiBatchDualZLongNT<8, false>",
                      "EpiPrimalY<float, false>", "EpiDualZ<float, false>")),
     ("general-int-4m", ("EpiBatchPrimalY<8, false>", "EpiSlicePartial<8>",
                         "k_batch_slice_dualz<8, false>",
