@@ -1,0 +1,103 @@
+"""Device parity on random tiny MILPs (tests/properties.py), drawn by
+hypothesis with a fixed derandomized sequence: the FP64 form, on both
+kernel paths, follows the oracle's trajectory (same termination and
+iteration count, pdhg x within 1e-9); the FP32 form, at the pump's
+tolerance 1e-4 (FP32 iterates can sit above 1e-5 on such instances: one
+drawn instance reached ITERATION_LIMIT at 1e-5), ends OPTIMAL at a point
+whose KKT residuals, evaluated in FP64 by the oracle, are within 10× the
+tolerance and whose ℓ1 distance is the oracle's optimum to the tolerance; the
+batch of 8 trajectories on the general passes reaches the single FP32
+projector's ℓ1 distances."""
+import numpy as np
+import pytest
+from hypothesis import HealthCheck, given, settings
+
+from oracle import oracle as O
+from pdfp_amd import pdfp as P
+from pdfp_amd.batch import PDFPBatch
+from pdfp_amd.pdfp import SolverParameters, projector_problem
+
+from .helpers import l1_distance, project_with
+from .properties import instances, integer_target, tiny_milp
+
+pytestmark = pytest.mark.gpu
+
+SETTINGS = settings(max_examples=40, deadline=None, derandomize=True,
+                    suppress_health_check=[HealthCheck.too_slow])
+
+
+def _state(path, precision):
+    def make(*a, **kw):
+        s = P.PDFPState(*a, precision=precision, **kw)
+        s.set_kernel_path(path)
+        return s
+    return make
+
+
+@pytest.mark.parametrize("path", ["auto", "one-workgroup"])
+@SETTINGS
+@given(spec=instances)
+def test_fp64_follows_the_oracle_on_random_instances(path, spec):
+    milp = tiny_milp(*spec)
+    xt = integer_target(milp, spec[-1])
+    print("fp64 property", path, spec, flush=True)
+    pp, s, st = project_with(P, _state(path, "fp64"), milp, xt, tol=1e-7)
+    pq, o, so = project_with(O, O.OraclePDFPState, milp, xt, tol=1e-7)
+    try:
+        assert st == so == "OPTIMAL", (st, so)
+        assert s.total_count == o.total_count, (s.total_count, o.total_count)
+        np.testing.assert_allclose(s.pdhg_x, o.pdhg_x, rtol=0, atol=1e-9)
+    finally:
+        s.close()
+
+
+@SETTINGS
+@given(spec=instances)
+def test_fp32_reaches_the_oracle_distance_on_random_instances(spec):
+    milp = tiny_milp(*spec)
+    xt = integer_target(milp, spec[-1])
+    print("fp32 property", spec, flush=True)
+    tol = 1e-4
+    pp, s, st = project_with(P, _state("auto", "fp32"), milp, xt, tol=tol)
+    pq, o, so = project_with(O, O.OraclePDFPState, milp, xt, tol=1e-8)
+    try:
+        assert st == so == "OPTIMAL", (st, so)
+        r = o.residual_of(s.pdhg_x, s.vector("pdhg_y"), s.vector("pdhg_z"),
+                          s.vector("dual_slack"))
+        assert r[0] <= 10 * tol and r[1] <= 10 * tol and r[2] <= 10 * tol, r
+        den = 1.0 + np.abs(xt[milp.int_idx]).sum()
+        lo = l1_distance(milp, o.pdhg_x * pq.D_c, xt)
+        assert abs(l1_distance(milp, s.pdhg_x * pp.D_c, xt) - lo) <= 10 * tol * den
+    finally:
+        s.close()
+
+
+@settings(max_examples=15, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow])
+@given(spec=instances)
+def test_batch_matches_single_fp32_on_random_instances(spec):
+    milp = tiny_milp(*spec)
+    pp = projector_problem(milp)
+    print("batch property", spec, flush=True)
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                         iteration_limit=200_000)
+    targets = [integer_target(milp, spec[-1] + 31 * k) / pp.D_c for k in range(8)]
+    ref = []
+    for k in range(8):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, targets[k])
+        assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
+        ref.append(s.info().l1_distance)
+        s.close()
+    bt = PDFPBatch(8, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    try:
+        bt.set_kernel_path("general")
+        for k in range(8):
+            bt.set_target(k, targets[k])
+        assert bt.solve(p, 1.0) == ["OPTIMAL"] * 8
+        for k in range(8):
+            l1b = bt.info(k).l1_distance
+            assert abs(l1b - ref[k]) <= 1e-3 * max(1.0, ref[k]), (k, l1b, ref[k])
+    finally:
+        bt.close()
