@@ -7,15 +7,21 @@ drawn instance reached ITERATION_LIMIT at 1e-5), ends OPTIMAL at a point
 whose KKT residuals, evaluated in FP64 by the oracle, are within 10× the
 tolerance and whose ℓ1 distance is the oracle's optimum to the tolerance; the
 batch of 8 trajectories on the general passes reaches the single FP32
-projector's ℓ1 distances."""
+projector's ℓ1 distances; the root LP follows the oracle's trajectory (same
+termination and iteration count) on both kernel paths; the device scaling
+and the device feasibility check equal the oracle's and the host's bit for
+bit."""
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings
 
 from oracle import oracle as O
 from pdfp_amd import pdfp as P
+from pdfp_amd import pdlp as L
 from pdfp_amd.batch import PDFPBatch
+from pdfp_amd.milp import DeviceViolationCheck, max_violation
 from pdfp_amd.pdfp import SolverParameters, projector_problem
+from pdfp_amd.preconditioner import scale_problem
 
 from .helpers import l1_distance, project_with
 from .properties import instances, integer_target, tiny_milp
@@ -101,3 +107,41 @@ def test_batch_matches_single_fp32_on_random_instances(spec):
             assert abs(l1b - ref[k]) <= 1e-3 * max(1.0, ref[k]), (k, l1b, ref[k])
     finally:
         bt.close()
+
+
+@pytest.mark.parametrize("cupdlpx", [False, True])
+@pytest.mark.parametrize("path", ["auto", "general"])
+@SETTINGS
+@given(spec=instances)
+def test_root_lp_follows_the_oracle_on_random_instances(path, cupdlpx, spec):
+    """Both terminations, whatever the oracle concludes (under cuPDLPx
+    termination that includes the reference's PRIMAL_INFEASIBLE verdicts on
+    bounded feasible LPs, tests/test_properties.py)."""
+    milp = tiny_milp(*spec)
+    milp.c = np.random.default_rng(spec[-1] + 3).standard_normal(len(milp.c))
+    print("root lp property", path, cupdlpx, spec, flush=True)
+    xo, dso, so, ito = O.solve_lp_relaxation(milp, tolerance=1e-6, cupdlpx_termination=cupdlpx)
+    x, ds, st, it = L.solve_lp_relaxation(milp, tolerance=1e-6, time_limit=30.0,
+                                          cupdlpx_termination=cupdlpx, kernel_path=path)
+    assert st == so, (st, so)
+    assert it == ito, (it, ito)
+    if so == "OPTIMAL":
+        np.testing.assert_allclose(x, xo, rtol=0, atol=1e-7 * (1.0 + np.abs(xo).max()))
+
+
+@SETTINGS
+@given(spec=instances)
+def test_scaling_and_check_equal_the_host_on_random_instances(spec):
+    milp = tiny_milp(*spec)
+    print("scale/check property", spec, flush=True)
+    con_lb = np.where(milp.row_sense == "eq", milp.b, -np.inf)
+    a = scale_problem(milp.A, milp.c, milp.l, milp.u, con_lb, milp.b, backend="device")
+    b = O.scale_problem(milp.A, milp.c, milp.l, milp.u, con_lb, milp.b)
+    for k in ("D_r", "D_c"):
+        assert np.array_equal(getattr(a, k), b[k]), k
+    assert np.array_equal(a.A.data, b["A"].data)
+    chk = DeviceViolationCheck(milp)
+    rng = np.random.default_rng(spec[-1])
+    for _ in range(3):
+        x = rng.random(len(milp.c)) * milp.u
+        assert chk(x) == max_violation(milp, x)
