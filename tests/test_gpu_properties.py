@@ -10,15 +10,19 @@ batch of 8 trajectories on the general passes reaches the single FP32
 projector's ℓ1 distances; the root LP follows the oracle's trajectory (same
 termination and iteration count) on both kernel paths; the device scaling
 and the device feasibility check equal the oracle's and the host's bit for
-bit."""
+bit; a pack of 2-6 drawn instances equals the single FP32 projector bit for
+bit on the one-workgroup path and reaches its distances on the general
+passes."""
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings
+from hypothesis import strategies as st
 
 from oracle import oracle as O
 from pdfp_amd import pdfp as P
 from pdfp_amd import pdlp as L
 from pdfp_amd.batch import PDFPBatch
+from pdfp_amd.pack import PDFPPack
 from pdfp_amd.milp import DeviceViolationCheck, max_violation
 from pdfp_amd.pdfp import SolverParameters, projector_problem
 from pdfp_amd.preconditioner import scale_problem
@@ -145,3 +149,41 @@ def test_scaling_and_check_equal_the_host_on_random_instances(spec):
     for _ in range(3):
         x = rng.random(len(milp.c)) * milp.u
         assert chk(x) == max_violation(milp, x)
+
+
+@settings(max_examples=15, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow])
+@given(specs=st.lists(instances, min_size=2, max_size=6))
+def test_pack_equals_single_projectors_on_random_instances(specs):
+    milps = [tiny_milp(*sp_) for sp_ in specs]
+    pps = [projector_problem(m) for m in milps]
+    print("pack property", specs, flush=True)
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                         iteration_limit=200_000)
+    targets = [integer_target(m, sp_[-1] + 17) / pp.D_c for m, sp_, pp in zip(milps, specs, pps)]
+    ref = []
+    for pp, t in zip(pps, targets):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, t)
+        stt = P.pdfp_solve(s, p, 1.0)
+        ref.append((stt, s.total_count, s.pdhg_x.copy(), s.info().l1_distance,
+                    s.info().small_path))
+        s.close()
+    for path in ("auto", "general"):
+        pk = PDFPPack(pps)
+        try:
+            pk.set_kernel_path(path)
+            for k, t in enumerate(targets):
+                pk.set_target(k, t)
+            terms = pk.solve(p, 1.0)
+            for k, r in enumerate(ref):
+                if path == "auto" and r[4] == 1:
+                    assert terms[k] == r[0] and pk.total_count(k) == r[1], (k, terms[k], r[:2])
+                    assert np.array_equal(pk.pdhg_x(k), r[2]), k
+                elif r[0] == "OPTIMAL":
+                    assert terms[k] == "OPTIMAL", (k, path)
+                    l1 = pk.info(k).l1_distance
+                    assert abs(l1 - r[3]) <= 1e-3 * max(1.0, r[3]), (k, path, l1, r[3])
+        finally:
+            pk.close()
