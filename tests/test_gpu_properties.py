@@ -12,7 +12,9 @@ termination and iteration count) on both kernel paths; the device scaling
 and the device feasibility check equal the oracle's and the host's bit for
 bit; a pack of 2-6 drawn instances equals the single FP32 projector bit for
 bit on the one-workgroup path and reaches its distances on the general
-passes."""
+passes; the batch with its A pass forced over the column slices (every row,
+or the rows longer than 2; drawn instances have fewer columns than slices,
+empty slices, rows and columns) reaches the same distances."""
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings
@@ -187,3 +189,40 @@ def test_pack_equals_single_projectors_on_random_instances(specs):
                     assert abs(l1 - r[3]) <= 1e-3 * max(1.0, r[3]), (k, path, l1, r[3])
         finally:
             pk.close()
+
+
+@pytest.mark.parametrize("min_len", [0, 2])
+@settings(max_examples=15, deadline=None, derandomize=True,
+          suppress_health_check=[HealthCheck.too_slow])
+@given(spec=instances)
+def test_batch_column_slices_on_random_instances(min_len, spec):
+    milp = tiny_milp(*spec)
+    pp = projector_problem(milp)
+    print("batch slices property", min_len, spec, flush=True)
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                         iteration_limit=200_000)
+    targets = [integer_target(milp, spec[-1] + 13 * k) / pp.D_c for k in range(8)]
+    ref = []
+    for k in range(8):
+        s = P.PDFPState(pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w,
+                        precision="fp32")
+        P.set_target(s, targets[k])
+        assert P.pdfp_solve(s, p, 1.0) == "OPTIMAL"
+        ref.append(s.info().l1_distance)
+        s.close()
+    bt = PDFPBatch(8, pp.A, pp.b, pp.is_eq, pp.is_int, pp.var_lb, pp.var_ub, pp.w)
+    try:
+        bt.set_kernel_path("general")
+        if pp.A.nnz == 0:
+            return  # nothing to slice (the library refuses, as documented)
+        bt.set_column_slices(True)
+        bt.set_slice_min_len(min_len)
+        for k in range(8):
+            bt.set_target(k, targets[k])
+        assert bt.solve(p, 1.0) == ["OPTIMAL"] * 8
+        assert bt.info(0).column_slices > 0 and bt.info(0).slice_min_len == min_len
+        for k in range(8):
+            l1b = bt.info(k).l1_distance
+            assert abs(l1b - ref[k]) <= 1e-3 * max(1.0, ref[k]), (k, l1b, ref[k])
+    finally:
+        bt.close()
