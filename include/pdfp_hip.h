@@ -161,6 +161,14 @@ int pdfp_set_dual(pdfp_ctx* ctx, const double* y, const double* z);
 /* set_target!(s, x̃)  src/pdfp.jl:98-101 (copies x̃, then warm_start!). */
 int pdfp_set_target(pdfp_ctx* ctx, const double* x_tilde_scaled);
 
+/* The pump's set_target!(projector, x̃ ./ D_c)  src/pump.jl:331 with the
+ * division on the device: x̃ in original units, divided (IEEE, as the host
+ * would) by the column scaling given once to pdfp_set_column_scale (n
+ * positive finite values; PDFP_ERR_STATE before it).  Equal, bit for bit, to
+ * pdfp_set_target on the host's x̃ ./ D_c. */
+int pdfp_set_column_scale(pdfp_ctx* ctx, const double* D_c);
+int pdfp_set_target_original(pdfp_ctx* ctx, const double* x_tilde);
+
 /* warm_start!(s)  src/pdfp.jl:104-114. */
 int pdfp_warm_start(pdfp_ctx* ctx);
 
@@ -231,6 +239,10 @@ int pdfp_batch_set_original_primal_residual(pdfp_batch* ctx, const double* D_r);
  * set_target! (src/pdfp.jl:98-101) and warm_start! (src/pdfp.jl:104-114). */
 int pdfp_batch_set_x(pdfp_batch* ctx, int32_t k, const double* x_scaled);
 int pdfp_batch_set_target(pdfp_batch* ctx, int32_t k, const double* x_tilde_scaled);
+/* Trajectory k's set_target!(projector, x̃ ./ D_c)  src/pump.jl:331 with the
+ * division on the device, as pdfp_set_target_original. */
+int pdfp_batch_set_column_scale(pdfp_batch* ctx, const double* D_c);
+int pdfp_batch_set_target_original(pdfp_batch* ctx, int32_t k, const double* x_tilde);
 int pdfp_batch_warm_start(pdfp_batch* ctx, int32_t k);
 /* pdfp_solve! for every k with active[k] (all when active is NULL), with
  * primal_weight[k]; returns when each of them has terminated.  An inactive

@@ -104,6 +104,8 @@ SIGNATURES = {
     "pdfp_set_x": (C.c_int, [_P, _D]),
     "pdfp_set_dual": (C.c_int, [_P, _D, _D]),
     "pdfp_set_target": (C.c_int, [_P, _D]),
+    "pdfp_set_column_scale": (C.c_int, [_P, _D]),
+    "pdfp_set_target_original": (C.c_int, [_P, _D]),
     "pdfp_warm_start": (C.c_int, [_P]),
     "pdfp_solve": (C.c_int, [_P, C.POINTER(PdfpParams), C.c_double, C.POINTER(C.c_int32), _I64]),
     "pdfp_get_x": (C.c_int, [_P, _D]),
@@ -137,6 +139,8 @@ SIGNATURES = {
     "pdfp_batch_set_original_primal_residual": (C.c_int, [_P, _D]),
     "pdfp_batch_set_x": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_batch_set_target": (C.c_int, [_P, C.c_int32, _D]),
+    "pdfp_batch_set_column_scale": (C.c_int, [_P, _D]),
+    "pdfp_batch_set_target_original": (C.c_int, [_P, C.c_int32, _D]),
     "pdfp_batch_warm_start": (C.c_int, [_P, C.c_int32]),
     "pdfp_batch_solve": (C.c_int, [_P, C.POINTER(PdfpParams), _D, _U8, C.POINTER(C.c_int32),
                                    _I64]),

@@ -71,6 +71,22 @@ class PDFPBatch:
         self._check(self._lib.pdfp_batch_set_target(self._ctx, k, dptr(x)))
         self.termination_reason[k] = "UNSPECIFIED"
 
+    def set_column_scale(self, D_c) -> None:
+        """The column scaling D_c (n positive values) for set_target_original."""
+        d = f64(D_c)
+        if d.shape != (self.n,):
+            raise ValueError(f"D_c has shape {d.shape}, expected ({self.n},)")
+        self._check(self._lib.pdfp_batch_set_column_scale(self._ctx, dptr(d)))
+
+    def set_target_original(self, k: int, x_tilde) -> None:
+        """Trajectory k's set_target!(projector, x̃ ./ D_c) (src/pump.jl:331)
+        with x̃ in original units and the division on the device."""
+        x = f64(x_tilde)
+        if x.shape != (self.n,):
+            raise ValueError(f"x_tilde has shape {x.shape}, expected ({self.n},)")
+        self._check(self._lib.pdfp_batch_set_target_original(self._ctx, k, dptr(x)))
+        self.termination_reason[k] = "UNSPECIFIED"
+
     def warm_start(self, k: int) -> None:
         self._check(self._lib.pdfp_batch_warm_start(self._ctx, k))
         self.termination_reason[k] = "UNSPECIFIED"

@@ -82,6 +82,7 @@ struct Trajectory {
   pdfp_params p{};       // the parameters of its current solve
   double t_start = 0.0;
   bool l1_den_pending = false;  // summed on the host while the next block runs
+  bool xt_original = false;     // its h_xt holds x̃ unscaled (set_target_original)
 };
 
 // What the C ABI sees of a batch; Batch<KB> implements it for a batch width
@@ -99,6 +100,8 @@ struct BatchBase {
   virtual void set_original_primal_residual(const double* D_r) = 0;
   virtual void set_x(int k, const double* xs) = 0;
   virtual void set_target(int k, const double* xts) = 0;
+  virtual void set_column_scale(const double* dc) = 0;
+  virtual void set_target_original(int k, const double* xs) = 0;
   virtual void warm_start(int k) = 0;
   virtual void begin(int k, const pdfp_params& p, double pw) = 0;
   virtual int run(int mode) = 0;
@@ -131,6 +134,9 @@ struct Batch final : BatchBase {
   // input) and on the device (for the polish), n values per trajectory
   PinnedBuf<double> h_xt;
   DevBuf<double> xt64;
+  // D_c for targets given in original units (set_target_original)
+  DevBuf<double> dc;
+  std::vector<double> h_dc;
   DevBuf<float> tmpf;
   DevBuf<double> part_t, part_a, part_v, sums;
   DevBuf<BatchStep> d_sp;
@@ -347,8 +353,14 @@ struct Batch final : BatchBase {
     if (!t.l1_den_pending) return;
     const double* h = h_xt.p + (size_t)k * (size_t)n;
     double acc = 0.0;
-    for (int64_t j = 0; j < n; ++j)
-      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h[(size_t)j]) : 0.0;
+    if (t.xt_original) {
+      for (int64_t j = 0; j < n; ++j)
+        acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h[(size_t)j] / h_dc[(size_t)j])
+                                    : 0.0;
+    } else {
+      for (int64_t j = 0; j < n; ++j)
+        acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h[(size_t)j]) : 0.0;
+    }
     t.l1_den = 1.0 + acc;
     t.l1_den_pending = false;
   }
@@ -366,6 +378,39 @@ struct Batch final : BatchBase {
       k_batch_scatter64<KB><<<grid_for(n), 256, 0, stream>>>((int)n, k, d, xt.p);
       PDFP_HIP(hipGetLastError());
     }
+    tr[(size_t)k].xt_original = false;
+    if (x64_owner == k) x64_owner = -1;
+    warm_start(k);
+  }
+
+  // The column scaling D_c, kept on the device for set_target_original.
+  void set_column_scale(const double* dcs) override {
+    for (int64_t j = 0; j < n; ++j)
+      if (!(dcs[j] > 0.0) || !std::isfinite(dcs[j]))
+        throw Error(PDFP_ERR_ARGUMENT, "column scaling must be positive and finite");
+    sync();
+    h_dc.assign(dcs, dcs + n);
+    if (!dc.p) dc.alloc((size_t)std::max<int64_t>(n, 1));
+    if (n) dc.upload(h_dc.data(), (size_t)n, stream);
+    sync();
+  }
+
+  // set_target!(s, x̃ ./ D_c)  src/pump.jl:331 with the division on the
+  // device: the copy and upload of set_target, then x̃ ./ D_c into the FP64
+  // target and the interleaved FP32 one in one kernel.
+  void set_target_original(int k, const double* xs) override {
+    check_k(k);
+    if (!dc.p) throw Error(PDFP_ERR_STATE, "no column scaling (pdfp_batch_set_column_scale)");
+    sync();  // no copy of an earlier target may still read h_xt
+    if (n) {
+      double* h = h_xt.p + (size_t)k * (size_t)n;
+      double* d = xt64.p + (size_t)k * (size_t)n;
+      std::memcpy(h, xs, sizeof(double) * (size_t)n);
+      PDFP_HIP(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, stream));
+      k_batch_unscale64<KB><<<grid_for(n), 256, 0, stream>>>((int)n, k, d, dc.p, xt.p);
+      PDFP_HIP(hipGetLastError());
+    }
+    tr[(size_t)k].xt_original = true;
     if (x64_owner == k) x64_owner = -1;
     warm_start(k);
   }
@@ -1511,6 +1556,20 @@ int pdfp_batch_set_target(pdfp_batch* ctx, int32_t k, const double* xt) {
   return batch_guarded(ctx, [&] {
     if (!xt) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null target");
     ctx->impl->set_target(k, xt);
+  });
+}
+
+int pdfp_batch_set_column_scale(pdfp_batch* ctx, const double* D_c) {
+  return batch_guarded(ctx, [&] {
+    if (!D_c) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null column scaling");
+    ctx->impl->set_column_scale(D_c);
+  });
+}
+
+int pdfp_batch_set_target_original(pdfp_batch* ctx, int32_t k, const double* xt) {
+  return batch_guarded(ctx, [&] {
+    if (!xt) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null target");
+    ctx->impl->set_target_original(k, xt);
   });
 }
 

@@ -545,6 +545,23 @@ __global__ __launch_bounds__(256) void k_batch_scatter64(int len, int k,
   if (i < len) out[(size_t)i * kBatch + k] = (float)in[i];
 }
 
+// Trajectory k's target in original units to scaled ones (x̃ ./ D_c,
+// src/pump.jl:331): io holds x̃ on entry and x̃ ./ D_c (IEEE division, as the
+// host's) on exit; out (interleaved) receives it in FP32.
+template <int KB>
+__global__ __launch_bounds__(256) void k_batch_unscale64(int len, int k,
+                                                         double* __restrict__ io,
+                                                         const double* __restrict__ dc,
+                                                         float* __restrict__ out) {
+  constexpr int kBatch = KB;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) {
+    const double v = io[i] / dc[i];
+    io[i] = v;
+    out[(size_t)i * kBatch + k] = (float)v;
+  }
+}
+
 // Reads column k of an interleaved vector into a dense FP32 vector.
 template <int KB>
 __global__ __launch_bounds__(256) void k_batch_gather(int len, int k,

@@ -35,6 +35,8 @@ struct ProjectorBase {
   virtual void set_x(const double* x) = 0;
   virtual void set_dual(const double* y, const double* z) = 0;
   virtual void set_target(const double* xt) = 0;
+  virtual void set_column_scale(const double* dc) = 0;
+  virtual void set_target_original(const double* xt) = 0;
   virtual void warm_start() = 0;
   virtual int solve(const pdfp_params& p, double pw) = 0;
   virtual void get_x(double* out) = 0;
@@ -76,6 +78,11 @@ struct Projector final : ProjectorBase {
   // block of a solve runs on the device
   PinnedBuf<double> h_xt;
   bool l1_den_pending = false;
+  // D_c for targets given in original units (set_target_original): h_xt
+  // then holds x̃ unscaled and the denominator divides as the host would
+  DevBuf<double> dc;
+  std::vector<double> h_dc;
+  bool xt_original = false;
 
   std::vector<double> h_w, h_b;  // w (0 off I) and b in FP64
   bool orig_primal = false;
@@ -300,7 +307,7 @@ struct Projector final : ProjectorBase {
                                &z, &z0, &pz, &rz})
       s += v->bytes();
     for (const DevBuf<double>* v : {&row_unscale, &atz64, &ax64, &lb64, &ub64, &xt64, &w64, &x64,
-                                    &part_t, &part_a, &part_v, &sums})
+                                    &dc, &part_t, &part_a, &part_v, &sums})
       s += v->bytes();
     return (int64_t)(s + is_eq.bytes());
   }
@@ -366,6 +373,37 @@ struct Projector final : ProjectorBase {
                                 hipMemcpyHostToDevice, stream));
       }
     }
+    xt_original = false;
+    warm_start();
+  }
+
+  // The column scaling D_c, kept on the device for set_target_original.
+  void set_column_scale(const double* dcs) override {
+    for (int64_t j = 0; j < n; ++j)
+      if (!(dcs[j] > 0.0) || !std::isfinite(dcs[j]))
+        throw Error(PDFP_ERR_ARGUMENT, "column scaling must be positive and finite");
+    sync();
+    h_dc.assign(dcs, dcs + n);
+    if (!dc.p) dc.alloc((size_t)std::max<int64_t>(n, 1));
+    if (!xt64.p) xt64.alloc((size_t)std::max<int64_t>(n, 1));  // the FP64 form's staging
+    if (n) dc.upload(h_dc.data(), (size_t)n, stream);
+    sync();
+  }
+
+  // set_target!(s, x̃ ./ D_c)  src/pump.jl:331 with the division on the
+  // device: the same copy and upload as set_target, then x̃ ./ D_c into the
+  // FP64 target and the iterates' precision in one kernel.
+  void set_target_original(const double* xs) override {
+    if (!dc.p) throw Error(PDFP_ERR_STATE, "no column scaling (pdfp_set_column_scale)");
+    sync();  // no copy of an earlier target may still read h_xt
+    if (n) {
+      std::memcpy(h_xt.p, xs, sizeof(double) * (size_t)n);
+      PDFP_HIP(hipMemcpyAsync(xt64.p, h_xt.p, sizeof(double) * (size_t)n,
+                              hipMemcpyHostToDevice, stream));
+      k_unscale_target<T><<<grid_for(n), 256, 0, stream>>>((int)n, xt64.p, dc.p, xt.p);
+      PDFP_HIP(hipGetLastError());
+    }
+    xt_original = true;
     warm_start();
   }
 
@@ -576,8 +614,10 @@ struct Projector final : ProjectorBase {
   void finish_l1_den() {
     if (!l1_den_pending) return;
     double acc = 0.0;
-    for (int64_t j = 0; j < n; ++j)
-      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(h_xt.p[(size_t)j]) : 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      const double v = xt_original ? h_xt.p[(size_t)j] / h_dc[(size_t)j] : h_xt.p[(size_t)j];
+      acc += h_w[(size_t)j] > 0.0 ? h_w[(size_t)j] * std::fabs(v) : 0.0;
+    }
     l1_den = 1.0 + acc;
     l1_den_pending = false;
   }
@@ -979,6 +1019,21 @@ int pdfp_set_target(pdfp_ctx* ctx, const double* xt) {
   return guarded(ctx, [&] {
     if (!xt) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null target");
     ctx->impl->set_target(xt);
+    ctx->target_set = true;
+  });
+}
+
+int pdfp_set_column_scale(pdfp_ctx* ctx, const double* D_c) {
+  return guarded(ctx, [&] {
+    if (!D_c) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null column scaling");
+    ctx->impl->set_column_scale(D_c);
+  });
+}
+
+int pdfp_set_target_original(pdfp_ctx* ctx, const double* xt) {
+  return guarded(ctx, [&] {
+    if (!xt) throw pdfp::Error(PDFP_ERR_ARGUMENT, "null target");
+    ctx->impl->set_target_original(xt);
     ctx->target_set = true;
   });
 }

@@ -356,6 +356,21 @@ __global__ __launch_bounds__(256) void k_narrow(int n, const double* __restrict_
   if (i < n) out[i] = (T)in[i];
 }
 
+// A target in original units to scaled ones (x̃ ./ D_c, src/pump.jl:331):
+// io holds x̃ on entry and x̃ ./ D_c (IEEE division, as the host's) on exit;
+// out receives it in the iterates' precision.
+template <typename T>
+__global__ __launch_bounds__(256) void k_unscale_target(int n, double* __restrict__ io,
+                                                        const double* __restrict__ dc,
+                                                        T* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double v = io[i] / dc[i];
+    io[i] = v;
+    out[i] = (T)v;
+  }
+}
+
 // FP64 polish of an FP32 projection: a coordinate that the primal prox
 // clamped to a bound gets that bound's FP64 value (x = D_c·clamp(x̂, l/D_c,
 // u/D_c) then holds exactly as in FP64); interior coordinates are widened.

@@ -241,6 +241,27 @@ def set_target(s: PDFPState, x_tilde_scaled) -> PDFPState:
     return s
 
 
+def set_column_scale(s: PDFPState, D_c) -> PDFPState:
+    """The column scaling D_c (n positive values) for set_target_original."""
+    d = f64(D_c)
+    if d.shape != (s.n,):
+        raise ValueError(f"D_c has shape {d.shape}, expected ({s.n},)")
+    s._check(s._lib.pdfp_set_column_scale(s._ctx, dptr(d)))
+    return s
+
+
+def set_target_original(s: PDFPState, x_tilde) -> PDFPState:
+    """The pump's ``set_target!(projector, x̃ ./ D_c)`` (src/pump.jl:331) with
+    x̃ in original units and the division on the device (after
+    set_column_scale); equal, bit for bit, to set_target(s, x̃ / D_c)."""
+    x = f64(x_tilde)
+    if x.shape != (s.n,):
+        raise ValueError(f"x_tilde has shape {x.shape}, expected ({s.n},)")
+    s._check(s._lib.pdfp_set_target_original(s._ctx, dptr(x)))
+    s.termination_reason = "UNSPECIFIED"
+    return s
+
+
 def warm_start(s: PDFPState) -> PDFPState:
     """src/pdfp.jl:104-114."""
     s._check(s._lib.pdfp_warm_start(s._ctx))

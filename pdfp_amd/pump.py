@@ -44,7 +44,24 @@ def _fp64_successor(P, pp, mixed: bool, device: int):
         st.set_kernel_path("one-workgroup")
     if mixed:
         P.original_primal_residual(st, pp.D_r)
+    _column_scale(P, st, pp)
     return st
+
+
+def _column_scale(P, st, pp) -> None:
+    """Hand the projector D_c where it divides targets on the device."""
+    if hasattr(P, "set_column_scale"):
+        P.set_column_scale(st, pp.D_c)
+
+
+def _put_target(P, st, pp, xt) -> None:
+    """set_target!(projector, x̃ ./ D_c)  (src/pump.jl:331): x̃ in original
+    units, divided on the device where the projector does it (bit for bit the
+    host's division), on the host otherwise."""
+    if hasattr(P, "set_target_original"):
+        P.set_target_original(st, xt)
+    else:
+        P.set_target(st, xt / pp.D_c)
 
 
 @dataclass
@@ -293,6 +310,7 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                          precision=precision, device=device)
         if mixed:
             P.original_primal_residual(st, pp.D_r)
+        _column_scale(P, st, pp)
         st.set_x(x / pp.D_c)
         proj.update(pp=pp, state=st)
 
@@ -300,7 +318,9 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
         if not proj:
             build()
         pp = proj["pp"]
-        target_scaled = None if kind == "tighten" else xt / pp.D_c
+        # x̃ in original units; the trajectory leaves it unchanged until its
+        # next projection, which brings a new one
+        target = None if kind == "tighten" else xt
         # FP32 iterates do not resolve relative KKT errors much below 1e-5:
         # from the first projection asked for less, the FP64 form takes over,
         # started at the last FP32 projection.
@@ -310,16 +330,16 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                     params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR):
             st64 = _fp64_successor(P, pp, mixed, device)
             st64.set_x(st.pdhg_x)
-            P.set_target(st64, proj["target"] if target_scaled is None else target_scaled)
+            _put_target(P, st64, pp, proj["target"] if target is None else target)
             emit(f"projector   FP64 form from gap tolerance "
                  f"{params.eps_optimal_relative:.2e} on\n")
             proj["state"] = st = st64
-        elif target_scaled is not None:
-            P.set_target(st, target_scaled)
+        elif target is not None:
+            _put_target(P, st, pp, target)
         else:
             P.warm_start(st)
-        if target_scaled is not None:
-            proj["target"] = target_scaled
+        if target is not None:
+            proj["target"] = target
         P.pdfp_solve(st, params, primal_weight)
         if polish and getattr(st, "precision", None) == "fp32":
             # the FP64 finish: started at the FP32 point (current and initial
@@ -330,7 +350,7 @@ def feasibility_pump(milp: MILP, *, time_limit: float = 1800.0, seed: int = 1,
                 proj["polish"] = st64
             st64.set_x(st.pdhg_x)
             st64.set_dual(st.vector("pdhg_y"), st.vector("pdhg_z"))
-            P.set_target(st64, proj["target"])
+            _put_target(P, st64, pp, proj["target"])
             P.pdfp_solve(st64, params, primal_weight)
             return pp.D_c * st64.pdhg_x, st.total_count + st64.total_count
         return pp.D_c * st.pdhg_x, st.total_count
@@ -623,12 +643,13 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
                       device=device)
     if mixed:
         batch.original_primal_residual(pp.D_r)
+    batch.set_column_scale(D_c)
     for k in range(K):
         batch.set_x(k, x / D_c)
     m = num_constraints(milp)
     primal_weight = math.sqrt(1.0 + (len(milp.int_idx) / m if m > 0 else 0.0))
     fp64: list = [None] * K        # a trajectory's own FP64 projector, once it needs one
-    target: list = [None] * K      # its last target (scaled)
+    target: list = [None] * K      # its last target (original units)
     fin64: list = [None] * K       # its FP64 context for the polish of its FP32 projections
 
     def polished(k, x_s, y, z, count, params):
@@ -640,7 +661,7 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
             fin64[k] = st
         st.set_x(x_s)
         st.set_dual(y, z)
-        _pdfp_mod.set_target(st, target[k])
+        _put_target(_pdfp_mod, st, pp, target[k])
         _pdfp_mod.pdfp_solve(st, params, primal_weight)
         return advance(k, (D_c * st.pdhg_x, count + st.total_count))
 
@@ -682,26 +703,26 @@ def feasibility_pump_batch(milp: MILP, *, trajectories: int = 8, seed: int = 1,
         for k in sorted(pending):
             kind, xt, params = pending[k]
             if kind == "target":
-                target[k] = xt / D_c
+                target[k] = xt  # unchanged by its trajectory until the next projection
             tight = min(params.eps_optimal_relative,
                         params.eps_feasible_relative) < FP32_TOLERANCE_FLOOR
             if fp64[k] is None and tight:
                 st64 = _fp64_successor(_pdfp_mod, pp, mixed, device)
                 st64.set_x(batch.pdhg_x(k))
-                _pdfp_mod.set_target(st64, target[k])
+                _put_target(_pdfp_mod, st64, pp, target[k])
                 fp64[k] = st64
                 kind = "started"
             if fp64[k] is not None:
                 st64 = fp64[k]
                 if kind == "target":
-                    _pdfp_mod.set_target(st64, target[k])
+                    _put_target(_pdfp_mod, st64, pp, target[k])
                 elif kind == "tighten":
                     _pdfp_mod.warm_start(st64)
                 _pdfp_mod.pdfp_solve(st64, params, primal_weight)
                 futures[k] = pool.submit(advance, k, (D_c * st64.pdhg_x, st64.total_count))
             else:
                 if kind == "target":
-                    batch.set_target(k, target[k])
+                    batch.set_target_original(k, target[k])
                 else:
                     batch.warm_start(k)
                 batch.begin(k, params, primal_weight)

@@ -563,3 +563,40 @@ def test_batch_general_integer_equality_heavy():
             l1o = o.info().l1_distance
             den = 1.0 + float(np.sum(np.asarray(pp.w)[ii] * np.abs(targets[k][ii])))
             assert abs(r[3] - l1o) <= 2 * 10 * tol * den, (k, r[3], l1o)
+
+
+@pytest.mark.parametrize("name,maker,path", [
+    ("p0201", lambda: read_mps(example_path("p0201")), "auto"),
+    ("mixed_2000x4000", lambda: G.mixed(2000, 4000, 1), "general")])
+def test_batch_target_in_original_units_equals_host_division(name, maker, path):
+    """Trajectories 0-3 take set_target on the host's x̃ ./ D_c, trajectories
+    4-7 set_target_original on the same x̃ (the division on the device, the
+    pump's src/pump.jl:331): each pair ends alike bit for bit (termination,
+    iteration count, ℓ1 gap, stored target, point)."""
+    print("batch target original units", name, path, flush=True)
+    m = maker()
+    pp = projector_problem(m)
+    n = m.A.shape[1]
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                         iteration_limit=100_000)
+    bt = _batch(pp, 8)
+    try:
+        bt.set_kernel_path(path)
+        bt.set_column_scale(pp.D_c)
+        for rnd in range(2):  # a cold start, then warm-started projections
+            targets = [G.binary_targets(m, 40 + 10 * rnd + k) for k in range(4)]
+            for k in range(4):
+                if rnd == 0:
+                    bt.set_x(k, np.zeros(n))
+                    bt.set_x(k + 4, np.zeros(n))
+                bt.set_target(k, targets[k] / pp.D_c)
+                bt.set_target_original(k + 4, targets[k])
+            assert bt.solve(p, 1.0) == ["OPTIMAL"] * 8
+            for k in range(4):
+                a, b = bt.info(k), bt.info(k + 4)
+                assert (a.total_count, a.relative_l1_gap, a.l1_distance) == \
+                    (b.total_count, b.relative_l1_gap, b.l1_distance), (rnd, k)
+                assert np.array_equal(bt.vector(k, "x_tilde"), bt.vector(k + 4, "x_tilde"))
+                assert np.array_equal(bt.pdhg_x(k), bt.pdhg_x(k + 4)), (rnd, k)
+    finally:
+        bt.close()

@@ -21,9 +21,11 @@ import pytest
 from oracle import oracle as O
 from pdfp_amd import generators as G
 from pdfp_amd import pdfp as P
-from pdfp_amd.milp import max_violation
+from pdfp_amd._lib import LibraryError
+from pdfp_amd.milp import max_violation, read_mps
 from pdfp_amd.pdfp import SolverParameters, projector_problem
 
+from .conftest import example_path
 from .helpers import build_instance, l1_distance, l1_fixtures, project_with
 
 pytestmark = pytest.mark.gpu
@@ -316,3 +318,69 @@ def test_block_graph_replays_equal_stream_launches(name, maker):
         for a, b in zip(runs[0], seq):
             assert a[:3] == b[:3]
             assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("name,maker,path", [
+    ("p0201", lambda: read_mps(example_path("p0201")), "auto"),
+    ("mixed_2000x4000", lambda: G.mixed(2000, 4000, 1), "general")])
+def test_target_in_original_units_equals_host_division(precision, name, maker, path):
+    """set_target_original (x̃ in original units, x̃ ./ D_c on the device, the
+    pump's src/pump.jl:331) gives the projections of set_target on the
+    host's x̃ ./ D_c bit for bit: the stored target, the iteration counts,
+    the ℓ1 gap (whose denominator is summed from the target), the points;
+    a warm-started sequence, each form on a fresh state with the same
+    schedule."""
+    say("target original units", precision, name, path)
+    m = maker()
+    pp = projector_problem(m)
+    n = m.A.shape[1]
+    targets = [G.binary_targets(m, 30 + k) for k in range(3)]
+    p = SolverParameters(eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                         iteration_limit=100_000)
+    runs = []
+    for device_division in (False, True):
+        s = hip_state(pp, precision)
+        s.set_kernel_path(path)
+        s.set_schedule(512, 512, 0, 0)
+        s.set_x(np.zeros(n))
+        if device_division:
+            P.set_column_scale(s, pp.D_c)
+        seq = []
+        for xt in targets:
+            if device_division:
+                P.set_target_original(s, xt)
+            else:
+                P.set_target(s, xt / pp.D_c)
+            st = P.pdfp_solve(s, p, 1.0)
+            info = s.info()
+            seq.append((st, info.total_count, info.relative_l1_gap, info.l1_distance,
+                        s.vector("x_tilde"), s.pdhg_x.copy()))
+        s.close()
+        runs.append(seq)
+    for a, b in zip(*runs):
+        assert a[0] == "OPTIMAL" and a[:4] == b[:4], (a[:4], b[:4])
+        assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
+
+
+def test_target_in_original_units_errors():
+    """No column scaling yet: a state error; a zero or short scaling: refused."""
+    m = G.set_cover(133, 201, 1)
+    pp = projector_problem(m)
+    s = hip_state(pp, "fp32")
+    try:
+        xt = G.binary_targets(m, 1)
+        with pytest.raises(LibraryError, match="column scaling"):
+            P.set_target_original(s, xt)
+        bad = pp.D_c.copy()
+        bad[3] = 0.0
+        with pytest.raises(LibraryError, match="positive"):
+            P.set_column_scale(s, bad)
+        with pytest.raises(ValueError):
+            P.set_column_scale(s, pp.D_c[:-1])
+        P.set_column_scale(s, pp.D_c)
+        P.set_target_original(s, xt)
+        assert np.array_equal(s.vector("x_tilde"),
+                              (xt / pp.D_c).astype(np.float32).astype(np.float64))
+    finally:
+        s.close()
